@@ -1,0 +1,212 @@
+// Negacyclic NTT of size N = 2048 over Goldilocks, as per-thread pass functions for a
+// 256-thread workgroup (4 waves of 64).  The transform is a 32 x 64 four-step split:
+//
+//   p[64m + r], m < 32, r < 64
+//   A  (per column r): 32-point negacyclic NTT over m, root psi = 8 = 2^3
+//        A1: pre-twist psi^m, DFT-8 over m1 (m = m0 + 4 m1), twist Omega^(k1 m0)
+//        A2: DFT-4 over m0, k = k1 + 8 k0
+//   B  twist by zeta^((2k+1) r), zeta the primitive 4096-th root of unity (table)
+//   C  (per row k): 64-point cyclic DFT over r, root omega = 8
+//        C1: DFT-8 over r1 (r = r0 + 8 r1), twist omega^(w1 r0)
+//        C2: DFT-8 over r0, w = w1 + 8 w0
+//   NTT-domain index 64k + w holds P(zeta^(2(k + 32w) + 1)).
+//
+// That output order is exactly the order of the reference model's ntt() with cut
+// [5,6] (sw/ntt/ntt_gf64.sage:367-445: negacyclic radix-32 column, network, ngc phi,
+// cyclic radix-64 column), i.e. the HPU's n5c6 cut, so NTT-domain operands given in
+// the reference order need no permutation.  All twiddles except B are powers of two
+// (omega_64 = 8, sw/ntt/ntt_gf64.sage:35-38) applied as shifts; B is the only general
+// modular multiplication.  The inverse mirrors the passes with inverse roots and
+// folds 1/N into the B twist table.
+//
+// Thread mappings (t = thread id in 0..255):
+//   A map   : r = t & 63, m0 = t >> 6   owns coefficients n = 64 (m0 + 4 m1) + r, m1 < 8
+//   A2 map  : r = t & 63, q = t >> 6    owns k1 in {2q, 2q+1}, all k0 < 4
+//   C map   : k = t >> 3, j = t & 7     (j = r0 in C1, w1 in C2)
+//   NTT domain: thread (k, w1) owns indices 64 k + w1 + 8 w0, w0 < 8.
+//
+// LDS layouts per polynomial (2048 u64), XOR-swizzled to be bank-conflict free for
+// 64-bit accesses with 32 or 64 banks:
+//   Y1(k1, m0, r) = (4 k1 + m0) * 64 + r
+//   T (k, r)      = 64 k + (r ^ 8 (k & 7))
+//   U (k, w1, r0) = 64 k + 8 (w1 ^ (k & 7)) + (r0 ^ w1)
+//
+// Each pass is split into load / compute / store so that a kernel can put a barrier
+// between all loads and all stores of the same buffer (in-place LDS reuse), and a
+// sequential CPU emulation of the 256 threads executes the identical code.
+#pragma once
+#include "gl64.h"
+
+namespace ntt {
+
+constexpr int N = 2048;
+constexpr int T = 256;
+
+GL_HD int idx_y1(int k1, int m0, int r) { return (k1 * 4 + m0) * 64 + r; }
+GL_HD int idx_t(int k, int r) { return k * 64 + (r ^ ((k & 7) << 3)); }
+GL_HD int idx_u(int k, int w1, int r0) { return k * 64 + 8 * (w1 ^ (k & 7)) + (r0 ^ w1); }
+
+// DFT-8 in natural order with root 2^S1 (S1 = 24 forward, 168 = -24 inverse).
+template <int S1>
+GL_HD void dft8(uint64_t* x) {
+  uint64_t a, b;
+  // stage 1: pairs (j, j+4), twiddle root^j
+  a = x[0]; b = x[4]; x[0] = gl::add(a, b); x[4] = gl::sub(a, b);
+  a = x[1]; b = x[5]; x[1] = gl::add(a, b); x[5] = gl::mul_pow2c<S1>(gl::sub(a, b));
+  a = x[2]; b = x[6]; x[2] = gl::add(a, b); x[6] = gl::mul_pow2c<2 * S1>(gl::sub(a, b));
+  a = x[3]; b = x[7]; x[3] = gl::add(a, b); x[7] = gl::mul_pow2c<3 * S1>(gl::sub(a, b));
+  // stage 2: pairs (h+j, h+j+2), twiddle root^(2j)
+  a = x[0]; b = x[2]; x[0] = gl::add(a, b); x[2] = gl::sub(a, b);
+  a = x[1]; b = x[3]; x[1] = gl::add(a, b); x[3] = gl::mul_pow2c<2 * S1>(gl::sub(a, b));
+  a = x[4]; b = x[6]; x[4] = gl::add(a, b); x[6] = gl::sub(a, b);
+  a = x[5]; b = x[7]; x[5] = gl::add(a, b); x[7] = gl::mul_pow2c<2 * S1>(gl::sub(a, b));
+  // stage 3
+  a = x[0]; b = x[1]; x[0] = gl::add(a, b); x[1] = gl::sub(a, b);
+  a = x[2]; b = x[3]; x[2] = gl::add(a, b); x[3] = gl::sub(a, b);
+  a = x[4]; b = x[5]; x[4] = gl::add(a, b); x[5] = gl::sub(a, b);
+  a = x[6]; b = x[7]; x[6] = gl::add(a, b); x[7] = gl::sub(a, b);
+  // bit-reversed -> natural
+  a = x[1]; x[1] = x[4]; x[4] = a;
+  a = x[3]; x[3] = x[6]; x[6] = a;
+}
+
+// DFT-4 in natural order with root 2^S2 (S2 = 48 forward, 144 = -48 inverse).
+template <int S2>
+GL_HD void dft4(uint64_t* x) {
+  uint64_t a0 = gl::add(x[0], x[2]), a1 = gl::sub(x[0], x[2]);
+  uint64_t b0 = gl::add(x[1], x[3]), b1 = gl::mul_pow2c<S2>(gl::sub(x[1], x[3]));
+  x[0] = gl::add(a0, b0);
+  x[2] = gl::sub(a0, b0);
+  x[1] = gl::add(a1, b1);
+  x[3] = gl::sub(a1, b1);
+}
+
+// ---------------------------------------------------------------- forward passes
+// A1: in regs v[m1] = p[64 (m0 + 4 m1) + r]
+GL_HD void fwd_a1_compute(uint64_t* v, int m0) {
+#pragma unroll
+  for (int m1 = 0; m1 < 8; m1++) v[m1] = gl::mul_pow2_rt(v[m1], 3 * (m0 + 4 * m1));
+  dft8<24>(v);
+#pragma unroll
+  for (int k1 = 1; k1 < 8; k1++) v[k1] = gl::mul_pow2_rt(v[k1], 6 * k1 * m0);
+}
+GL_HD void fwd_a1_store(const uint64_t* v, uint64_t* lds, int r, int m0) {
+#pragma unroll
+  for (int k1 = 0; k1 < 8; k1++) lds[idx_y1(k1, m0, r)] = v[k1];
+}
+// A2
+GL_HD void fwd_a2_load(uint64_t* v, const uint64_t* lds, int r, int q) {
+#pragma unroll
+  for (int h = 0; h < 2; h++)
+#pragma unroll
+    for (int m0 = 0; m0 < 4; m0++) v[h * 4 + m0] = lds[idx_y1(2 * q + h, m0, r)];
+}
+GL_HD void fwd_a2_compute(uint64_t* v, int r, int q, const uint64_t* tw) {
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    dft4<48>(v + h * 4);
+#pragma unroll
+    for (int k0 = 0; k0 < 4; k0++) {
+      int k = 2 * q + h + 8 * k0;
+      v[h * 4 + k0] = gl::mul(v[h * 4 + k0], tw[k * 64 + r]);
+    }
+  }
+}
+GL_HD void fwd_a2_store(const uint64_t* v, uint64_t* lds, int r, int q) {
+#pragma unroll
+  for (int h = 0; h < 2; h++)
+#pragma unroll
+    for (int k0 = 0; k0 < 4; k0++) lds[idx_t(2 * q + h + 8 * k0, r)] = v[h * 4 + k0];
+}
+// C1
+GL_HD void fwd_c1_load(uint64_t* v, const uint64_t* lds, int k, int r0) {
+#pragma unroll
+  for (int r1 = 0; r1 < 8; r1++) v[r1] = lds[idx_t(k, r0 + 8 * r1)];
+}
+GL_HD void fwd_c1_compute(uint64_t* v, int r0) {
+  dft8<24>(v);
+#pragma unroll
+  for (int w1 = 1; w1 < 8; w1++) v[w1] = gl::mul_pow2_rt(v[w1], 3 * w1 * r0);
+}
+GL_HD void fwd_c1_store(const uint64_t* v, uint64_t* lds, int k, int r0) {
+#pragma unroll
+  for (int w1 = 0; w1 < 8; w1++) lds[idx_u(k, w1, r0)] = v[w1];
+}
+// C2
+GL_HD void fwd_c2_load(uint64_t* v, const uint64_t* lds, int k, int w1) {
+#pragma unroll
+  for (int r0 = 0; r0 < 8; r0++) v[r0] = lds[idx_u(k, w1, r0)];
+}
+GL_HD void fwd_c2_compute(uint64_t* v) { dft8<24>(v); }  // v[w0] = Z[64k + w1 + 8 w0]
+
+// ---------------------------------------------------------------- inverse passes
+GL_HD void inv_c2_compute(uint64_t* v, int w1) {
+  dft8<168>(v);
+#pragma unroll
+  for (int r0 = 1; r0 < 8; r0++) v[r0] = gl::mul_pow2_rt(v[r0], (192 - 3 * w1 * r0) % 192);
+}
+GL_HD void inv_c2_store(const uint64_t* v, uint64_t* lds, int k, int w1) {
+#pragma unroll
+  for (int r0 = 0; r0 < 8; r0++) lds[idx_u(k, w1, r0)] = v[r0];
+}
+GL_HD void inv_c1_load(uint64_t* v, const uint64_t* lds, int k, int r0) {
+#pragma unroll
+  for (int w1 = 0; w1 < 8; w1++) v[w1] = lds[idx_u(k, w1, r0)];
+}
+GL_HD void inv_c1_compute(uint64_t* v, int k, int r0, const uint64_t* itw) {
+  dft8<168>(v);
+#pragma unroll
+  for (int r1 = 0; r1 < 8; r1++) v[r1] = gl::mul(v[r1], itw[k * 64 + r0 + 8 * r1]);
+}
+GL_HD void inv_c1_store(const uint64_t* v, uint64_t* lds, int k, int r0) {
+#pragma unroll
+  for (int r1 = 0; r1 < 8; r1++) lds[idx_t(k, r0 + 8 * r1)] = v[r1];
+}
+GL_HD void inv_a2_load(uint64_t* v, const uint64_t* lds, int r, int q) {
+#pragma unroll
+  for (int h = 0; h < 2; h++)
+#pragma unroll
+    for (int k0 = 0; k0 < 4; k0++) v[h * 4 + k0] = lds[idx_t(2 * q + h + 8 * k0, r)];
+}
+GL_HD void inv_a2_compute(uint64_t* v, int q) {
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    dft4<144>(v + h * 4);
+    int k1 = 2 * q + h;
+#pragma unroll
+    for (int m0 = 1; m0 < 4; m0++) v[h * 4 + m0] = gl::mul_pow2_rt(v[h * 4 + m0], (192 - 6 * k1 * m0) % 192);
+  }
+}
+GL_HD void inv_a2_store(const uint64_t* v, uint64_t* lds, int r, int q) {
+#pragma unroll
+  for (int h = 0; h < 2; h++)
+#pragma unroll
+    for (int m0 = 0; m0 < 4; m0++) lds[idx_y1(2 * q + h, m0, r)] = v[h * 4 + m0];
+}
+GL_HD void inv_a1_load(uint64_t* v, const uint64_t* lds, int r, int m0) {
+#pragma unroll
+  for (int k1 = 0; k1 < 8; k1++) v[k1] = lds[idx_y1(k1, m0, r)];
+}
+// Output v[m1] = p[64 (m0 + 4 m1) + r], canonical.
+GL_HD void inv_a1_compute(uint64_t* v, int m0) {
+  dft8<168>(v);
+#pragma unroll
+  for (int m1 = 0; m1 < 8; m1++) v[m1] = gl::canonical(gl::mul_pow2_rt(v[m1], (192 - 3 * (m0 + 4 * m1)) % 192));
+}
+
+// Twist tables (host side): tw[k*64 + r] = zeta^((2k+1) r), itw = zeta^(-(2k+1) r) / 2048,
+// zeta = OMG_2_32^(2^20) (sw/ntt/ntt_gf64.sage:44,69-75 power_of_rou(4096, 1)).
+inline void make_twist_tables(uint64_t* tw, uint64_t* itw) {
+  const uint64_t OMG_2_32 = 16334397945464290598ull;
+  uint64_t zeta = gl::pow(OMG_2_32, 1ull << 20);
+  uint64_t izeta = gl::inv(zeta);
+  uint64_t inv_n = gl::inv(2048);
+  for (int k = 0; k < 32; k++)
+    for (int r = 0; r < 64; r++) {
+      uint64_t e = (uint64_t)(2 * k + 1) * (uint64_t)r;
+      tw[k * 64 + r] = gl::pow(zeta, e);
+      itw[k * 64 + r] = gl::canonical(gl::mul(gl::pow(izeta, e), inv_n));
+    }
+}
+
+}  // namespace ntt
