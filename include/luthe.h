@@ -1,0 +1,232 @@
+/*
+ * luthe.h -- C ABI of the MI355X TFHE bootstrapping engine (libluthe.so).
+ *
+ * The engine takes the place of the HPU's PBS/ALU/memory processing elements behind
+ * its digit-operation (DOp) surface: a C host hands it 32-bit DOp words (or IOp
+ * words plus registered DOp templates) instead of streaming them to the FPGA.
+ * Every entry point below names the reference interface it replaces.  Reference
+ * paths are relative to the reference checkout.
+ *
+ * Conventions
+ *  - Ciphertexts are "BLWE": k*N mask coefficients (polynomial g at [g*N, (g+1)*N),
+ *    natural coefficient order) followed by the body, all u64 mod 2^64.
+ *  - LUTs are GLWE: (k+1)*N u64, mask polynomials then the body polynomial.
+ *  - Status: every int-returning call returns LUTHE_OK (0) or a negative LUTHE_ERR_*;
+ *    luthe_last_error() gives the message of the last failure on that context.
+ *  - Threading: a context is externally synchronised (one submitting thread at a
+ *    time).  Device memory belongs to the context and is freed by luthe_destroy().
+ *    "d_" pointers are device pointers on the context's device; other pointers are
+ *    host memory owned by the caller.
+ *  - The library requires a gfx950 GPU; there is no CPU execution path.
+ */
+#ifndef LUTHE_H
+#define LUTHE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LUTHE_ABI_VERSION 1
+
+enum {
+  LUTHE_OK = 0,
+  LUTHE_ERR_ARG = -1,         /* bad argument (null pointer, size) */
+  LUTHE_ERR_PARAMS = -2,      /* parameter set not supported */
+  LUTHE_ERR_HIP = -3,         /* HIP runtime error */
+  LUTHE_ERR_NOMEM = -4,       /* device memory / HBM admission refused */
+  LUTHE_ERR_DOP = -5,         /* malformed DOp or IOp word */
+  LUTHE_ERR_RANGE = -6,       /* register, slot or LUT id out of range */
+  LUTHE_ERR_STATE = -7,       /* keys not uploaded, unknown ticket, ... */
+  LUTHE_ERR_TIMEOUT = -8,     /* luthe_wait timed out */
+  LUTHE_ERR_NODEV = -9        /* no usable gfx950 device */
+};
+
+/* TFHE parameter set.  Replaces the compile-time parameter packages
+ * hw/module/param/module/<name>/rtl/param_tfhe_definition_pkg.sv (N, GLWE_K, PBS_L,
+ * PBS_B_W, LWE_K, KS_L, KS_B_W, MOD_KSK_W, PAYLOAD/PADDING) and the memory sizes of
+ * fw/ublaze/src/hpu_dop_fmt.h:43 (CT_MEM_SLOT), hpu_config.toml.j2:22 (lut_mem) and
+ * docs/hpu.md:31-32 (64 registers). */
+typedef struct luthe_params {
+  uint32_t lwe_dim;        /* n (LWE_K) */
+  uint32_t glwe_k;         /* k (GLWE_K), must be 1 */
+  uint32_t poly_size;      /* N, must be 2048 */
+  uint32_t pbs_base_log;   /* PBS_B_W */
+  uint32_t pbs_level;      /* PBS_L in {1, 2} */
+  uint32_t ks_base_log;    /* KS_B_W in [1, 8] */
+  uint32_t ks_level;       /* KS_L in [1, 8] */
+  uint32_t ks_mod_log;     /* MOD_KSK_W in [13, 32] */
+  uint32_t msg_bits;       /* message bits of a block (2) */
+  uint32_t carry_bits;     /* carry bits of a block (2) */
+  double lwe_noise_std;    /* keygen/encryption only: std relative to the modulus */
+  double glwe_noise_std;
+  uint32_t regfile_size;   /* BLWE registers, <= 128 (RID_W = 7) */
+  uint32_t ct_slots;       /* ciphertext memory slots, <= 65536 (CID_W = 16) */
+  uint32_t lut_slots;      /* LUT GLWE slots, <= 4096 (GID_W = 12) */
+} luthe_params;
+
+typedef struct luthe_ctx luthe_ctx;
+
+/* Counters.  Replaces the HPU info/counter registers (pep_common_param_pkg.sv:277-330,
+ * hpu_regif_counter_manager.sv): PBS issued, batches, fill, DOps by kind. */
+typedef struct luthe_counters {
+  uint64_t pbs;            /* PBS (KS + blind rotation + extract) executed */
+  uint64_t pbs_batches;    /* blind-rotation launches */
+  uint64_t pbs_max_batch;  /* largest batch launched */
+  uint64_t alu;            /* ALU DOps executed */
+  uint64_t mem;            /* LD/ST DOps executed */
+  uint64_t dops;           /* DOps accepted */
+  uint64_t iops;           /* IOps accepted */
+  uint64_t tickets;        /* submissions */
+  uint64_t levels;         /* dependency levels scheduled */
+} luthe_counters;
+
+/* ---------------------------------------------------------------- context */
+
+/* Config-0 defaults: n=724, k=1, N=2048, PBS 2^23 x 1, KS 2^3 x 5, w_ks=21,
+ * 2+2 bit blocks, noise 2^-15 / 2^-51, 64 registers, 4096 ct slots, 256 LUT slots. */
+void luthe_params_default(luthe_params* p);
+
+/* Create a context on HIP device `device`.  Allocates the register file, ciphertext
+ * and LUT memories.  Replaces the card bring-up and info-register read-back
+ * (docs/hpu.md:42-48, hpu_regif_core_cfg_1in3.toml:43-200). */
+int luthe_create(const luthe_params* p, int device, luthe_ctx** out);
+int luthe_destroy(luthe_ctx* ctx);
+const char* luthe_last_error(const luthe_ctx* ctx);
+int luthe_get_params(const luthe_ctx* ctx, luthe_params* out);
+int luthe_get_counters(const luthe_ctx* ctx, luthe_counters* out);
+/* HBM bytes held by the context (keys, memories, scratch). */
+uint64_t luthe_device_bytes(const luthe_ctx* ctx);
+
+/* ---------------------------------------------------------------- keys and memories */
+
+#define LUTHE_BSK_COEF 0 /* [n][k+1][L][k+1][N] coefficient domain, values < p */
+#define LUTHE_BSK_NTT 1  /* same shape, NTT domain in the order of the reference model's
+                            ntt() with cut [5,6] (sw/ntt/ntt_gf64.sage:367-445) */
+
+/* Upload the bootstrapping key (GGSW_i of LWE key bit i, mod p = 2^64-2^32+1).
+ * Row (g, l): g-th GLWE component gadget level l (l = 0 is the least significant
+ * level, gadget round(p / 2^(b*(L-l)))), columns j = GLWE components.  Converted on
+ * the device into the blind-rotation layout.  Replaces the BSK image written to HBM
+ * and streamed by hw/module/bsk/rtl/bsk_manager.sv (header: reverse order (R,N)). */
+int luthe_upload_bsk(luthe_ctx* ctx, const uint64_t* bsk, int layout);
+
+/* Upload the key-switching key [k*N][ks_level][n+1] (u32, values mod 2^ks_mod_log;
+ * level 0 = least significant digit, weight 2^(w - b*(L-l))).  Replaces the KSK
+ * image streamed by hw/module/ksk/rtl/ksk_manager.sv. */
+int luthe_upload_ksk(luthe_ctx* ctx, const uint32_t* ksk);
+
+/* LUT memory (GLWE indexed by gid, hpu_dop_fmt.h:94-99 / GID_W=12). */
+int luthe_write_lut(luthe_ctx* ctx, uint32_t gid, const uint64_t* glwe, uint32_t count);
+
+/* Ciphertext memory (slots addressed by cid, hpu_dop_fmt.h:79-85 / CID_W=16) and the
+ * register file (rid, RID_W=7).  count = number of consecutive BLWEs. */
+int luthe_write_ct(luthe_ctx* ctx, uint32_t cid, const uint64_t* blwe, uint32_t count);
+int luthe_read_ct(luthe_ctx* ctx, uint32_t cid, uint64_t* blwe, uint32_t count);
+int luthe_write_reg(luthe_ctx* ctx, uint32_t rid, const uint64_t* blwe, uint32_t count);
+int luthe_read_reg(luthe_ctx* ctx, uint32_t rid, uint64_t* blwe, uint32_t count);
+
+/* ---------------------------------------------------------------- DOp stream */
+
+/* Submit a stream of 32-bit DOp words (formats of fw/ublaze/src/hpu_dop_fmt.h:50-116,
+ * opcodes of hpu_common_instruction_pkg.sv:42-68).  The whole stream is validated
+ * first (nothing runs if any word is invalid); memory operands must be concrete
+ * (MEM_ADDR) and immediates constant (IMM_CST).  DOps run as soon as their register
+ * and slot dependencies allow, PBS DOps of the same dependency level sharing one
+ * batched launch; results equal in-order execution.  SYNC words are accepted and
+ * order nothing further (completion is per ticket).  Replaces the firmware's DOp
+ * stream to the scheduler (ucore_fw.c:96-110) and its ack path (ucore_fw.c:224-232). */
+int luthe_submit_dops(luthe_ctx* ctx, const uint32_t* words, size_t n, uint64_t* ticket);
+/* 1 = done, 0 = pending, < 0 = error. */
+int luthe_poll(luthe_ctx* ctx, uint64_t ticket);
+/* Block until the ticket completes; timeout_us < 0 waits forever. */
+int luthe_wait(luthe_ctx* ctx, uint64_t ticket, int64_t timeout_us);
+
+/* ---------------------------------------------------------------- IOp path */
+
+/* Register the DOp template program of IOp `opcode` for operand width `max_align`
+ * blocks (the translation table at DOP_LUT_ADDR, ucore.h:40-42, ucore_fw.c:207-220).
+ * Templates may use MEM_HEAP / MEM_SRC / MEM_DST slots and IMM_VAR immediates. */
+int luthe_register_iop(luthe_ctx* ctx, uint8_t opcode, uint8_t max_align, const uint32_t* dops, size_t n);
+/* Submit one IOp (header, destination and source operand words, optional immediates:
+ * hpu_iop_fmt.h:19-76): parse_iop + get_lookup + patch_dop of ucore_fw.c:135-220 and
+ * hpu_dop_fmt.h:148-229, then as luthe_submit_dops. */
+int luthe_submit_iop(luthe_ctx* ctx, const uint32_t* words, size_t n, uint64_t* ticket);
+
+/* ---------------------------------------------------------------- batched hot path */
+
+/* Batched PBS on caller device memory: for c < n, d_out[c*lut_nb + j] = PBS of d_in[c]
+ * with LUT gid d_gid[c], Many-LUT output j < lut_nb = 2^log_lut_nb (BLWE stride
+ * k*N+1).  Key switch, blind rotation and sample extract in one stream-ordered
+ * sequence on `stream` (a hipStream_t, NULL = the context stream).  Replaces the PBS
+ * processing element pe_pbs (hw/module/pe_pbs/rtl/pe_pbs.sv:14-60) and its batch
+ * command (pep_common_param_pkg.sv:95-159).  LUT ids >= lut_slots are clamped to the
+ * last LUT slot (device-side ids cannot be validated without a host round trip). */
+int luthe_pbs_batch(luthe_ctx* ctx, const uint64_t* d_in, uint64_t* d_out, const uint32_t* d_gid,
+                    uint32_t log_lut_nb, size_t n, void* stream);
+
+/* Key switch + mod switch to 2N only: d_lwe2n[c*(n+1) + j] in [0, 2N)
+ * (pep_key_switch: pep_ks_blram_format.sv:159-246, pep_ks_out_process.sv:276-341). */
+int luthe_keyswitch_batch(luthe_ctx* ctx, const uint64_t* d_blwe, uint16_t* d_lwe2n, size_t n, void* stream);
+
+/* Blind rotation + sample extract from mod-switched LWEs (pep_mono_mult_acc +
+ * ntt_core_gf64 + decomposer); d_lwe2n rows of n+1 values. */
+int luthe_blind_rotate_batch(luthe_ctx* ctx, const uint16_t* d_lwe2n, uint64_t* d_out, const uint32_t* d_gid,
+                             uint32_t log_lut_nb, size_t n, void* stream);
+
+/* Negacyclic NTT core, N = 2048 (ntt_core_gf64: ntt_core_gf64.sv:106-163,
+ * ntt_core_gf64_middle.sv:201-475): d_out[b] = INTT(NTT(d_a[b]) (.) d_bn[b]) for
+ * b < npoly; d_bn in the NTT domain (reference ntt() order, cut [5,6]); output
+ * canonical in [0, p). */
+int luthe_ntt_polymul_batch(luthe_ctx* ctx, const uint64_t* d_a, const uint64_t* d_bn, uint64_t* d_out,
+                            size_t npoly, void* stream);
+/* Forward / inverse transforms alone (NTT-domain in reference ntt() order). */
+int luthe_ntt_forward_batch(luthe_ctx* ctx, const uint64_t* d_in, uint64_t* d_out, size_t npoly, void* stream);
+int luthe_ntt_inverse_batch(luthe_ctx* ctx, const uint64_t* d_in, uint64_t* d_out, size_t npoly, void* stream);
+
+/* Synchronise the context stream. */
+int luthe_sync(luthe_ctx* ctx);
+/* The context's HIP stream (hipStream_t), on which DOp submissions and NULL-stream
+ * batch calls run.  Batch calls share per-context scratch: calls that overlap in time
+ * must be issued on the same stream. */
+void* luthe_stream(luthe_ctx* ctx);
+/* Blind-rotation launch width (workgroups of the persistent kernel); 0 = occupancy x CUs. */
+int luthe_set_br_grid(luthe_ctx* ctx, int workgroups);
+int luthe_get_br_grid(luthe_ctx* ctx);
+
+/* ---------------------------------------------------------------- client-side helpers
+ * Key generation, encryption, decryption and LUT encoding live in tfhe-rs for the
+ * reference (not part of the checkout); these builder-defined, seeded host routines
+ * play that role for tests and benchmarks.  Secret keys are arrays of 0/1 u64. */
+
+int luthe_keygen_secret(const luthe_params* p, uint64_t seed, uint64_t* lwe_sk, uint64_t* glwe_sk);
+/* bsk: [n][k+1][L][k+1][N] coefficient domain (LUTHE_BSK_COEF). */
+int luthe_keygen_bsk(const luthe_params* p, uint64_t seed, const uint64_t* lwe_sk, const uint64_t* glwe_sk,
+                     uint64_t* bsk, int threads);
+int luthe_keygen_ksk(const luthe_params* p, uint64_t seed, const uint64_t* lwe_sk, const uint64_t* glwe_sk,
+                     uint32_t* ksk);
+/* BLWE encryption of raw plaintexts pt[c] (already scaled into the top bits) under
+ * the flattened GLWE key with GLWE noise; stream `seed`. */
+int luthe_encrypt_blwe(const luthe_params* p, uint64_t seed, const uint64_t* glwe_sk, const uint64_t* pt, size_t n,
+                       uint64_t* blwe);
+/* Phase b - <a, s> mod 2^64 of n BLWEs. */
+int luthe_decrypt_blwe(const luthe_params* p, const uint64_t* glwe_sk, const uint64_t* blwe, size_t n,
+                       uint64_t* phase);
+/* Message scale Delta = 2^64 / 2^(msg_bits + carry_bits + 1) (padding bit). */
+uint64_t luthe_delta(const luthe_params* p);
+/* Decode a phase to the message in [0, 2^(msg+carry+1)). */
+uint32_t luthe_decode(const luthe_params* p, uint64_t phase);
+/* Trivial GLWE LUT of table[m] (m < 2^(msg+carry)), boxes of N / 2^(msg+carry)
+ * coefficients, shifted by half a box (negacyclic) so noisy phases round to m. */
+int luthe_lut_encode(const luthe_params* p, const uint8_t* table, uint32_t table_len, uint64_t* glwe);
+/* Many-LUT GLWE: 2^log_lut_nb functions of an input restricted to
+ * 2^(msg+carry-log_lut_nb) values; function j lands on output lut j. */
+int luthe_lut_encode_many(const luthe_params* p, const uint8_t* tables, uint32_t table_len, uint32_t log_lut_nb,
+                          uint64_t* glwe);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,447 @@
+"""Python binding of libluthe.so, the MI355X TFHE bootstrapping engine.
+
+The product is the C ABI declared in ``include/luthe.h``; this module binds it with
+ctypes for tests, benchmarks and Python hosts.  Device buffers and events are taken
+from the same HIP runtime libluthe links (``libamdhip64``), so one runtime serves the
+whole process.  There is no CPU execution path: loading fails loudly when the
+library has not been built, and ``Context`` fails when no gfx950 GPU is present.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libluthe.so")
+_LIB = None
+_HIP = None
+
+P = 0xFFFFFFFF00000001
+ERRORS = {0: "OK", -1: "ARG", -2: "PARAMS", -3: "HIP", -4: "NOMEM", -5: "DOP", -6: "RANGE",
+          -7: "STATE", -8: "TIMEOUT", -9: "NODEV"}
+BSK_COEF, BSK_NTT = 0, 1
+
+
+class LutheError(RuntimeError):
+    def __init__(self, code, msg=""):
+        super().__init__(f"luthe error {code} ({ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [("lwe_dim", C.c_uint32), ("glwe_k", C.c_uint32), ("poly_size", C.c_uint32),
+                ("pbs_base_log", C.c_uint32), ("pbs_level", C.c_uint32), ("ks_base_log", C.c_uint32),
+                ("ks_level", C.c_uint32), ("ks_mod_log", C.c_uint32), ("msg_bits", C.c_uint32),
+                ("carry_bits", C.c_uint32), ("lwe_noise_std", C.c_double), ("glwe_noise_std", C.c_double),
+                ("regfile_size", C.c_uint32), ("ct_slots", C.c_uint32), ("lut_slots", C.c_uint32)]
+
+    @property
+    def blen(self):
+        return self.glwe_k * self.poly_size + 1
+
+    @property
+    def lut_len(self):
+        return (self.glwe_k + 1) * self.poly_size
+
+    def bsk_shape(self):
+        k1 = self.glwe_k + 1
+        return (self.lwe_dim, k1, self.pbs_level, k1, self.poly_size)
+
+    def ksk_shape(self):
+        return (self.glwe_k * self.poly_size, self.ks_level, self.lwe_dim + 1)
+
+
+class Counters(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("pbs", "pbs_batches", "pbs_max_batch", "alu", "mem", "dops",
+                                          "iops", "tickets", "levels")]
+
+
+def lib():
+    """Load libluthe.so (built by ``__graft_entry__.build()``)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    L = C.CDLL(LIB_PATH)
+    vp, u32, u64, sz, i32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_size_t, C.c_int
+    P_ = C.POINTER(Params)
+    sigs = {
+        "luthe_params_default": (None, [P_]),
+        "luthe_create": (i32, [P_, i32, C.POINTER(vp)]),
+        "luthe_destroy": (i32, [vp]),
+        "luthe_last_error": (C.c_char_p, [vp]),
+        "luthe_get_params": (i32, [vp, P_]),
+        "luthe_get_counters": (i32, [vp, C.POINTER(Counters)]),
+        "luthe_device_bytes": (u64, [vp]),
+        "luthe_upload_bsk": (i32, [vp, vp, i32]),
+        "luthe_upload_ksk": (i32, [vp, vp]),
+        "luthe_write_lut": (i32, [vp, u32, vp, u32]),
+        "luthe_write_ct": (i32, [vp, u32, vp, u32]),
+        "luthe_read_ct": (i32, [vp, u32, vp, u32]),
+        "luthe_write_reg": (i32, [vp, u32, vp, u32]),
+        "luthe_read_reg": (i32, [vp, u32, vp, u32]),
+        "luthe_submit_dops": (i32, [vp, vp, sz, C.POINTER(u64)]),
+        "luthe_poll": (i32, [vp, u64]),
+        "luthe_wait": (i32, [vp, u64, C.c_int64]),
+        "luthe_register_iop": (i32, [vp, C.c_uint8, C.c_uint8, vp, sz]),
+        "luthe_submit_iop": (i32, [vp, vp, sz, C.POINTER(u64)]),
+        "luthe_pbs_batch": (i32, [vp, vp, vp, vp, u32, sz, vp]),
+        "luthe_keyswitch_batch": (i32, [vp, vp, vp, sz, vp]),
+        "luthe_blind_rotate_batch": (i32, [vp, vp, vp, vp, u32, sz, vp]),
+        "luthe_ntt_polymul_batch": (i32, [vp, vp, vp, vp, sz, vp]),
+        "luthe_ntt_forward_batch": (i32, [vp, vp, vp, sz, vp]),
+        "luthe_ntt_inverse_batch": (i32, [vp, vp, vp, sz, vp]),
+        "luthe_sync": (i32, [vp]),
+        "luthe_stream": (vp, [vp]),
+        "luthe_set_br_grid": (i32, [vp, i32]),
+        "luthe_get_br_grid": (i32, [vp]),
+        "luthe_keygen_secret": (i32, [P_, u64, vp, vp]),
+        "luthe_keygen_bsk": (i32, [P_, u64, vp, vp, vp, i32]),
+        "luthe_keygen_ksk": (i32, [P_, u64, vp, vp, vp]),
+        "luthe_encrypt_blwe": (i32, [P_, u64, vp, vp, sz, vp]),
+        "luthe_decrypt_blwe": (i32, [P_, vp, vp, sz, vp]),
+        "luthe_delta": (u64, [P_]),
+        "luthe_decode": (u32, [P_, u64]),
+        "luthe_lut_encode": (i32, [P_, vp, u32, vp]),
+        "luthe_lut_encode_many": (i32, [P_, vp, u32, u32, vp]),
+    }
+    for name, (res, args) in sigs.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _LIB = L
+    return L
+
+
+def exported_symbols():
+    """Names of the C-ABI entry points this binding expects."""
+    lib()
+    return [n for n in dir(_LIB) if n.startswith("luthe_")]
+
+
+def hip():
+    """The HIP runtime libluthe.so is linked against (already loaded by it)."""
+    global _HIP
+    if _HIP is None:
+        lib()
+        H = C.CDLL("libamdhip64.so.7")
+        vp, sz = C.c_void_p, C.c_size_t
+        for name, res, args in [
+            ("hipMalloc", C.c_int, [C.POINTER(vp), sz]),
+            ("hipFree", C.c_int, [vp]),
+            ("hipMemcpy", C.c_int, [vp, vp, sz, C.c_int]),
+            ("hipMemset", C.c_int, [vp, C.c_int, sz]),
+            ("hipDeviceSynchronize", C.c_int, []),
+            ("hipStreamSynchronize", C.c_int, [vp]),
+            ("hipEventCreate", C.c_int, [C.POINTER(vp)]),
+            ("hipEventDestroy", C.c_int, [vp]),
+            ("hipEventRecord", C.c_int, [vp, vp]),
+            ("hipEventSynchronize", C.c_int, [vp]),
+            ("hipEventElapsedTime", C.c_int, [C.POINTER(C.c_float), vp, vp]),
+            ("hipGetDeviceCount", C.c_int, [C.POINTER(C.c_int)]),
+        ]:
+            f = getattr(H, name)
+            f.restype = res
+            f.argtypes = args
+        _HIP = H
+    return _HIP
+
+
+def _hipchk(rc, what):
+    if rc != 0:
+        raise LutheError(-3, f"{what} returned hipError {rc}")
+
+
+def device_count():
+    n = C.c_int(0)
+    rc = hip().hipGetDeviceCount(C.byref(n))
+    return n.value if rc == 0 else 0
+
+
+class DeviceBuffer:
+    """A hipMalloc'd buffer viewed as a typed array."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(int(s) for s in np.atleast_1d(shape))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.ptr = C.c_void_p()
+        _hipchk(hip().hipMalloc(C.byref(self.ptr), max(self.nbytes, 8)), "hipMalloc")
+
+    @classmethod
+    def from_numpy(cls, arr):
+        arr = np.ascontiguousarray(arr)
+        b = cls(arr.shape, arr.dtype)
+        b.upload(arr)
+        return b
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr, dtype=self.dtype)
+        assert arr.nbytes == self.nbytes
+        _hipchk(hip().hipMemcpy(self.ptr, arr.ctypes.data_as(C.c_void_p), self.nbytes, 1), "hipMemcpy H2D")
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        _hipchk(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
+        _hipchk(hip().hipMemcpy(out.ctypes.data_as(C.c_void_p), self.ptr, self.nbytes, 2), "hipMemcpy D2H")
+        return out
+
+    def zero(self):
+        _hipchk(hip().hipMemset(self.ptr, 0, self.nbytes), "hipMemset")
+
+    def free(self):
+        if self.ptr:
+            hip().hipFree(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _vp(a):
+    if isinstance(a, DeviceBuffer):
+        return a.ptr
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def default_params(**over):
+    p = Params()
+    lib().luthe_params_default(C.byref(p))
+    for k, v in over.items():
+        setattr(p, k, v)
+    return p
+
+
+class Context:
+    """An engine context on one GPU (owns register file, memories and keys)."""
+
+    def __init__(self, params=None, device=0):
+        self.params = params if params is not None else default_params()
+        self._h = C.c_void_p()
+        rc = lib().luthe_create(C.byref(self.params), device, C.byref(self._h))
+        if rc != 0:
+            raise LutheError(rc, "luthe_create")
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = lib().luthe_last_error(self._h)
+            raise LutheError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def stream(self):
+        return lib().luthe_stream(self._h)
+
+    def close(self):
+        if self._h:
+            lib().luthe_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload_bsk(self, bsk, layout=BSK_COEF):
+        bsk = np.ascontiguousarray(bsk, np.uint64)
+        assert bsk.size == int(np.prod(self.params.bsk_shape()))
+        self._chk(lib().luthe_upload_bsk(self._h, _vp(bsk), layout), "upload_bsk")
+
+    def upload_ksk(self, ksk):
+        ksk = np.ascontiguousarray(ksk, np.uint32)
+        assert ksk.size == int(np.prod(self.params.ksk_shape()))
+        self._chk(lib().luthe_upload_ksk(self._h, _vp(ksk)), "upload_ksk")
+
+    def write_lut(self, gid, glwe):
+        glwe = np.ascontiguousarray(glwe, np.uint64).reshape(-1, self.params.lut_len)
+        self._chk(lib().luthe_write_lut(self._h, gid, _vp(glwe), glwe.shape[0]), "write_lut")
+
+    def write_ct(self, cid, blwe):
+        blwe = np.ascontiguousarray(blwe, np.uint64).reshape(-1, self.params.blen)
+        self._chk(lib().luthe_write_ct(self._h, cid, _vp(blwe), blwe.shape[0]), "write_ct")
+
+    def read_ct(self, cid, count=1):
+        out = np.empty((count, self.params.blen), np.uint64)
+        self._chk(lib().luthe_read_ct(self._h, cid, _vp(out), count), "read_ct")
+        return out
+
+    def write_reg(self, rid, blwe):
+        blwe = np.ascontiguousarray(blwe, np.uint64).reshape(-1, self.params.blen)
+        self._chk(lib().luthe_write_reg(self._h, rid, _vp(blwe), blwe.shape[0]), "write_reg")
+
+    def read_reg(self, rid, count=1):
+        out = np.empty((count, self.params.blen), np.uint64)
+        self._chk(lib().luthe_read_reg(self._h, rid, _vp(out), count), "read_reg")
+        return out
+
+    def submit_dops(self, words):
+        w = np.ascontiguousarray(words, np.uint32)
+        t = C.c_uint64(0)
+        self._chk(lib().luthe_submit_dops(self._h, _vp(w), w.size, C.byref(t)), "submit_dops")
+        return t.value
+
+    def register_iop(self, opcode, max_align, words):
+        w = np.ascontiguousarray(words, np.uint32)
+        self._chk(lib().luthe_register_iop(self._h, opcode, max_align, _vp(w), w.size), "register_iop")
+
+    def submit_iop(self, words):
+        w = np.ascontiguousarray(words, np.uint32)
+        t = C.c_uint64(0)
+        self._chk(lib().luthe_submit_iop(self._h, _vp(w), w.size, C.byref(t)), "submit_iop")
+        return t.value
+
+    def poll(self, ticket):
+        rc = lib().luthe_poll(self._h, ticket)
+        if rc < 0:
+            self._chk(rc, "poll")
+        return rc == 1
+
+    def wait(self, ticket, timeout_us=-1):
+        self._chk(lib().luthe_wait(self._h, ticket, timeout_us), "wait")
+
+    def sync(self):
+        self._chk(lib().luthe_sync(self._h), "sync")
+
+    def counters(self):
+        c = Counters()
+        self._chk(lib().luthe_get_counters(self._h, C.byref(c)), "counters")
+        return {n: getattr(c, n) for n, _ in Counters._fields_}
+
+    def device_bytes(self):
+        return lib().luthe_device_bytes(self._h)
+
+    def set_br_grid(self, wg):
+        self._chk(lib().luthe_set_br_grid(self._h, wg), "set_br_grid")
+
+    def br_grid(self):
+        return lib().luthe_get_br_grid(self._h)
+
+    # ---- batched device-memory entry points
+    def pbs_batch_dev(self, d_in, d_out, d_gid, n, log_lut_nb=0, stream=None):
+        self._chk(lib().luthe_pbs_batch(self._h, _vp(d_in), _vp(d_out), _vp(d_gid), log_lut_nb, n, stream),
+                  "pbs_batch")
+
+    def pbs_batch(self, blwe, gids, log_lut_nb=0):
+        """Host convenience: upload, run luthe_pbs_batch, download."""
+        blwe = np.ascontiguousarray(blwe, np.uint64).reshape(-1, self.params.blen)
+        n = blwe.shape[0]
+        d_in = DeviceBuffer.from_numpy(blwe)
+        d_gid = DeviceBuffer.from_numpy(np.ascontiguousarray(gids, np.uint32).reshape(n))
+        d_out = DeviceBuffer((n << log_lut_nb, self.params.blen), np.uint64)
+        self.pbs_batch_dev(d_in, d_out, d_gid, n, log_lut_nb)
+        self.sync()
+        return d_out.numpy()
+
+    def keyswitch_batch(self, blwe):
+        blwe = np.ascontiguousarray(blwe, np.uint64).reshape(-1, self.params.blen)
+        n = blwe.shape[0]
+        d_in = DeviceBuffer.from_numpy(blwe)
+        d_out = DeviceBuffer((n, self.params.lwe_dim + 1), np.uint16)
+        self._chk(lib().luthe_keyswitch_batch(self._h, d_in.ptr, d_out.ptr, n, None), "keyswitch_batch")
+        self.sync()
+        return d_out.numpy()
+
+    def blind_rotate_batch(self, lwe2n, gids, log_lut_nb=0):
+        lwe2n = np.ascontiguousarray(lwe2n, np.uint16).reshape(-1, self.params.lwe_dim + 1)
+        n = lwe2n.shape[0]
+        d_in = DeviceBuffer.from_numpy(lwe2n)
+        d_gid = DeviceBuffer.from_numpy(np.ascontiguousarray(gids, np.uint32).reshape(n))
+        d_out = DeviceBuffer((n << log_lut_nb, self.params.blen), np.uint64)
+        self._chk(lib().luthe_blind_rotate_batch(self._h, d_in.ptr, d_out.ptr, d_gid.ptr, log_lut_nb, n, None),
+                  "blind_rotate_batch")
+        self.sync()
+        return d_out.numpy()
+
+    def ntt_polymul(self, a, bn):
+        a = np.ascontiguousarray(a, np.uint64).reshape(-1, 2048)
+        bn = np.ascontiguousarray(bn, np.uint64).reshape(-1, 2048)
+        d_a, d_b = DeviceBuffer.from_numpy(a), DeviceBuffer.from_numpy(bn)
+        d_o = DeviceBuffer(a.shape, np.uint64)
+        self._chk(lib().luthe_ntt_polymul_batch(self._h, d_a.ptr, d_b.ptr, d_o.ptr, a.shape[0], None), "ntt_polymul")
+        self.sync()
+        return d_o.numpy()
+
+    def ntt_forward(self, a):
+        a = np.ascontiguousarray(a, np.uint64).reshape(-1, 2048)
+        d_a, d_o = DeviceBuffer.from_numpy(a), DeviceBuffer(a.shape, np.uint64)
+        self._chk(lib().luthe_ntt_forward_batch(self._h, d_a.ptr, d_o.ptr, a.shape[0], None), "ntt_forward")
+        self.sync()
+        return d_o.numpy()
+
+    def ntt_inverse(self, a):
+        a = np.ascontiguousarray(a, np.uint64).reshape(-1, 2048)
+        d_a, d_o = DeviceBuffer.from_numpy(a), DeviceBuffer(a.shape, np.uint64)
+        self._chk(lib().luthe_ntt_inverse_batch(self._h, d_a.ptr, d_o.ptr, a.shape[0], None), "ntt_inverse")
+        self.sync()
+        return d_o.numpy()
+
+
+# ---------------------------------------------------------------- client-side helpers
+class Keys:
+    """Seeded key material (secret keys, BSK in coefficient domain, KSK)."""
+
+    def __init__(self, params, seed=1, threads=0, with_bsk=True):
+        self.params = params
+        L = lib()
+        self.lwe_sk = np.zeros(params.lwe_dim, np.uint64)
+        self.glwe_sk = np.zeros(params.glwe_k * params.poly_size, np.uint64)
+        assert L.luthe_keygen_secret(C.byref(params), seed, _vp(self.lwe_sk), _vp(self.glwe_sk)) == 0
+        self.ksk = np.zeros(params.ksk_shape(), np.uint32)
+        assert L.luthe_keygen_ksk(C.byref(params), seed, _vp(self.lwe_sk), _vp(self.glwe_sk), _vp(self.ksk)) == 0
+        self.bsk = None
+        if with_bsk:
+            self.bsk = np.zeros(params.bsk_shape(), np.uint64)
+            assert L.luthe_keygen_bsk(C.byref(params), seed, _vp(self.lwe_sk), _vp(self.glwe_sk), _vp(self.bsk),
+                                      threads) == 0
+
+    def encrypt(self, plaintexts, seed=7):
+        pt = np.ascontiguousarray(plaintexts, np.uint64).reshape(-1)
+        out = np.empty((pt.size, self.params.blen), np.uint64)
+        assert lib().luthe_encrypt_blwe(C.byref(self.params), seed, _vp(self.glwe_sk), _vp(pt), pt.size,
+                                        _vp(out)) == 0
+        return out
+
+    def encrypt_msg(self, msgs, seed=7):
+        d = lib().luthe_delta(C.byref(self.params))
+        m = np.ascontiguousarray(msgs, np.uint64).reshape(-1)
+        return self.encrypt((m * np.uint64(d)).astype(np.uint64), seed)
+
+    def phase(self, blwe):
+        blwe = np.ascontiguousarray(blwe, np.uint64).reshape(-1, self.params.blen)
+        out = np.empty(blwe.shape[0], np.uint64)
+        assert lib().luthe_decrypt_blwe(C.byref(self.params), _vp(self.glwe_sk), _vp(blwe), blwe.shape[0],
+                                        _vp(out)) == 0
+        return out
+
+    def decrypt_msg(self, blwe):
+        return np.array([lib().luthe_decode(C.byref(self.params), int(v)) for v in self.phase(blwe)], np.uint32)
+
+
+def lut_encode(params, table):
+    t = np.ascontiguousarray(table, np.uint8)
+    out = np.empty(params.lut_len, np.uint64)
+    rc = lib().luthe_lut_encode(C.byref(params), _vp(t), t.size, _vp(out))
+    if rc != 0:
+        raise LutheError(rc, "lut_encode")
+    return out
+
+
+def lut_encode_many(params, tables, log_lut_nb):
+    t = np.ascontiguousarray(tables, np.uint8)
+    out = np.empty(params.lut_len, np.uint64)
+    rc = lib().luthe_lut_encode_many(C.byref(params), _vp(t), t.shape[-1], log_lut_nb, _vp(out))
+    if rc != 0:
+        raise LutheError(rc, "lut_encode_many")
+    return out
+
+
+def delta(params):
+    return lib().luthe_delta(C.byref(params))

@@ -1,0 +1,60 @@
+// Kernel-side descriptors and host launchers shared by kernels.hip and the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace lk {
+
+// One PBS of a batch: BLWE row to read (key switch input), first BLWE row to write
+// (lut_nb consecutive rows for Many-LUT), LUT id, log2 of the number of LUTs.
+struct PbsJob {
+  uint32_t in_row;
+  uint32_t out_row;
+  uint32_t gid;
+  uint32_t log_lut_nb;
+};
+
+struct AluOp {
+  uint32_t opcode;
+  uint32_t dst_row;
+  uint32_t src0_row;
+  uint32_t src1_row;
+  uint32_t mul_factor;
+  uint32_t msg_cst;
+};
+
+struct CopyOp {
+  uint32_t dst_row;
+  uint32_t src_row;
+};
+
+struct KsArgs {
+  int kN;          // mask length of the input BLWE
+  int n_lwe;       // output LWE dimension
+  int lwe_stride;  // u16 elements between output rows (>= n_lwe + 1)
+  uint32_t base_log, mod_log, lc;  // KS_B_W, MOD_KSK_W, log2(2N)
+};
+
+struct BrArgs {
+  int n_lwe;
+  int lwe_stride;
+  uint32_t base_log;  // PBS_B_W
+};
+
+hipError_t launch_ntt(int mode, const uint64_t* a, const uint64_t* bn, uint64_t* out, size_t npoly,
+                      const uint64_t* tw, const uint64_t* itw, hipStream_t s);
+hipError_t launch_bsk_prepare(const uint64_t* src, uint64_t* dst, int n_lwe, int rows, int K1, int is_ntt,
+                              const uint64_t* tw, hipStream_t s);
+hipError_t launch_keyswitch(const uint64_t* base, const PbsJob* jobs, int nct, const uint32_t* ksk, uint16_t* lwe2n,
+                            const KsArgs& a, int ks_level, hipStream_t s);
+hipError_t launch_blind_rotate(const uint64_t* bsk, const uint64_t* luts, const uint16_t* lwe2n, const PbsJob* jobs,
+                               int nct, uint64_t* out_base, const uint64_t* tw, const uint64_t* itw, const BrArgs& a,
+                               int pbs_level, int grid_override, hipStream_t s);
+int blind_rotate_occupancy(int pbs_level);
+hipError_t launch_alu(uint64_t* base, const AluOp* ops, int nops, int blen, hipStream_t s);
+hipError_t launch_copy(uint64_t* base, const CopyOp* ops, int nops, int blen, hipStream_t s);
+hipError_t launch_make_jobs(PbsJob* jobs, const uint32_t* gid, int n, uint32_t log_lut_nb, uint32_t lut_slots,
+                            hipStream_t s);
+
+}  // namespace lk

@@ -1,0 +1,450 @@
+// HIP kernels of the MI355X PBS engine (gfx950, wave64).
+//
+//  k_ntt_*        batched negacyclic NTT / INTT / NTT-multiply-INTT, N = 2048
+//  k_bsk_prepare  bootstrapping key -> NTT domain, blind-rotation layout
+//  k_keyswitch    batched LWE key switch (integer GEMM mod 2^32 on VALU, 24-bit MACs)
+//  k_blind_rotate persistent blind rotation: one ciphertext per 256-thread workgroup,
+//                 accumulator in registers, NTT work buffer in LDS, BSK streamed from
+//                 HBM/L2 in a coalesced per-thread layout; sample extract fused
+//  k_alu, k_copy  DOp ALU and LD/ST on BLWE rows
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+#include "ntt2048.h"
+#include "pbs_core.h"
+
+using namespace ntt;
+
+namespace lk {
+
+// ------------------------------------------------------------------------------------
+// NTT kernels: NP polynomials per 256-thread workgroup.
+// ------------------------------------------------------------------------------------
+template <int NP>
+__device__ __forceinline__ void fwd_passes(uint64_t (*v)[8], uint64_t* lds, const uint64_t* tw, int t) {
+  const int r = t & 63, m0 = t >> 6, k = t >> 3, j = t & 7;
+#pragma unroll
+  for (int p = 0; p < NP; p++) fwd_a1_compute(v[p], m0);
+#pragma unroll
+  for (int p = 0; p < NP; p++) fwd_a1_store(v[p], lds + p * N, r, m0);
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < NP; p++) {
+    fwd_a2_load(v[p], lds + p * N, r, m0);
+    fwd_a2_compute(v[p], r, m0, tw);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < NP; p++) fwd_a2_store(v[p], lds + p * N, r, m0);
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < NP; p++) {
+    fwd_c1_load(v[p], lds + p * N, k, j);
+    fwd_c1_compute(v[p], j);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < NP; p++) fwd_c1_store(v[p], lds + p * N, k, j);
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < NP; p++) {
+    fwd_c2_load(v[p], lds + p * N, k, j);
+    fwd_c2_compute(v[p]);
+  }
+}
+
+// Entry: v in NTT domain (thread (k, w1) layout), LDS free for writing only after a
+// barrier that orders the previous readers.  Exit: v[p][m1] canonical coefficients.
+template <int NP>
+__device__ __forceinline__ void inv_passes(uint64_t (*v)[8], uint64_t* lds, const uint64_t* itw, int t) {
+  const int r = t & 63, m0 = t >> 6, k = t >> 3, j = t & 7;
+#pragma unroll
+  for (int p = 0; p < NP; p++) inv_c2_compute(v[p], j);
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < NP; p++) inv_c2_store(v[p], lds + p * N, k, j);
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < NP; p++) {
+    inv_c1_load(v[p], lds + p * N, k, j);
+    inv_c1_compute(v[p], k, j, itw);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < NP; p++) inv_c1_store(v[p], lds + p * N, k, j);
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < NP; p++) {
+    inv_a2_load(v[p], lds + p * N, r, m0);
+    inv_a2_compute(v[p], m0);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < NP; p++) inv_a2_store(v[p], lds + p * N, r, m0);
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < NP; p++) {
+    inv_a1_load(v[p], lds + p * N, r, m0);
+    inv_a1_compute(v[p], m0);
+  }
+}
+
+// mode 0: forward (out NTT domain), 1: inverse (in NTT domain), 2: polymul
+template <int NP, int MODE>
+__global__ __launch_bounds__(256) void k_ntt(const uint64_t* __restrict__ a, const uint64_t* __restrict__ bn,
+                                             uint64_t* __restrict__ out, size_t npoly,
+                                             const uint64_t* __restrict__ tw, const uint64_t* __restrict__ itw) {
+  __shared__ uint64_t lds[NP * N];
+  const int t = threadIdx.x;
+  const int r = t & 63, m0 = t >> 6, k = t >> 3, w1 = t & 7;
+  const size_t base = (size_t)blockIdx.x * NP;
+  uint64_t v[NP][8];
+  if (MODE == 1) {
+#pragma unroll
+    for (int p = 0; p < NP; p++) {
+      size_t poly = base + p < npoly ? base + p : npoly - 1;
+#pragma unroll
+      for (int w0 = 0; w0 < 8; w0++) v[p][w0] = a[poly * N + 64 * k + w1 + 8 * w0];
+    }
+  } else {
+#pragma unroll
+    for (int p = 0; p < NP; p++) {
+      size_t poly = base + p < npoly ? base + p : npoly - 1;
+#pragma unroll
+      for (int m1 = 0; m1 < 8; m1++) v[p][m1] = a[poly * N + 64 * (m0 + 4 * m1) + r];
+    }
+    fwd_passes<NP>(v, lds, tw, t);
+  }
+  if (MODE == 0) {
+#pragma unroll
+    for (int p = 0; p < NP; p++) {
+      if (base + p >= npoly) break;
+#pragma unroll
+      for (int w0 = 0; w0 < 8; w0++) out[(base + p) * N + 64 * k + w1 + 8 * w0] = gl::canonical(v[p][w0]);
+    }
+    return;
+  }
+  if (MODE == 2) {
+#pragma unroll
+    for (int p = 0; p < NP; p++) {
+      size_t poly = base + p < npoly ? base + p : npoly - 1;
+#pragma unroll
+      for (int w0 = 0; w0 < 8; w0++) v[p][w0] = gl::mul(v[p][w0], bn[poly * N + 64 * k + w1 + 8 * w0]);
+    }
+  }
+  inv_passes<NP>(v, lds, itw, t);
+#pragma unroll
+  for (int p = 0; p < NP; p++) {
+    if (base + p >= npoly) break;
+#pragma unroll
+    for (int m1 = 0; m1 < 8; m1++) out[(base + p) * N + 64 * (m0 + 4 * m1) + r] = v[p][m1];
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// Bootstrapping key preparation: one workgroup per polynomial (i, gl, j).
+// Internal layout [i][w0][K1*L][K1][T]: thread t of the blind rotation reads its own
+// 8 values per (gl, j) with fully coalesced loads.
+// ------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_bsk_prepare(const uint64_t* __restrict__ src, uint64_t* __restrict__ dst,
+                                                     int rows, int K1, int is_ntt, const uint64_t* __restrict__ tw) {
+  __shared__ uint64_t lds[N];
+  const int t = threadIdx.x;
+  const int r = t & 63, m0 = t >> 6, k = t >> 3, w1 = t & 7;
+  const size_t poly = blockIdx.x;  // = (i * rows + gl) * K1 + j
+  const size_t j = poly % K1, gl_ = (poly / K1) % rows, i = poly / ((size_t)K1 * rows);
+  uint64_t v[1][8];
+  if (is_ntt) {
+#pragma unroll
+    for (int w0 = 0; w0 < 8; w0++) v[0][w0] = src[poly * N + 64 * k + w1 + 8 * w0];
+  } else {
+#pragma unroll
+    for (int m1 = 0; m1 < 8; m1++) v[0][m1] = src[poly * N + 64 * (m0 + 4 * m1) + r];
+    fwd_passes<1>(v, lds, tw, t);
+  }
+#pragma unroll
+  for (int w0 = 0; w0 < 8; w0++)
+    dst[(((i * 8 + w0) * rows + gl_) * K1 + j) * T + t] = gl::canonical(v[0][w0]);
+}
+
+// ------------------------------------------------------------------------------------
+// Key switch.  Workgroup = 32 ciphertexts x 256 output coefficients; K dimension =
+// (k*N mask coefficients) x KSL levels, streamed in chunks of 64 coefficients whose
+// digits are decomposed once into LDS and broadcast to all threads.
+// ------------------------------------------------------------------------------------
+constexpr int KS_CT = 32;
+constexpr int KS_CHUNK = 64;
+
+template <int KSL>
+__global__ __launch_bounds__(256) void k_keyswitch(const uint64_t* __restrict__ base, const PbsJob* __restrict__ jobs,
+                                                   int nct, const uint32_t* __restrict__ ksk,
+                                                   uint16_t* __restrict__ lwe2n, KsArgs a) {
+  __shared__ int32_t dig[KS_CHUNK][KSL][KS_CT / 4];  // 4 signed 8-bit digits per word
+  const int t = threadIdx.x;
+  const int c0 = blockIdx.y * KS_CT;
+  const int j = blockIdx.x * 256 + t;
+  const int n1 = a.n_lwe + 1;
+  const size_t blen = (size_t)a.kN + 1;
+  int32_t acc[KS_CT];
+#pragma unroll
+  for (int c = 0; c < KS_CT; c++) acc[c] = 0;
+  int8_t* dig8 = reinterpret_cast<int8_t*>(&dig[0][0][0]);
+  for (int i0 = 0; i0 < a.kN; i0 += KS_CHUNK) {
+    __syncthreads();
+    // 32 x 64 coefficients, 8 per thread; a row's coefficients are contiguous.
+#pragma unroll
+    for (int e = 0; e < (KS_CT * KS_CHUNK) / 256; e++) {
+      int idx = e * 256 + t;
+      int c = idx / KS_CHUNK, ii = idx % KS_CHUNK;
+      uint64_t x = 0;
+      if (c0 + c < nct && i0 + ii < a.kN) x = base[(size_t)jobs[c0 + c].in_row * blen + i0 + ii];
+      int64_t d[KSL];
+      pbs::decompose<KSL>(x, a.base_log, d);
+#pragma unroll
+      for (int l = 0; l < KSL; l++) dig8[(ii * KSL + l) * KS_CT + c] = (int8_t)d[l];
+    }
+    __syncthreads();
+    if (j < n1) {
+      const int iimax = (a.kN - i0) < KS_CHUNK ? (a.kN - i0) : KS_CHUNK;
+      for (int ii = 0; ii < iimax; ii++) {
+#pragma unroll
+        for (int l = 0; l < KSL; l++) {
+          int32_t kv = (int32_t)ksk[((size_t)(i0 + ii) * KSL + l) * n1 + j];
+#pragma unroll
+          for (int c4 = 0; c4 < KS_CT / 4; c4++) {
+            int32_t w = dig[ii][l][c4];
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+              int32_t dv = (int32_t)(int8_t)(w >> (8 * e));
+              acc[c4 * 4 + e] += __mul24(dv, kv);
+            }
+          }
+        }
+      }
+    }
+  }
+  if (j < n1) {
+#pragma unroll
+    for (int c = 0; c < KS_CT; c++) {
+      if (c0 + c >= nct) break;
+      uint32_t body = 0;
+      if (j == a.n_lwe) body = pbs::ks_body(base[(size_t)jobs[c0 + c].in_row * blen + a.kN], a.mod_log);
+      lwe2n[(size_t)(c0 + c) * a.lwe_stride + j] = (uint16_t)pbs::ks_finish((uint32_t)acc[c], body, a.mod_log, a.lc);
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// Blind rotation + sample extract.  Persistent: workgroup b handles ciphertexts
+// b, b + gridDim.x, ...  The CMUX order i = 0..n-1 is fixed per ciphertext, so the
+// result does not depend on batch composition.
+// ------------------------------------------------------------------------------------
+template <int K1, int L>
+__global__ __launch_bounds__(256) void k_blind_rotate(const uint64_t* __restrict__ bsk,
+                                                      const uint64_t* __restrict__ luts,
+                                                      const uint16_t* __restrict__ lwe2n,
+                                                      const PbsJob* __restrict__ jobs, int nct,
+                                                      uint64_t* __restrict__ out_base,
+                                                      const uint64_t* __restrict__ tw,
+                                                      const uint64_t* __restrict__ itw, BrArgs a) {
+  constexpr int NPF = K1 * L;
+  __shared__ uint64_t lds[NPF * N];
+  const int t = threadIdx.x;
+  const int r = t & 63, m0 = t >> 6;
+  const size_t blen = (size_t)(K1 - 1) * N + 1;
+  const size_t bsk_stride = (size_t)8 * NPF * K1 * T;
+  for (int c = blockIdx.x; c < nct; c += gridDim.x) {
+    const PbsJob job = jobs[c];
+    const uint16_t* a2n = lwe2n + (size_t)c * a.lwe_stride;
+    uint64_t acc[K1][8];
+    const uint64_t* lut = luts + (size_t)job.gid * K1 * N;
+#pragma unroll
+    for (int g = 0; g < K1; g++)
+#pragma unroll
+      for (int m1 = 0; m1 < 8; m1++) acc[g][m1] = lut[g * N + 64 * (m0 + 4 * m1) + r];
+    uint64_t v[NPF][8];
+    for (int i = 0; i < a.n_lwe; i++) {
+      const uint32_t rot = a2n[i];
+      const uint64_t* bsk_i = bsk + (size_t)i * bsk_stride;
+      // Opaque copy of the thread id: keeps the compiler from hoisting every derived
+      // shift amount, twiddle and LDS address out of the CMUX loop (which would hold
+      // hundreds of registers live); they are recomputed per pass instead.
+      int tt = t;
+      asm volatile("" : "+v"(tt));
+      const int rr = tt & 63, mm0 = tt >> 6;
+      __syncthreads();
+#pragma unroll
+      for (int g = 0; g < K1; g++)
+#pragma unroll
+        for (int m1 = 0; m1 < 8; m1++) lds[g * N + 64 * (mm0 + 4 * m1) + rr] = acc[g][m1];
+      __syncthreads();
+#pragma unroll
+      for (int g = 0; g < K1; g++) pbs::rotate_decompose<L>(lds + g * N, acc[g], tt, rot, a.base_log, &v[g * L]);
+      __syncthreads();
+      fwd_passes<NPF>(v, lds, tw, tt);
+      pbs::mac<K1, L>(v, bsk_i, tt);
+      inv_passes<K1>(v, lds, itw, tt);
+#pragma unroll
+      for (int g = 0; g < K1; g++)
+#pragma unroll
+        for (int m1 = 0; m1 < 8; m1++) acc[g][m1] += gl::modswitch_p_to_q(v[g][m1]);
+    }
+    // sample extract
+    __syncthreads();
+#pragma unroll
+    for (int g = 0; g < K1; g++)
+#pragma unroll
+      for (int m1 = 0; m1 < 8; m1++) lds[g * N + 64 * (m0 + 4 * m1) + r] = acc[g][m1];
+    __syncthreads();
+    const uint32_t b = a2n[a.n_lwe];
+    const int lut_nb = 1 << job.log_lut_nb;
+    for (int lu = 0; lu < lut_nb; lu++) {
+      const uint32_t offs = (uint32_t)(lu * (N >> job.log_lut_nb));
+      uint64_t* o = out_base + (size_t)(job.out_row + lu) * blen;
+#pragma unroll
+      for (int g = 0; g < K1 - 1; g++)
+#pragma unroll
+        for (int e = 0; e < N / T; e++) {
+          int m = t + T * e;
+          o[g * N + m] = pbs::extract_coef(lds, g, m, b, offs);
+        }
+      if (t == 0) o[(K1 - 1) * N] = pbs::extract_coef(lds, K1 - 1, 0, b, offs);
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// ALU DOps (pea_alu_core.sv:55-160) and LD/ST row copies.
+// ------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_alu(uint64_t* __restrict__ base, const AluOp* __restrict__ ops, int blen) {
+  const AluOp op = ops[blockIdx.y];
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= blen) return;
+  const uint64_t x = base[(size_t)op.src0_row * blen + e];
+  const uint64_t y = (op.opcode == 0x01 || op.opcode == 0x02 || op.opcode == 0x05)
+                         ? base[(size_t)op.src1_row * blen + e] : 0;
+  const uint64_t cst = (e == blen - 1) ? ((uint64_t)(op.msg_cst & 31u) << 59) : 0;
+  uint64_t z;
+  switch (op.opcode) {
+    case 0x01: z = x + y; break;
+    case 0x02: z = x - y; break;
+    case 0x05: z = x * (uint64_t)(op.mul_factor & 31u) + y; break;
+    case 0x09: z = x + cst; break;
+    case 0x0A: z = x - cst; break;
+    case 0x0B: z = cst - x; break;
+    default: z = x * (uint64_t)(op.msg_cst & 31u); break;  // 0x0C MULS
+  }
+  // Reads of all ops of a launch happen before writes only when rows do not alias
+  // across ops; the scheduler guarantees that (one dependency level per launch).
+  base[(size_t)op.dst_row * blen + e] = z;
+}
+
+__global__ __launch_bounds__(256) void k_copy(uint64_t* __restrict__ base, const CopyOp* __restrict__ ops, int blen) {
+  const CopyOp op = ops[blockIdx.y];
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e < blen) base[(size_t)op.dst_row * blen + e] = base[(size_t)op.src_row * blen + e];
+}
+
+// LUT ids come from caller device memory: ids beyond the LUT memory are clamped to its
+// last slot so that a bad id can never read outside the allocation.
+__global__ void k_make_jobs(PbsJob* jobs, const uint32_t* gid, int n, uint32_t log_lut_nb, uint32_t lut_slots) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < n) {
+    uint32_t g = gid ? gid[c] : 0u;
+    g = g < lut_slots ? g : lut_slots - 1;
+    jobs[c] = PbsJob{(uint32_t)c, (uint32_t)c << log_lut_nb, g, log_lut_nb};
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// Host launchers
+// ------------------------------------------------------------------------------------
+hipError_t launch_ntt(int mode, const uint64_t* a, const uint64_t* bn, uint64_t* out, size_t npoly,
+                      const uint64_t* tw, const uint64_t* itw, hipStream_t s) {
+  constexpr int NP = 2;
+  if (npoly == 0) return hipSuccess;
+  dim3 grid((unsigned)((npoly + NP - 1) / NP));
+  if (mode == 0)
+    k_ntt<NP, 0><<<grid, 256, 0, s>>>(a, bn, out, npoly, tw, itw);
+  else if (mode == 1)
+    k_ntt<NP, 1><<<grid, 256, 0, s>>>(a, bn, out, npoly, tw, itw);
+  else
+    k_ntt<NP, 2><<<grid, 256, 0, s>>>(a, bn, out, npoly, tw, itw);
+  return hipGetLastError();
+}
+
+hipError_t launch_bsk_prepare(const uint64_t* src, uint64_t* dst, int n_lwe, int rows, int K1, int is_ntt,
+                              const uint64_t* tw, hipStream_t s) {
+  k_bsk_prepare<<<dim3((unsigned)((size_t)n_lwe * rows * K1)), 256, 0, s>>>(src, dst, rows, K1, is_ntt, tw);
+  return hipGetLastError();
+}
+
+hipError_t launch_keyswitch(const uint64_t* base, const PbsJob* jobs, int nct, const uint32_t* ksk, uint16_t* lwe2n,
+                            const KsArgs& a, int ks_level, hipStream_t s) {
+  if (nct == 0) return hipSuccess;
+  dim3 grid((unsigned)((a.n_lwe + 1 + 255) / 256), (unsigned)((nct + KS_CT - 1) / KS_CT));
+  switch (ks_level) {
+#define KS_CASE(L) \
+  case L: k_keyswitch<L><<<grid, 256, 0, s>>>(base, jobs, nct, ksk, lwe2n, a); break;
+    KS_CASE(1) KS_CASE(2) KS_CASE(3) KS_CASE(4) KS_CASE(5) KS_CASE(6) KS_CASE(7) KS_CASE(8)
+#undef KS_CASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int K1, int L>
+static int br_max_grid() {
+  static int cached = 0;
+  if (!cached) {
+    int dev = 0, cus = 0, per = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_blind_rotate<K1, L>, 256, 0);
+    cached = (cus > 0 ? cus : 256) * (per > 0 ? per : 1);
+  }
+  return cached;
+}
+
+hipError_t launch_blind_rotate(const uint64_t* bsk, const uint64_t* luts, const uint16_t* lwe2n, const PbsJob* jobs,
+                               int nct, uint64_t* out_base, const uint64_t* tw, const uint64_t* itw, const BrArgs& a,
+                               int pbs_level, int grid_override, hipStream_t s) {
+  if (nct == 0) return hipSuccess;
+  if (pbs_level == 1) {
+    int g = grid_override > 0 ? grid_override : br_max_grid<2, 1>();
+    g = g < nct ? g : nct;
+    k_blind_rotate<2, 1><<<dim3(g), 256, 0, s>>>(bsk, luts, lwe2n, jobs, nct, out_base, tw, itw, a);
+  } else if (pbs_level == 2) {
+    int g = grid_override > 0 ? grid_override : br_max_grid<2, 2>();
+    g = g < nct ? g : nct;
+    k_blind_rotate<2, 2><<<dim3(g), 256, 0, s>>>(bsk, luts, lwe2n, jobs, nct, out_base, tw, itw, a);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+int blind_rotate_occupancy(int pbs_level) {
+  return pbs_level == 1 ? br_max_grid<2, 1>() : br_max_grid<2, 2>();
+}
+
+hipError_t launch_alu(uint64_t* base, const AluOp* ops, int nops, int blen, hipStream_t s) {
+  if (nops == 0) return hipSuccess;
+  k_alu<<<dim3((unsigned)((blen + 255) / 256), (unsigned)nops), 256, 0, s>>>(base, ops, blen);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy(uint64_t* base, const CopyOp* ops, int nops, int blen, hipStream_t s) {
+  if (nops == 0) return hipSuccess;
+  k_copy<<<dim3((unsigned)((blen + 255) / 256), (unsigned)nops), 256, 0, s>>>(base, ops, blen);
+  return hipGetLastError();
+}
+
+hipError_t launch_make_jobs(PbsJob* jobs, const uint32_t* gid, int n, uint32_t log_lut_nb, uint32_t lut_slots,
+                            hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  k_make_jobs<<<dim3((unsigned)((n + 255) / 256)), 256, 0, s>>>(jobs, gid, n, log_lut_nb, lut_slots);
+  return hipGetLastError();
+}
+
+}  // namespace lk

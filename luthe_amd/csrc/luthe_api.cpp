@@ -1,0 +1,611 @@
+// C ABI of the MI355X PBS engine: context, device memories, key upload, batched hot
+// path and the DOp/IOp front end.
+//
+// DOp execution model.  A submitted stream is validated whole, then scheduled by
+// dependency level: each DOp's level is one more than the latest level of any earlier
+// DOp it conflicts with (read-after-write, write-after-read, write-after-write on a
+// register or a ciphertext slot).  DOps of one level are independent, so a level runs
+// as at most three launches: all its LD/ST (one copy kernel), all its ALU DOps (one ALU
+// kernel) and all its PBS (one key-switch + one blind-rotation launch).  Each PBS is a
+// pure function of its source register and LUT and every ciphertext runs its CMUX
+// chain in the fixed order i = 0..n-1, so the results equal in-order execution and do
+// not depend on which other DOps share a batch.  This replaces the HPU instruction
+// scheduler's issue logic (hw/module/instruction_scheduler) and the BPIP batch former
+// (pep_sequencer.sv:584-608) with a host-side level schedule.
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <map>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/luthe.h"
+#include "dop.h"
+#include "kernels.h"
+#include "ntt2048.h"
+
+using lk::AluOp;
+using lk::CopyOp;
+using lk::PbsJob;
+
+struct Ticket {
+  hipEvent_t done = nullptr;
+};
+
+struct luthe_ctx {
+  luthe_params prm{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  size_t blen = 0;           // k*N + 1
+  size_t lut_len = 0;        // (k+1)*N
+  // device memories
+  uint64_t* d_mem = nullptr;  // [regfile_size + ct_slots] BLWE rows
+  uint64_t* d_luts = nullptr;
+  uint64_t* d_bsk = nullptr;
+  uint32_t* d_ksk = nullptr;
+  uint64_t* d_tw = nullptr;
+  uint64_t* d_itw = nullptr;
+  bool bsk_ok = false, ksk_ok = false;
+  // scratch (grow-only)
+  uint16_t* d_lwe2n = nullptr;
+  size_t lwe2n_cap = 0;
+  int lwe_stride = 0;
+  PbsJob* d_jobs = nullptr;
+  size_t jobs_cap = 0;
+  uint8_t* d_stage = nullptr;  // per-submission descriptors
+  size_t stage_cap = 0;
+  uint8_t* h_stage = nullptr;  // pinned mirror
+  size_t h_stage_cap = 0;
+  hipEvent_t stage_copied = nullptr;
+  uint64_t dev_bytes = 0;
+  luthe_counters cnt{};
+  uint64_t next_ticket = 1;
+  std::map<uint64_t, Ticket> tickets;
+  std::map<uint32_t, std::vector<uint32_t>> iop_table;  // key = max_align << 8 | opcode
+  int br_grid_override = 0;
+};
+
+namespace {
+
+int fail(luthe_ctx* c, int code, const char* fmt, ...) {
+  if (c) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    c->err = buf;
+  }
+  return code;
+}
+
+#define HIPCHK(c, expr)                                                                    \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess) return fail((c), LUTHE_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+int dev_alloc(luthe_ctx* c, void** p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) return fail(c, LUTHE_ERR_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+  c->dev_bytes += bytes;
+  return LUTHE_OK;
+}
+
+int check_params(const luthe_params* p, std::string& why) {
+  if (p->glwe_k != 1) return why = "glwe_k must be 1", -1;
+  if (p->poly_size != 2048) return why = "poly_size must be 2048", -1;
+  if (p->lwe_dim < 1 || p->lwe_dim > 4095) return why = "lwe_dim out of range [1, 4095]", -1;
+  if (p->pbs_level < 1 || p->pbs_level > 2) return why = "pbs_level must be 1 or 2", -1;
+  if (p->pbs_base_log < 1 || p->pbs_base_log * p->pbs_level >= 64) return why = "pbs_base_log*pbs_level must be < 64", -1;
+  if (p->ks_level < 1 || p->ks_level > 8) return why = "ks_level must be in [1, 8]", -1;
+  if (p->ks_base_log < 1 || p->ks_base_log > 8) return why = "ks_base_log must be in [1, 8]", -1;
+  if (p->ks_base_log * p->ks_level >= 64) return why = "ks_base_log*ks_level must be < 64", -1;
+  if (p->ks_mod_log < 13 || p->ks_mod_log > 32) return why = "ks_mod_log must be in [13, 32]", -1;
+  if (p->msg_bits + p->carry_bits < 1 || p->msg_bits + p->carry_bits > 8) return why = "msg_bits+carry_bits in [1, 8]", -1;
+  if (p->regfile_size < 1 || p->regfile_size > 128) return why = "regfile_size in [1, 128]", -1;
+  if (p->ct_slots > 65536) return why = "ct_slots <= 65536", -1;
+  if (p->lut_slots < 1 || p->lut_slots > 4096) return why = "lut_slots in [1, 4096]", -1;
+  return 0;
+}
+
+int ensure_lwe2n(luthe_ctx* c, size_t n) {
+  if (n <= c->lwe2n_cap) return LUTHE_OK;
+  size_t cap = std::max(n, c->lwe2n_cap * 2);
+  if (c->d_lwe2n) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipFree(c->d_lwe2n));
+    c->dev_bytes -= c->lwe2n_cap * c->lwe_stride * sizeof(uint16_t);
+  }
+  int rc = dev_alloc(c, (void**)&c->d_lwe2n, cap * c->lwe_stride * sizeof(uint16_t));
+  if (rc) return rc;
+  c->lwe2n_cap = cap;
+  return LUTHE_OK;
+}
+
+int ensure_jobs(luthe_ctx* c, size_t n) {
+  if (n <= c->jobs_cap) return LUTHE_OK;
+  size_t cap = std::max(n, c->jobs_cap * 2);
+  if (c->d_jobs) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipFree(c->d_jobs));
+    c->dev_bytes -= c->jobs_cap * sizeof(PbsJob);
+  }
+  int rc = dev_alloc(c, (void**)&c->d_jobs, cap * sizeof(PbsJob));
+  if (rc) return rc;
+  c->jobs_cap = cap;
+  return LUTHE_OK;
+}
+
+// Staging of per-submission descriptors: host pinned buffer -> device buffer.
+int stage_upload(luthe_ctx* c, const std::vector<uint8_t>& bytes) {
+  size_t n = bytes.size();
+  if (n == 0) return LUTHE_OK;
+  // the pinned buffer may still be read by the previous submission's copy
+  HIPCHK(c, hipEventSynchronize(c->stage_copied));
+  if (n > c->h_stage_cap) {
+    if (c->h_stage) HIPCHK(c, hipHostFree(c->h_stage));
+    size_t cap = std::max(n, c->h_stage_cap * 2);
+    HIPCHK(c, hipHostMalloc((void**)&c->h_stage, cap, hipHostMallocDefault));
+    c->h_stage_cap = cap;
+  }
+  if (n > c->stage_cap) {
+    if (c->d_stage) {
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      HIPCHK(c, hipFree(c->d_stage));
+      c->dev_bytes -= c->stage_cap;
+    }
+    size_t cap = std::max(n, c->stage_cap * 2);
+    int rc = dev_alloc(c, (void**)&c->d_stage, cap);
+    if (rc) return rc;
+    c->stage_cap = cap;
+  }
+  memcpy(c->h_stage, bytes.data(), n);
+  HIPCHK(c, hipMemcpyAsync(c->d_stage, c->h_stage, n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipEventRecord(c->stage_copied, c->stream));
+  return LUTHE_OK;
+}
+
+hipStream_t pick_stream(luthe_ctx* c, void* s) { return s ? (hipStream_t)s : c->stream; }
+
+// KS + BR over jobs already on the device.
+int run_pbs(luthe_ctx* c, const uint64_t* in_base, const PbsJob* d_jobs, int n, uint64_t* out_base, hipStream_t s) {
+  if (!c->bsk_ok || !c->ksk_ok) return fail(c, LUTHE_ERR_STATE, "keys not uploaded");
+  if (n == 0) return LUTHE_OK;
+  int rc = ensure_lwe2n(c, (size_t)n);
+  if (rc) return rc;
+  const luthe_params& p = c->prm;
+  lk::KsArgs ka{(int)(p.glwe_k * p.poly_size), (int)p.lwe_dim, c->lwe_stride, p.ks_base_log, p.ks_mod_log, 12};
+  HIPCHK(c, lk::launch_keyswitch(in_base, d_jobs, n, c->d_ksk, c->d_lwe2n, ka, (int)p.ks_level, s));
+  lk::BrArgs ba{(int)p.lwe_dim, c->lwe_stride, p.pbs_base_log};
+  HIPCHK(c, lk::launch_blind_rotate(c->d_bsk, c->d_luts, c->d_lwe2n, d_jobs, n, out_base, c->d_tw, c->d_itw, ba,
+                                    (int)p.pbs_level, c->br_grid_override, s));
+  c->cnt.pbs += (uint64_t)n;
+  c->cnt.pbs_batches += 1;
+  c->cnt.pbs_max_batch = std::max<uint64_t>(c->cnt.pbs_max_batch, (uint64_t)n);
+  return LUTHE_OK;
+}
+
+}  // namespace
+
+// ===================================================================================
+extern "C" {
+
+void luthe_params_default(luthe_params* p) {
+  if (!p) return;
+  memset(p, 0, sizeof *p);
+  p->lwe_dim = 724;
+  p->glwe_k = 1;
+  p->poly_size = 2048;
+  p->pbs_base_log = 23;
+  p->pbs_level = 1;
+  p->ks_base_log = 3;
+  p->ks_level = 5;
+  p->ks_mod_log = 21;
+  p->msg_bits = 2;
+  p->carry_bits = 2;
+  p->lwe_noise_std = 1.0 / 32768.0;            // 2^-15
+  p->glwe_noise_std = 1.0 / 2251799813685248.0;  // 2^-51
+  p->regfile_size = 64;
+  p->ct_slots = 4096;
+  p->lut_slots = 256;
+}
+
+int luthe_create(const luthe_params* p, int device, luthe_ctx** out) {
+  if (!p || !out) return LUTHE_ERR_ARG;
+  *out = nullptr;
+  std::string why;
+  if (check_params(p, why)) return LUTHE_ERR_PARAMS;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) return LUTHE_ERR_NODEV;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return LUTHE_ERR_NODEV;
+  if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) return LUTHE_ERR_NODEV;
+  luthe_ctx* c = new luthe_ctx();
+  c->prm = *p;
+  c->device = device;
+  c->blen = (size_t)p->glwe_k * p->poly_size + 1;
+  c->lut_len = (size_t)(p->glwe_k + 1) * p->poly_size;
+  c->lwe_stride = (int)((p->lwe_dim + 1 + 7) & ~7u);
+  int rc = LUTHE_OK;
+  auto bail = [&](int code) {
+    luthe_destroy(c);
+    return code;
+  };
+  if (hipSetDevice(device) != hipSuccess) return bail(LUTHE_ERR_HIP);
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail(LUTHE_ERR_HIP);
+  if (hipEventCreateWithFlags(&c->stage_copied, hipEventDisableTiming) != hipSuccess) return bail(LUTHE_ERR_HIP);
+  size_t rows = (size_t)p->regfile_size + p->ct_slots;
+  if ((rc = dev_alloc(c, (void**)&c->d_mem, rows * c->blen * 8))) return bail(rc);
+  if ((rc = dev_alloc(c, (void**)&c->d_luts, (size_t)p->lut_slots * c->lut_len * 8))) return bail(rc);
+  if (hipMemset(c->d_mem, 0, rows * c->blen * 8) != hipSuccess) return bail(LUTHE_ERR_HIP);
+  if (hipMemset(c->d_luts, 0, (size_t)p->lut_slots * c->lut_len * 8) != hipSuccess) return bail(LUTHE_ERR_HIP);
+  std::vector<uint64_t> tw(2048), itw(2048);
+  ntt::make_twist_tables(tw.data(), itw.data());
+  if ((rc = dev_alloc(c, (void**)&c->d_tw, 2048 * 8))) return bail(rc);
+  if ((rc = dev_alloc(c, (void**)&c->d_itw, 2048 * 8))) return bail(rc);
+  if (hipMemcpy(c->d_tw, tw.data(), 2048 * 8, hipMemcpyHostToDevice) != hipSuccess) return bail(LUTHE_ERR_HIP);
+  if (hipMemcpy(c->d_itw, itw.data(), 2048 * 8, hipMemcpyHostToDevice) != hipSuccess) return bail(LUTHE_ERR_HIP);
+  *out = c;
+  return LUTHE_OK;
+}
+
+int luthe_destroy(luthe_ctx* c) {
+  if (!c) return LUTHE_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& kv : c->tickets)
+    if (kv.second.done) (void)hipEventDestroy(kv.second.done);
+  void* bufs[] = {c->d_mem, c->d_luts, c->d_bsk, c->d_ksk, c->d_tw, c->d_itw, c->d_lwe2n, c->d_jobs, c->d_stage};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  if (c->h_stage) (void)hipHostFree(c->h_stage);
+  if (c->stage_copied) (void)hipEventDestroy(c->stage_copied);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return LUTHE_OK;
+}
+
+const char* luthe_last_error(const luthe_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int luthe_get_params(const luthe_ctx* c, luthe_params* out) {
+  if (!c || !out) return LUTHE_ERR_ARG;
+  *out = c->prm;
+  return LUTHE_OK;
+}
+
+int luthe_get_counters(const luthe_ctx* c, luthe_counters* out) {
+  if (!c || !out) return LUTHE_ERR_ARG;
+  *out = c->cnt;
+  return LUTHE_OK;
+}
+
+uint64_t luthe_device_bytes(const luthe_ctx* c) { return c ? c->dev_bytes : 0; }
+
+void* luthe_stream(luthe_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int luthe_set_br_grid(luthe_ctx* c, int workgroups) {
+  if (!c || workgroups < 0) return LUTHE_ERR_ARG;
+  c->br_grid_override = workgroups;
+  return LUTHE_OK;
+}
+
+int luthe_get_br_grid(luthe_ctx* c) {
+  if (!c) return LUTHE_ERR_ARG;
+  if (c->br_grid_override) return c->br_grid_override;
+  (void)hipSetDevice(c->device);
+  return lk::blind_rotate_occupancy((int)c->prm.pbs_level);
+}
+
+int luthe_sync(luthe_ctx* c) {
+  if (!c) return LUTHE_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return LUTHE_OK;
+}
+
+// ---------------------------------------------------------------- keys and memories
+int luthe_upload_bsk(luthe_ctx* c, const uint64_t* bsk, int layout) {
+  if (!c || !bsk) return LUTHE_ERR_ARG;
+  if (layout != LUTHE_BSK_COEF && layout != LUTHE_BSK_NTT) return fail(c, LUTHE_ERR_ARG, "bad BSK layout %d", layout);
+  const luthe_params& p = c->prm;
+  const int K1 = (int)p.glwe_k + 1, rows = K1 * (int)p.pbs_level;
+  size_t elems = (size_t)p.lwe_dim * rows * K1 * p.poly_size;
+  HIPCHK(c, hipSetDevice(c->device));
+  if (!c->d_bsk) {
+    int rc = dev_alloc(c, (void**)&c->d_bsk, elems * 8);
+    if (rc) return rc;
+  }
+  uint64_t* tmp = nullptr;
+  int rc = dev_alloc(c, (void**)&tmp, elems * 8);
+  if (rc) return rc;
+  hipError_t e = hipMemcpyAsync(tmp, bsk, elems * 8, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess)
+    e = lk::launch_bsk_prepare(tmp, c->d_bsk, (int)p.lwe_dim, rows, K1, layout == LUTHE_BSK_NTT, c->d_tw, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(tmp);
+  c->dev_bytes -= elems * 8;
+  if (e != hipSuccess) return fail(c, LUTHE_ERR_HIP, "BSK upload: %s", hipGetErrorString(e));
+  c->bsk_ok = true;
+  return LUTHE_OK;
+}
+
+int luthe_upload_ksk(luthe_ctx* c, const uint32_t* ksk) {
+  if (!c || !ksk) return LUTHE_ERR_ARG;
+  const luthe_params& p = c->prm;
+  size_t elems = (size_t)p.glwe_k * p.poly_size * p.ks_level * (p.lwe_dim + 1);
+  HIPCHK(c, hipSetDevice(c->device));
+  if (!c->d_ksk) {
+    int rc = dev_alloc(c, (void**)&c->d_ksk, elems * 4);
+    if (rc) return rc;
+  }
+  HIPCHK(c, hipMemcpy(c->d_ksk, ksk, elems * 4, hipMemcpyHostToDevice));
+  c->ksk_ok = true;
+  return LUTHE_OK;
+}
+
+static int rows_io(luthe_ctx* c, uint64_t* dbase, uint32_t first, uint32_t count, uint32_t limit, const uint64_t* hin,
+                   uint64_t* hout, size_t len, const char* what) {
+  if (!c || (!hin && !hout)) return LUTHE_ERR_ARG;
+  if ((uint64_t)first + count > limit) return fail(c, LUTHE_ERR_RANGE, "%s %u+%u out of range %u", what, first, count, limit);
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (hin)
+    HIPCHK(c, hipMemcpy(dbase + (size_t)first * len, hin, (size_t)count * len * 8, hipMemcpyHostToDevice));
+  else
+    HIPCHK(c, hipMemcpy(hout, dbase + (size_t)first * len, (size_t)count * len * 8, hipMemcpyDeviceToHost));
+  return LUTHE_OK;
+}
+
+int luthe_write_lut(luthe_ctx* c, uint32_t gid, const uint64_t* glwe, uint32_t count) {
+  if (!c) return LUTHE_ERR_ARG;
+  return rows_io(c, c->d_luts, gid, count, c->prm.lut_slots, glwe, nullptr, c->lut_len, "lut");
+}
+int luthe_write_ct(luthe_ctx* c, uint32_t cid, const uint64_t* blwe, uint32_t count) {
+  if (!c) return LUTHE_ERR_ARG;
+  return rows_io(c, c->d_mem + (size_t)c->prm.regfile_size * c->blen, cid, count, c->prm.ct_slots, blwe, nullptr,
+                 c->blen, "ct slot");
+}
+int luthe_read_ct(luthe_ctx* c, uint32_t cid, uint64_t* blwe, uint32_t count) {
+  if (!c) return LUTHE_ERR_ARG;
+  return rows_io(c, c->d_mem + (size_t)c->prm.regfile_size * c->blen, cid, count, c->prm.ct_slots, nullptr, blwe,
+                 c->blen, "ct slot");
+}
+int luthe_write_reg(luthe_ctx* c, uint32_t rid, const uint64_t* blwe, uint32_t count) {
+  if (!c) return LUTHE_ERR_ARG;
+  return rows_io(c, c->d_mem, rid, count, c->prm.regfile_size, blwe, nullptr, c->blen, "register");
+}
+int luthe_read_reg(luthe_ctx* c, uint32_t rid, uint64_t* blwe, uint32_t count) {
+  if (!c) return LUTHE_ERR_ARG;
+  return rows_io(c, c->d_mem, rid, count, c->prm.regfile_size, nullptr, blwe, c->blen, "register");
+}
+
+// ---------------------------------------------------------------- batched hot path
+int luthe_keyswitch_batch(luthe_ctx* c, const uint64_t* d_blwe, uint16_t* d_lwe2n, size_t n, void* stream) {
+  if (!c || (!d_blwe && n) || (!d_lwe2n && n)) return LUTHE_ERR_ARG;
+  if (!c->ksk_ok) return fail(c, LUTHE_ERR_STATE, "KSK not uploaded");
+  if (n == 0) return LUTHE_OK;
+  hipStream_t s = pick_stream(c, stream);
+  int rc = ensure_jobs(c, n);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, lk::launch_make_jobs(c->d_jobs, nullptr, (int)n, 0, c->prm.lut_slots, s));
+  const luthe_params& p = c->prm;
+  lk::KsArgs ka{(int)(p.glwe_k * p.poly_size), (int)p.lwe_dim, (int)p.lwe_dim + 1, p.ks_base_log, p.ks_mod_log, 12};
+  HIPCHK(c, lk::launch_keyswitch(d_blwe, c->d_jobs, (int)n, c->d_ksk, d_lwe2n, ka, (int)p.ks_level, s));
+  return LUTHE_OK;
+}
+
+int luthe_blind_rotate_batch(luthe_ctx* c, const uint16_t* d_lwe2n, uint64_t* d_out, const uint32_t* d_gid,
+                             uint32_t log_lut_nb, size_t n, void* stream) {
+  if (!c || (n && (!d_lwe2n || !d_out || !d_gid))) return LUTHE_ERR_ARG;
+  if (!c->bsk_ok) return fail(c, LUTHE_ERR_STATE, "BSK not uploaded");
+  if (log_lut_nb > 3) return fail(c, LUTHE_ERR_ARG, "log_lut_nb must be <= 3");
+  if (n == 0) return LUTHE_OK;
+  hipStream_t s = pick_stream(c, stream);
+  int rc = ensure_jobs(c, n);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, lk::launch_make_jobs(c->d_jobs, d_gid, (int)n, log_lut_nb, c->prm.lut_slots, s));
+  const luthe_params& p = c->prm;
+  lk::BrArgs ba{(int)p.lwe_dim, (int)p.lwe_dim + 1, p.pbs_base_log};
+  HIPCHK(c, lk::launch_blind_rotate(c->d_bsk, c->d_luts, d_lwe2n, c->d_jobs, (int)n, d_out, c->d_tw, c->d_itw, ba,
+                                    (int)p.pbs_level, c->br_grid_override, s));
+  return LUTHE_OK;
+}
+
+int luthe_pbs_batch(luthe_ctx* c, const uint64_t* d_in, uint64_t* d_out, const uint32_t* d_gid, uint32_t log_lut_nb,
+                    size_t n, void* stream) {
+  if (!c || (n && (!d_in || !d_out || !d_gid))) return LUTHE_ERR_ARG;
+  if (log_lut_nb > 3) return fail(c, LUTHE_ERR_ARG, "log_lut_nb must be <= 3");
+  if (n == 0) return LUTHE_OK;
+  if (n > (size_t)1 << 28) return fail(c, LUTHE_ERR_ARG, "batch too large");
+  hipStream_t s = pick_stream(c, stream);
+  HIPCHK(c, hipSetDevice(c->device));
+  int rc = ensure_jobs(c, n);
+  if (rc) return rc;
+  HIPCHK(c, lk::launch_make_jobs(c->d_jobs, d_gid, (int)n, log_lut_nb, c->prm.lut_slots, s));
+  return run_pbs(c, d_in, c->d_jobs, (int)n, d_out, s);
+}
+
+int luthe_ntt_polymul_batch(luthe_ctx* c, const uint64_t* d_a, const uint64_t* d_bn, uint64_t* d_out, size_t npoly,
+                            void* stream) {
+  if (!c || (npoly && (!d_a || !d_bn || !d_out))) return LUTHE_ERR_ARG;
+  HIPCHK(c, lk::launch_ntt(2, d_a, d_bn, d_out, npoly, c->d_tw, c->d_itw, pick_stream(c, stream)));
+  return LUTHE_OK;
+}
+int luthe_ntt_forward_batch(luthe_ctx* c, const uint64_t* d_in, uint64_t* d_out, size_t npoly, void* stream) {
+  if (!c || (npoly && (!d_in || !d_out))) return LUTHE_ERR_ARG;
+  HIPCHK(c, lk::launch_ntt(0, d_in, nullptr, d_out, npoly, c->d_tw, c->d_itw, pick_stream(c, stream)));
+  return LUTHE_OK;
+}
+int luthe_ntt_inverse_batch(luthe_ctx* c, const uint64_t* d_in, uint64_t* d_out, size_t npoly, void* stream) {
+  if (!c || (npoly && (!d_in || !d_out))) return LUTHE_ERR_ARG;
+  HIPCHK(c, lk::launch_ntt(1, d_in, nullptr, d_out, npoly, c->d_tw, c->d_itw, pick_stream(c, stream)));
+  return LUTHE_OK;
+}
+
+// ---------------------------------------------------------------- DOp stream
+static int execute_program(luthe_ctx* c, const std::vector<dop::Dop>& prog, uint64_t* ticket) {
+  const luthe_params& p = c->prm;
+  const uint32_t R = p.regfile_size;
+  std::vector<int> level;
+  int nlev = dop::schedule_levels(prog, R, level);
+  // descriptors per level, packed into one staging blob
+  struct LevelPlan {
+    size_t copy_off = 0, alu_off = 0, pbs_off = 0;
+    int ncopy = 0, nalu = 0, npbs = 0;
+  };
+  std::vector<LevelPlan> plan(nlev);
+  std::vector<std::vector<CopyOp>> copies(nlev);
+  std::vector<std::vector<AluOp>> alus(nlev);
+  std::vector<std::vector<PbsJob>> pbss(nlev);
+  for (size_t i = 0; i < prog.size(); i++) {
+    const dop::Dop& d = prog[i];
+    int L = level[i];
+    if (L < 0) continue;  // SYNC
+    switch (d.kind) {
+      case dop::KIND_MEM:
+        if (d.opcode == dop::OP_LD)
+          copies[L].push_back(CopyOp{d.rid, R + d.slot});
+        else
+          copies[L].push_back(CopyOp{R + d.slot, d.rid});
+        break;
+      case dop::KIND_ARITH:
+        alus[L].push_back(AluOp{d.opcode, d.dst, d.src0, d.src1, d.mul_factor, d.msg_cst});
+        break;
+      case dop::KIND_PBS:
+        pbss[L].push_back(PbsJob{d.src0, d.dst, d.gid, d.log_lut_nb});
+        break;
+      default:
+        break;
+    }
+  }
+  std::vector<uint8_t> blob;
+  auto append = [&](const void* src, size_t bytes) {
+    size_t off = (blob.size() + 15) & ~(size_t)15;
+    blob.resize(off + bytes);
+    if (bytes) memcpy(blob.data() + off, src, bytes);
+    return off;
+  };
+  size_t maxp = 0;
+  for (int L = 0; L < nlev; L++) {
+    plan[L].ncopy = (int)copies[L].size();
+    plan[L].nalu = (int)alus[L].size();
+    plan[L].npbs = (int)pbss[L].size();
+    plan[L].copy_off = append(copies[L].data(), copies[L].size() * sizeof(CopyOp));
+    plan[L].alu_off = append(alus[L].data(), alus[L].size() * sizeof(AluOp));
+    plan[L].pbs_off = append(pbss[L].data(), pbss[L].size() * sizeof(PbsJob));
+    maxp = std::max(maxp, pbss[L].size());
+  }
+  if (maxp && (!c->bsk_ok || !c->ksk_ok)) return fail(c, LUTHE_ERR_STATE, "keys not uploaded");
+  HIPCHK(c, hipSetDevice(c->device));
+  int rc = stage_upload(c, blob);
+  if (rc) return rc;
+  const int blen = (int)c->blen;
+  for (int L = 0; L < nlev; L++) {
+    const LevelPlan& lp = plan[L];
+    if (lp.ncopy)
+      HIPCHK(c, lk::launch_copy(c->d_mem, (const CopyOp*)(c->d_stage + lp.copy_off), lp.ncopy, blen, c->stream));
+    if (lp.nalu)
+      HIPCHK(c, lk::launch_alu(c->d_mem, (const AluOp*)(c->d_stage + lp.alu_off), lp.nalu, blen, c->stream));
+    if (lp.npbs) {
+      rc = run_pbs(c, c->d_mem, (const PbsJob*)(c->d_stage + lp.pbs_off), lp.npbs, c->d_mem, c->stream);
+      if (rc) return rc;
+    }
+    c->cnt.alu += (uint64_t)lp.nalu;
+    c->cnt.mem += (uint64_t)lp.ncopy;
+  }
+  c->cnt.levels += (uint64_t)nlev;
+  c->cnt.dops += prog.size();
+  c->cnt.tickets += 1;
+  Ticket tk;
+  HIPCHK(c, hipEventCreateWithFlags(&tk.done, hipEventDisableTiming));
+  HIPCHK(c, hipEventRecord(tk.done, c->stream));
+  uint64_t id = c->next_ticket++;
+  c->tickets[id] = tk;
+  if (ticket) *ticket = id;
+  return LUTHE_OK;
+}
+
+int luthe_submit_dops(luthe_ctx* c, const uint32_t* words, size_t n, uint64_t* ticket) {
+  if (!c || (!words && n)) return LUTHE_ERR_ARG;
+  std::vector<dop::Dop> prog;
+  std::string why;
+  int rc = dop::decode_stream(words, n, c->prm.regfile_size, c->prm.ct_slots, c->prm.lut_slots, prog, why);
+  if (rc) return fail(c, rc, "%s", why.c_str());
+  return execute_program(c, prog, ticket);
+}
+
+int luthe_poll(luthe_ctx* c, uint64_t ticket) {
+  if (!c) return LUTHE_ERR_ARG;
+  auto it = c->tickets.find(ticket);
+  if (it == c->tickets.end()) return fail(c, LUTHE_ERR_STATE, "unknown ticket %llu", (unsigned long long)ticket);
+  hipError_t e = hipEventQuery(it->second.done);
+  if (e == hipErrorNotReady) return 0;
+  if (e != hipSuccess) return fail(c, LUTHE_ERR_HIP, "ticket: %s", hipGetErrorString(e));
+  (void)hipEventDestroy(it->second.done);
+  c->tickets.erase(it);
+  return 1;
+}
+
+int luthe_wait(luthe_ctx* c, uint64_t ticket, int64_t timeout_us) {
+  if (!c) return LUTHE_ERR_ARG;
+  auto it = c->tickets.find(ticket);
+  if (it == c->tickets.end()) return fail(c, LUTHE_ERR_STATE, "unknown ticket %llu", (unsigned long long)ticket);
+  if (timeout_us < 0) {
+    HIPCHK(c, hipEventSynchronize(it->second.done));
+  } else {
+    auto t0 = std::chrono::steady_clock::now();
+    for (;;) {
+      hipError_t e = hipEventQuery(it->second.done);
+      if (e == hipSuccess) break;
+      if (e != hipErrorNotReady) return fail(c, LUTHE_ERR_HIP, "ticket: %s", hipGetErrorString(e));
+      auto us = std::chrono::duration_cast<std::chrono::microseconds>(std::chrono::steady_clock::now() - t0).count();
+      if (us >= timeout_us) return fail(c, LUTHE_ERR_TIMEOUT, "ticket %llu not done after %lld us",
+                                       (unsigned long long)ticket, (long long)timeout_us);
+      std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+  }
+  (void)hipEventDestroy(it->second.done);
+  c->tickets.erase(it);
+  return LUTHE_OK;
+}
+
+// ---------------------------------------------------------------- IOp path
+int luthe_register_iop(luthe_ctx* c, uint8_t opcode, uint8_t max_align, const uint32_t* dops, size_t n) {
+  if (!c || (!dops && n)) return LUTHE_ERR_ARG;
+  std::string why;
+  int rc = dop::check_template(dops, n, c->prm.regfile_size, c->prm.lut_slots, why);
+  if (rc) return fail(c, rc, "IOp 0x%02x template: %s", opcode, why.c_str());
+  c->iop_table[((uint32_t)max_align << 8) | opcode] = std::vector<uint32_t>(dops, dops + n);
+  return LUTHE_OK;
+}
+
+int luthe_submit_iop(luthe_ctx* c, const uint32_t* words, size_t n, uint64_t* ticket) {
+  if (!c || !words || n == 0) return LUTHE_ERR_ARG;
+  dop::IopBundle iop;
+  std::string why;
+  int rc = dop::parse_iop(words, n, iop, why);
+  if (rc) return fail(c, rc, "%s", why.c_str());
+  auto it = c->iop_table.find(((uint32_t)iop.max_align << 8) | iop.opcode);
+  if (it == c->iop_table.end())
+    return fail(c, LUTHE_ERR_DOP, "no DOp program registered for IOp 0x%02x align %u", iop.opcode, iop.max_align);
+  std::vector<uint32_t> patched;
+  rc = dop::patch_program(it->second, iop, c->prm.ct_slots, patched, why);
+  if (rc) return fail(c, rc, "%s", why.c_str());
+  std::vector<dop::Dop> prog;
+  rc = dop::decode_stream(patched.data(), patched.size(), c->prm.regfile_size, c->prm.ct_slots, c->prm.lut_slots, prog,
+                          why);
+  if (rc) return fail(c, rc, "%s", why.c_str());
+  c->cnt.iops += 1;
+  return execute_program(c, prog, ticket);
+}
+
+}  // extern "C"

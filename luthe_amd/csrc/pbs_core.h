@@ -1,0 +1,127 @@
+// Per-thread building blocks of the blind rotation, key switch and sample extract,
+// shared by the HIP kernels and by the sequential CPU emulation used in tests.
+//
+// Thread mapping follows ntt2048.h: in the coefficient domain thread (r, m0) owns
+// coefficients n = 64 (m0 + 4 m1) + r of every polynomial (m1 < 8); in the NTT domain
+// thread t = 8k + w1 owns indices 64k + w1 + 8 w0 (w0 < 8).
+#pragma once
+#include "gl64.h"
+#include "ntt2048.h"
+
+namespace pbs {
+
+constexpr int N = ntt::N;
+constexpr int T = ntt::T;
+
+GL_HD int coef_index(int t, int m1) { return 64 * ((t >> 6) + 4 * m1) + (t & 63); }
+
+// Balanced signed decomposition of a w-bit word into L digits of b bits, LSB level
+// first: closest representable value (top L*b bits + round bit), sign of the closest
+// representable, then the carry chain of decomp_parallel.sv:56-150 (same algorithm as
+// decomp_balseq_core.sv:10-173 and balanced_decomp_unfix_sign_round_bit.py:42-105).
+// Requires L*b < 64.  Digits are returned as signed ints.
+template <int L>
+GL_HD void decompose(uint64_t x, uint32_t b, int64_t* d) {
+  const uint32_t R = (uint32_t)L * b;
+  const uint32_t ofs = 64 - R;
+  const uint64_t bmask = (1ull << b) - 1;
+  const uint64_t hlow = (b >= 2) ? ((1ull << (b - 1)) - 1) : 0;
+  uint64_t rb = (x >> (ofs - 1)) & 1;
+  uint64_t top = x >> ofs;                                 // R bits
+  uint64_t closest = (top + rb) & ((1ull << R) - 1);
+  uint64_t msb = x >> 63;
+  uint64_t below_mask = (1ull << (R - 1)) - 1;
+  uint64_t below = top & below_mask;
+  uint64_t sign = (msb & (uint64_t)((below != 0) | rb)) | ((msb ^ 1) & (uint64_t)(below == below_mask) & rb);
+  uint64_t c = 0;
+#pragma unroll
+  for (int i = 0; i < L; i++) {
+    uint64_t u = (closest >> (i * b)) & bmask;
+    uint64_t un = (i + 1 < L) ? ((closest >> ((i + 1) * b)) & bmask) : (sign ? bmask : 0);
+    uint64_t half = (u >> (b - 1)) & 1;
+    uint64_t low = u & hlow;
+    uint64_t gt = half & (uint64_t)((low != 0) | c);
+    uint64_t ge = half | ((uint64_t)(low == hlow) & c);
+    uint64_t ov = (uint64_t)(u == bmask) & c;
+    uint64_t prop = ov | gt | (ge & ((un >> (b - 1)) & 1));
+    int64_t t = (int64_t)(u + c);
+    d[i] = prop ? t - (int64_t)(1ull << b) : t;
+    c = prop;
+  }
+}
+
+// ct10 = X^a * ACC - ACC for the thread's coefficients of one polynomial, decomposed
+// and lifted into Z_p: out[l][m1].  acc_lds is the polynomial in plain order.
+// Rotation semantics of pep_mmacc_splitc_feed_read.sv:484-488 and feed_final.sv:203-330:
+// coefficient n reads ACC[(n - a) mod 2N], negated when that index lands in [N, 2N).
+template <int L>
+GL_HD void rotate_decompose(const uint64_t* acc_lds, const uint64_t* acc_reg, int t, uint32_t a,
+                            uint32_t base_log, uint64_t (*out)[8]) {
+#pragma unroll
+  for (int m1 = 0; m1 < 8; m1++) {
+    int n = coef_index(t, m1);
+    uint32_t idx = ((uint32_t)n + 2 * N - a) & (2 * N - 1);
+    uint64_t val = acc_lds[idx & (N - 1)];
+    val = (idx >= (uint32_t)N) ? (0 - val) : val;
+    uint64_t ct10 = val - acc_reg[m1];
+    int64_t d[L];
+    decompose<L>(ct10, base_log, d);
+#pragma unroll
+    for (int l = 0; l < L; l++) out[l][m1] = gl::from_signed(d[l]);
+  }
+}
+
+// Pointwise external product for one thread: z[gl][w0] are the NTT-domain digits of
+// row gl = g*L + l; bsk_i is the key slice of this CMUX in the internal layout
+// [w0][K1*L][K1][T].  Result written over z[j][w0] for j < K1.
+template <int K1, int L>
+GL_HD void mac(uint64_t (*z)[8], const uint64_t* bsk_i, int t) {
+#pragma unroll
+  for (int w0 = 0; w0 < 8; w0++) {
+    uint64_t o[K1];
+#pragma unroll
+    for (int j = 0; j < K1; j++) {
+      uint64_t s = 0;
+#pragma unroll
+      for (int gl_ = 0; gl_ < K1 * L; gl_++) {
+        uint64_t b = bsk_i[((w0 * (K1 * L) + gl_) * K1 + j) * T + t];
+        s = gl::add(s, gl::mul(z[gl_][w0], b));
+      }
+      o[j] = s;
+    }
+#pragma unroll
+    for (int j = 0; j < K1; j++) z[j][w0] = o[j];
+  }
+}
+
+// Sample extract of one output coefficient with Many-LUT
+// (tb_pep_mmacc_sample_extract.sv:646-690): v = X^(-b) ACC; mask[g][m] = v_g[offs - m]
+// (negated across the wrap), body = v_k[offs].  acc_lds holds the K1 polynomials.
+GL_HD uint64_t extract_coef(const uint64_t* acc_lds, int g, int m, uint32_t b, uint32_t offs) {
+  int ci = (int)offs - m;
+  uint32_t x = (ci >= 0) ? (uint32_t)ci : (uint32_t)(N + ci);
+  uint64_t neg1 = (ci < 0);
+  uint32_t idx = (x + b) & (2 * N - 1);
+  uint64_t val = acc_lds[g * N + (idx & (N - 1))];
+  uint64_t neg2 = (idx >= (uint32_t)N);
+  return (neg1 ^ neg2) ? (0 - val) : val;
+}
+
+// Key-switch output stage (pep_ks_out_process.sv:276-282,336-341): out = [j==n] b' - acc
+// mod 2^w, then rounded to LWE_COEF_W bits.
+GL_HD uint32_t ks_finish(uint32_t acc, uint32_t body, uint32_t w, uint32_t lc) {
+  uint32_t wmask = (w >= 32) ? 0xFFFFFFFFu : ((1u << w) - 1);
+  uint32_t v = (body - acc) & wmask;
+  uint32_t top = v >> (w - lc);
+  uint32_t rb = (v >> (w - lc - 1)) & 1;
+  return (top + rb) & ((1u << lc) - 1);
+}
+
+// Body switch 2^64 -> 2^w (pep_ks_blram_format.sv:236-246).
+GL_HD uint32_t ks_body(uint64_t b, uint32_t w) {
+  uint64_t top = b >> (64 - w);
+  uint64_t rb = (b >> (64 - w - 1)) & 1;
+  return (uint32_t)((top + rb) & ((1ull << w) - 1));
+}
+
+}  // namespace pbs

@@ -88,3 +88,125 @@ uint64_t emu_gl_mul_pow2(uint64_t a, uint32_t s) { return gl::mul_pow2(a, s); }
 uint64_t emu_gl_reduce128(uint64_t lo, uint64_t hi) { return gl::reduce128(lo, hi); }
 uint64_t emu_modswitch(uint64_t e) { return gl::modswitch_p_to_q(e); }
 }
+
+#include "../../luthe_amd/csrc/pbs_core.h"
+
+namespace {
+// Internal BSK layout of the blind-rotation kernel: [i][w0][K1*L][K1][T].
+void bsk_to_internal(int n, int K1, int L, const uint64_t* bsk_coef, uint64_t* out) {
+  std::vector<uint64_t> z(N);
+  for (int i = 0; i < n; i++)
+    for (int gl_ = 0; gl_ < K1 * L; gl_++)
+      for (int j = 0; j < K1; j++) {
+        const uint64_t* src = bsk_coef + (((size_t)i * K1 * L + gl_) * K1 + j) * N;
+        emu_ntt_fwd(src, z.data());
+        for (int t = 0; t < T; t++) {
+          int k = t >> 3, w1 = t & 7;
+          for (int w0 = 0; w0 < 8; w0++)
+            out[(((size_t)i * 8 + w0) * (K1 * L) + gl_) * K1 * T + (size_t)j * T + t] = z[64 * k + w1 + 8 * w0];
+        }
+      }
+}
+
+template <int K1, int L>
+void blind_rotate(int n, const uint64_t* bsk_int, const uint64_t* lut, const uint16_t* a2n, uint32_t base_log,
+                  uint64_t* acc_out) {
+  constexpr int NPF = K1 * L;
+  const Tables& tb = tables();
+  std::vector<uint64_t> lds(NPF * N);
+  static uint64_t acc[T][K1][8];
+  static uint64_t v[T][NPF][8];
+  for (int t = 0; t < T; t++)
+    for (int g = 0; g < K1; g++)
+      for (int m1 = 0; m1 < 8; m1++) acc[t][g][m1] = lut[g * N + pbs::coef_index(t, m1)];
+  for (int i = 0; i < n; i++) {
+    uint32_t a = a2n[i];
+    for (int t = 0; t < T; t++)
+      for (int g = 0; g < K1; g++)
+        for (int m1 = 0; m1 < 8; m1++) lds[g * N + pbs::coef_index(t, m1)] = acc[t][g][m1];
+    for (int t = 0; t < T; t++)
+      for (int g = 0; g < K1; g++) pbs::rotate_decompose<L>(lds.data() + g * N, acc[t][g], t, a, base_log, &v[t][g * L]);
+    for (int t = 0; t < T; t++)
+      for (int p = 0; p < NPF; p++) fwd_a1_compute(v[t][p], t >> 6);
+    for (int t = 0; t < T; t++)
+      for (int p = 0; p < NPF; p++) fwd_a1_store(v[t][p], lds.data() + p * N, t & 63, t >> 6);
+    for (int t = 0; t < T; t++)
+      for (int p = 0; p < NPF; p++) {
+        fwd_a2_load(v[t][p], lds.data() + p * N, t & 63, t >> 6);
+        fwd_a2_compute(v[t][p], t & 63, t >> 6, tb.tw);
+      }
+    for (int t = 0; t < T; t++)
+      for (int p = 0; p < NPF; p++) fwd_a2_store(v[t][p], lds.data() + p * N, t & 63, t >> 6);
+    for (int t = 0; t < T; t++)
+      for (int p = 0; p < NPF; p++) {
+        fwd_c1_load(v[t][p], lds.data() + p * N, t >> 3, t & 7);
+        fwd_c1_compute(v[t][p], t & 7);
+      }
+    for (int t = 0; t < T; t++)
+      for (int p = 0; p < NPF; p++) fwd_c1_store(v[t][p], lds.data() + p * N, t >> 3, t & 7);
+    const uint64_t* bsk_i = bsk_int + (size_t)i * 8 * NPF * K1 * T;
+    for (int t = 0; t < T; t++) {
+      for (int p = 0; p < NPF; p++) {
+        fwd_c2_load(v[t][p], lds.data() + p * N, t >> 3, t & 7);
+        fwd_c2_compute(v[t][p]);
+      }
+      pbs::mac<K1, L>(v[t], bsk_i, t);
+      for (int j = 0; j < K1; j++) inv_c2_compute(v[t][j], t & 7);
+    }
+    for (int t = 0; t < T; t++)
+      for (int j = 0; j < K1; j++) inv_c2_store(v[t][j], lds.data() + j * N, t >> 3, t & 7);
+    for (int t = 0; t < T; t++)
+      for (int j = 0; j < K1; j++) {
+        inv_c1_load(v[t][j], lds.data() + j * N, t >> 3, t & 7);
+        inv_c1_compute(v[t][j], t >> 3, t & 7, tb.itw);
+      }
+    for (int t = 0; t < T; t++)
+      for (int j = 0; j < K1; j++) inv_c1_store(v[t][j], lds.data() + j * N, t >> 3, t & 7);
+    for (int t = 0; t < T; t++)
+      for (int j = 0; j < K1; j++) {
+        inv_a2_load(v[t][j], lds.data() + j * N, t & 63, t >> 6);
+        inv_a2_compute(v[t][j], t >> 6);
+      }
+    for (int t = 0; t < T; t++)
+      for (int j = 0; j < K1; j++) inv_a2_store(v[t][j], lds.data() + j * N, t & 63, t >> 6);
+    for (int t = 0; t < T; t++)
+      for (int j = 0; j < K1; j++) {
+        inv_a1_load(v[t][j], lds.data() + j * N, t & 63, t >> 6);
+        inv_a1_compute(v[t][j], t >> 6);
+        for (int m1 = 0; m1 < 8; m1++) acc[t][j][m1] += gl::modswitch_p_to_q(v[t][j][m1]);
+      }
+  }
+  for (int t = 0; t < T; t++)
+    for (int g = 0; g < K1; g++)
+      for (int m1 = 0; m1 < 8; m1++) acc_out[g * N + pbs::coef_index(t, m1)] = acc[t][g][m1];
+}
+}  // namespace
+
+extern "C" {
+void emu_bsk_to_internal(int n, int L, const uint64_t* bsk_coef, uint64_t* out) {
+  bsk_to_internal(n, 2, L, bsk_coef, out);
+}
+int emu_blind_rotate(int n, int L, const uint64_t* bsk_int, const uint64_t* lut, const uint16_t* a2n,
+                     uint32_t base_log, uint64_t* acc_out) {
+  if (L == 1)
+    blind_rotate<2, 1>(n, bsk_int, lut, a2n, base_log, acc_out);
+  else if (L == 2)
+    blind_rotate<2, 2>(n, bsk_int, lut, a2n, base_log, acc_out);
+  else
+    return -1;
+  return 0;
+}
+void emu_decompose(uint64_t x, uint32_t b, int L, int64_t* d) {
+  switch (L) {
+    case 1: pbs::decompose<1>(x, b, d); break;
+    case 2: pbs::decompose<2>(x, b, d); break;
+    case 3: pbs::decompose<3>(x, b, d); break;
+    case 4: pbs::decompose<4>(x, b, d); break;
+    case 5: pbs::decompose<5>(x, b, d); break;
+    default: break;
+  }
+}
+uint64_t emu_extract_coef(const uint64_t* acc, int g, int m, uint32_t b, uint32_t offs) {
+  return pbs::extract_coef(acc, g, m, b, offs);
+}
+}
