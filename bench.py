@@ -1,0 +1,211 @@
+"""Benchmark: batched PBS throughput on one MI355X.
+
+Workload (one "step"): one batched PBS (key switch + blind rotation + sample extract)
+over 8192 ciphertexts with the config-0 parameters (n=724, k=1, N=2048, PBS base 2^23
+level 1, KS base 2^3 level 5, w_ks=21), seeded keys, uniform 4-bit plaintexts, identity
+LUT; inputs resident in HBM before the timed region.  Beside it:
+  * the plain path: the same kernels with one ciphertext per launch;
+  * the blind-rotation kernel's own duration (HIP events on the engine stream) and its
+    roofline: algorithmic modmul-equivalents per launch / duration against the measured
+    full-occupancy rate of the same Goldilocks modmul primitive;
+  * cpu_baseline: the C oracle (oracle/luthe_ref.c, OpenMP over ciphertexts) timed on a
+    bounded sample of the same batch; its outputs are compared bit for bit with the GPU's.
+
+Prints one JSON line.  Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = "PBS/sec on 1 MI355X (batched vs plain), bit-exact; p50/p99 IOp latency"
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+class Timer:
+    """HIP events recorded on a given stream."""
+
+    def __init__(self, hip, stream):
+        self.hip, self.stream = hip, stream
+        self.e0, self.e1 = C.c_void_p(), C.c_void_p()
+        hip.hipEventCreate(C.byref(self.e0))
+        hip.hipEventCreate(C.byref(self.e1))
+
+    def start(self):
+        self.hip.hipEventRecord(self.e0, self.stream)
+
+    def stop_ms(self):
+        self.hip.hipEventRecord(self.e1, self.stream)
+        self.hip.hipEventSynchronize(self.e1)
+        ms = C.c_float(0)
+        self.hip.hipEventElapsedTime(C.byref(ms), self.e0, self.e1)
+        return ms.value
+
+
+def modmul_equiv_per_pbs(p):
+    """Algorithmic Goldilocks modmul-equivalents of one blind rotation: per CMUX,
+    (k+1)L forward + (k+1) inverse NTTs of 11,264 butterflies each, (k+1)^2 L pointwise
+    products per NTT-domain index, and (k+1)N p->2^64 switch multiplies."""
+    k1, L, N = p.glwe_k + 1, p.pbs_level, p.poly_size
+    per_cmux = (k1 * L + k1) * 11264 + k1 * L * k1 * N + k1 * N
+    return p.lwe_dim * per_cmux
+
+
+def bsk_bytes(p):
+    k1 = p.glwe_k + 1
+    return p.lwe_dim * k1 * p.pbs_level * k1 * p.poly_size * 8
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=8192)
+    ap.add_argument("--plain-reps", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--br-grid", type=int, default=0, help="blind-rotation workgroups (0 = occupancy x CUs)")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="only run a device-to-device copy of this size (HBM counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (this engine shards by replicating keys and splitting "
+              f"batches; no multi-GPU figure is produced): --gpus {args.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    import luthe_amd as la
+    hip = la.hip()
+
+    if args.calib_copy_mib:
+        nbytes = args.calib_copy_mib << 20
+        a = la.DeviceBuffer((nbytes // 8,), np.uint64)
+        b = la.DeviceBuffer((nbytes // 8,), np.uint64)
+        a.zero()
+        for _ in range(3):
+            hip.hipMemcpy(b.ptr, a.ptr, nbytes, 3)
+        hip.hipDeviceSynchronize()
+        print(json.dumps({"calib_copy_bytes": nbytes}))
+        return
+
+    prm = la.default_params()
+    log(f"bench: config-2 batch={args.batch} n={prm.lwe_dim} N={prm.poly_size}: keygen")
+    keys = la.Keys(prm, seed=1234)
+    ctx = la.Context(prm, device=0)
+    if args.br_grid:
+        ctx.set_br_grid(args.br_grid)
+    ctx.upload_bsk(keys.bsk, la.BSK_COEF)
+    ctx.upload_ksk(keys.ksk)
+    lut = la.lut_encode(prm, np.arange(16, dtype=np.uint8))
+    ctx.write_lut(0, lut)
+    rng = np.random.default_rng(99)
+    msgs = rng.integers(0, 16, args.batch)
+    cts = keys.encrypt_msg(msgs, seed=4321)
+    d_in = la.DeviceBuffer.from_numpy(cts)
+    d_gid = la.DeviceBuffer.from_numpy(np.zeros(args.batch, np.uint32))
+    d_out = la.DeviceBuffer((args.batch, prm.blen), np.uint64)
+    stream = ctx.stream
+    tm = Timer(hip, stream)
+
+    log(f"bench: warmup {args.warmup}")
+    for _ in range(args.warmup):
+        ctx.pbs_batch_dev(d_in, d_out, d_gid, args.batch)
+    ctx.sync()
+    log(f"bench: timed {args.steps} steps")
+    hip.hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    tm.start()
+    for _ in range(args.steps):
+        ctx.pbs_batch_dev(d_in, d_out, d_gid, args.batch)
+    ev_ms = tm.stop_ms()
+    hip.hipDeviceSynchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = ev_ms / args.steps
+    value = args.batch * args.steps / (ev_ms * 1e-3)
+    gpu_out = d_out.numpy()
+    dec = keys.decrypt_msg(gpu_out[:2048])
+    decrypt_ok = float((dec == msgs[:2048]).mean())
+    log(f"bench: {value:.1f} PBS/s ({ms_per_step:.2f} ms/step, wall {wall:.3f}s), decrypt rate {decrypt_ok:.4f}")
+
+    # dominant kernel alone: blind rotation over the batch's key-switched LWEs
+    d_lwe = la.DeviceBuffer((args.batch, prm.lwe_dim + 1), np.uint16)
+    la.lib().luthe_keyswitch_batch(ctx.handle, d_in.ptr, d_lwe.ptr, args.batch, None)
+    ctx.sync()
+    tm.start()
+    la.lib().luthe_keyswitch_batch(ctx.handle, d_in.ptr, d_lwe.ptr, args.batch, None)
+    ks_ms = tm.stop_ms()
+    br_ms = []
+    for _ in range(2):
+        tm.start()
+        rc = la.lib().luthe_blind_rotate_batch(ctx.handle, d_lwe.ptr, d_out.ptr, d_gid.ptr, 0, args.batch, None)
+        br_ms.append(tm.stop_ms())
+        assert rc == 0
+    br_ms = min(br_ms)
+    peak = ctx.modmul_peak()
+    work = modmul_equiv_per_pbs(prm) * args.batch
+    achieved = work / (br_ms * 1e-3) / 1e9
+    log(f"bench: blind rotation {br_ms:.2f} ms, key switch {ks_ms:.2f} ms, {achieved:.1f} Gmodmul/s of {peak:.1f}")
+
+    # plain path: one ciphertext per launch
+    plain_ms = []
+    for r in range(args.plain_reps):
+        tm.start()
+        ctx.pbs_batch_dev(d_in, d_out, d_gid, 1)
+        plain_ms.append(tm.stop_ms())
+    plain = 1000.0 / float(np.median(plain_ms))
+    log(f"bench: plain path {np.median(plain_ms):.2f} ms per PBS -> {plain:.1f} PBS/s")
+
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "PBS/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+        "data": "synthetic: seeded keys, uniform 4-bit plaintexts, identity LUT",
+        "config": {"workload": f"batched PBS+KS (key switch, blind rotation, sample extract) over {args.batch} "
+                               f"ciphertexts, n=724 k=1 N=2048 PBS 2^23x1 KS 2^3x5 w_ks=21",
+                   "batch": args.batch, "lwe_dim": prm.lwe_dim, "poly_size": prm.poly_size,
+                   "pbs_base_log": prm.pbs_base_log, "pbs_level": prm.pbs_level,
+                   "ks_base_log": prm.ks_base_log, "ks_level": prm.ks_level, "br_workgroups": ctx.br_grid()},
+        "plain_pbs_per_s": round(plain, 2),
+        "batched_over_plain": round(value / plain, 1),
+        "decrypt_success": decrypt_ok,
+        "kernels_ms": {"blind_rotate": round(br_ms, 3), "keyswitch": round(ks_ms, 3)},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 2), "peak": round(peak, 2),
+                     "unit": "Gmodmul/s", "frac": round(achieved / peak, 4), "traffic": None,
+                     "algorithmic_work": "modmul-equivalents per launch = batch x n x ((k+1)(L+1) x 11264 "
+                                         "+ (k+1)^2 L x N + (k+1) N)",
+                     "bsk_stream_gbps": round(bsk_bytes(prm) / (br_ms * 1e-3) / 1e9, 2)},
+    }
+
+    if not args.no_cpu_baseline:
+        from oracle import ref  # CPU baseline leg only
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        sample = 4 * threads
+        log(f"bench: cpu baseline, oracle on {sample} ciphertexts with {threads} threads")
+        rp = ref.params()
+        bsk_ntt = ref.bsk_to_ntt(rp, keys.bsk.reshape(-1), threads=threads)
+        t0 = time.perf_counter()
+        exp = ref.pbs_batch(rp, bsk_ntt, keys.ksk, lut.reshape(1, -1), np.zeros(sample, np.uint32),
+                            cts[:sample], threads=threads)
+        cpu_s = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(sample / cpu_s, 3), "unit": "PBS/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"first {sample} ciphertexts of the batch, same keys and LUT "
+                                         f"({cpu_s:.1f} s wall)"}
+        out["bit_exact_sample"] = bool(np.array_equal(exp, gpu_out[:sample]))
+        log(f"bench: cpu {sample / cpu_s:.3f} PBS/s; bit-exact sample: {out['bit_exact_sample']}")
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -195,6 +195,10 @@ void* luthe_stream(luthe_ctx* ctx);
 /* Blind-rotation launch width (workgroups of the persistent kernel); 0 = occupancy x CUs. */
 int luthe_set_br_grid(luthe_ctx* ctx, int workgroups);
 int luthe_get_br_grid(luthe_ctx* ctx);
+/* Diagnostics: throughput of the Goldilocks modular multiply (64x64 -> 128-bit product
+ * plus reduction) at full occupancy, in 10^9 modmul/s; the VALU roofline denominator of
+ * the blind rotation.  No reference counterpart (the HPU's rate is fixed by its RTL). */
+int luthe_microbench_modmul(luthe_ctx* ctx, double* gmodmul_per_s);
 
 /* ---------------------------------------------------------------- client-side helpers
  * Key generation, encryption, decryption and LUT encoding live in tfhe-rs for the

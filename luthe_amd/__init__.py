@@ -96,6 +96,7 @@ def lib():
         "luthe_stream": (vp, [vp]),
         "luthe_set_br_grid": (i32, [vp, i32]),
         "luthe_get_br_grid": (i32, [vp]),
+        "luthe_microbench_modmul": (i32, [vp, C.POINTER(C.c_double)]),
         "luthe_keygen_secret": (i32, [P_, u64, vp, vp]),
         "luthe_keygen_bsk": (i32, [P_, u64, vp, vp, vp, i32]),
         "luthe_keygen_ksk": (i32, [P_, u64, vp, vp, vp]),
@@ -323,6 +324,12 @@ class Context:
 
     def br_grid(self):
         return lib().luthe_get_br_grid(self._h)
+
+    def modmul_peak(self):
+        """Measured Goldilocks modmul throughput at full occupancy, Gmodmul/s."""
+        v = C.c_double(0)
+        self._chk(lib().luthe_microbench_modmul(self._h, C.byref(v)), "microbench_modmul")
+        return v.value
 
     # ---- batched device-memory entry points
     def pbs_batch_dev(self, d_in, d_out, d_gid, n, log_lut_nb=0, stream=None):

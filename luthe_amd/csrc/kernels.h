@@ -52,6 +52,7 @@ hipError_t launch_blind_rotate(const uint64_t* bsk, const uint64_t* luts, const 
                                int nct, uint64_t* out_base, const uint64_t* tw, const uint64_t* itw, const BrArgs& a,
                                int pbs_level, int grid_override, hipStream_t s);
 int blind_rotate_occupancy(int pbs_level);
+hipError_t launch_modmul_peak(uint64_t* out, int blocks, int iters, hipStream_t s);
 hipError_t launch_alu(uint64_t* base, const AluOp* ops, int nops, int blen, hipStream_t s);
 hipError_t launch_copy(uint64_t* base, const CopyOp* ops, int nops, int blen, hipStream_t s);
 hipError_t launch_make_jobs(PbsJob* jobs, const uint32_t* gid, int n, uint32_t log_lut_nb, uint32_t lut_slots,

@@ -357,6 +357,25 @@ __global__ void k_make_jobs(PbsJob* jobs, const uint32_t* gid, int n, uint32_t l
 }
 
 // ------------------------------------------------------------------------------------
+// Throughput of the Goldilocks modular multiply at full occupancy: 8 independent
+// dependency chains per thread.  Denominator of the blind rotation's VALU roofline.
+// ------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_modmul_peak(uint64_t* out, uint64_t seed, int iters) {
+  uint64_t x[8];
+  const uint64_t y = seed | 0x8000000000000001ull;
+#pragma unroll
+  for (int j = 0; j < 8; j++) x[j] = seed * (uint64_t)(threadIdx.x + 977 * j + 1) + blockIdx.x;
+  for (int it = 0; it < iters; it++) {
+#pragma unroll
+    for (int j = 0; j < 8; j++) x[j] = gl::mul(x[j], y);
+  }
+  uint64_t s = 0;
+#pragma unroll
+  for (int j = 0; j < 8; j++) s ^= x[j];
+  if (s == 0x123456789ull) out[0] = s;  // keeps the chains live; practically never stores
+}
+
+// ------------------------------------------------------------------------------------
 // Host launchers
 // ------------------------------------------------------------------------------------
 hipError_t launch_ntt(int mode, const uint64_t* a, const uint64_t* bn, uint64_t* out, size_t npoly,
@@ -437,6 +456,11 @@ hipError_t launch_alu(uint64_t* base, const AluOp* ops, int nops, int blen, hipS
 hipError_t launch_copy(uint64_t* base, const CopyOp* ops, int nops, int blen, hipStream_t s) {
   if (nops == 0) return hipSuccess;
   k_copy<<<dim3((unsigned)((blen + 255) / 256), (unsigned)nops), 256, 0, s>>>(base, ops, blen);
+  return hipGetLastError();
+}
+
+hipError_t launch_modmul_peak(uint64_t* out, int blocks, int iters, hipStream_t s) {
+  k_modmul_peak<<<dim3(blocks), 256, 0, s>>>(out, 0x9E3779B97F4A7C15ull, iters);
   return hipGetLastError();
 }
 

@@ -304,6 +304,33 @@ int luthe_get_br_grid(luthe_ctx* c) {
   return lk::blind_rotate_occupancy((int)c->prm.pbs_level);
 }
 
+int luthe_microbench_modmul(luthe_ctx* c, double* gmodmul_per_s) {
+  if (!c || !gmodmul_per_s) return LUTHE_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  int cus = 0;
+  HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+  const int blocks = cus * 8, iters = 4096;
+  uint64_t* d_out = nullptr;
+  int rc = dev_alloc(c, (void**)&d_out, 64);
+  if (rc) return rc;
+  hipEvent_t e0, e1;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  HIPCHK(c, lk::launch_modmul_peak(d_out, blocks, 64, c->stream));  // warm up
+  HIPCHK(c, hipEventRecord(e0, c->stream));
+  HIPCHK(c, lk::launch_modmul_peak(d_out, blocks, iters, c->stream));
+  HIPCHK(c, hipEventRecord(e1, c->stream));
+  HIPCHK(c, hipEventSynchronize(e1));
+  float ms = 0;
+  HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(d_out);
+  c->dev_bytes -= 64;
+  *gmodmul_per_s = (double)blocks * 256.0 * 8.0 * iters / (ms * 1e-3) / 1e9;
+  return LUTHE_OK;
+}
+
 int luthe_sync(luthe_ctx* c) {
   if (!c) return LUTHE_ERR_ARG;
   HIPCHK(c, hipStreamSynchronize(c->stream));
