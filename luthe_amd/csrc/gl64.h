@@ -1,8 +1,8 @@
 // Goldilocks field arithmetic (p = 2^64 - 2^32 + 1) for the MI355X PBS engine.
 //
-// Values are kept in a "weak" form: any u64 congruent to the field element, so the
-// hot loops skip canonicalisation; canonical() is applied where bits leave the field
-// (INTT output before the p -> 2^64 switch).  Reductions use 2^64 = 2^32 - 1 and
+// Field elements are kept canonical ([0, p)) so that every addition and subtraction
+// needs exactly one conditional correction; products and shifts accept any 64-bit
+// input and return canonical values.  Reductions use 2^64 = 2^32 - 1 and
 // 2^96 = -1 (mod p), the identities of the reference's Goldilocks reducer
 // (hw/common_lib/mod_arithmetic/.../mod_reduct_64bgoldilocks_karatsuba.sv:8-22,112-114).
 // Powers of two are multiplications by shifts: omega_64 = 8 = 2^3
@@ -24,37 +24,32 @@ constexpr uint64_t EPS = 0xFFFFFFFFull;  // 2^64 mod p
 
 GL_HD uint64_t canonical(uint64_t x) { return x >= P ? x - P : x; }
 
-// a + b (mod p), inputs and output weak.
+// a + b (mod p) for canonical a, b (< p); canonical result.  a + b < 2p < 2^65: when
+// the 64-bit sum wrapped or is >= p, a + b - p is the 64-bit sum plus EPS (mod 2^64).
 GL_HD uint64_t add(uint64_t a, uint64_t b) {
-  uint64_t s = a + b;
-  uint64_t c = (uint64_t)(s < a);            // wrapped by 2^64 == EPS (mod p)
-  uint64_t t = s + ((0 - c) & EPS);
-  uint64_t c2 = (uint64_t)(t < s);           // at most one more wrap, then t < EPS
-  return t + ((0 - c2) & EPS);
+  uint64_t s, t;
+  bool c1 = __builtin_add_overflow(a, b, &s);
+  bool c2 = __builtin_add_overflow(s, EPS, &t);
+  return (c1 | c2) ? t : s;
 }
 
-// a - b (mod p), inputs and output weak.
+// a - b (mod p) for canonical a, b; canonical result (a - b + p when a < b).
 GL_HD uint64_t sub(uint64_t a, uint64_t b) {
   uint64_t d = a - b;
-  uint64_t w = (uint64_t)(a < b);            // borrowed 2^64 == EPS
-  uint64_t m = (0 - w) & EPS;
-  uint64_t t = d - m;
-  uint64_t w2 = (uint64_t)(d < m);
-  return t - ((0 - w2) & EPS);
+  return (a < b) ? d + P : d;
 }
 
 GL_HD uint64_t neg(uint64_t a) { return sub(0, a); }
 
-// (lo + hi * 2^64) mod p, weak output.
+// (lo + hi * 2^64) mod p, canonical.  hi * 2^64 = hh * 2^96 + hl * 2^64 == -hh + hl * EPS.
 GL_HD uint64_t reduce128(uint64_t lo, uint64_t hi) {
   uint64_t hh = hi >> 32, hl = hi & EPS;
-  // hi * 2^64 = hh * 2^96 + hl * 2^64 == -hh + hl * (2^32 - 1)
-  uint64_t t0 = lo - hh;
-  t0 -= (0 - (uint64_t)(lo < hh)) & EPS;     // borrow: cannot wrap twice
-  uint64_t t1 = (hl << 32) - hl;             // hl * 2^64 == hl * (2^32 - 1)
-  uint64_t r = t0 + t1;
-  r += (0 - (uint64_t)(r < t1)) & EPS;       // carry: cannot wrap twice
-  return r;
+  uint64_t t0;
+  if (__builtin_sub_overflow(lo, hh, &t0)) t0 -= EPS;  // borrowed 2^64 == EPS; cannot wrap again
+  uint64_t t1 = (hl << 32) - hl;                        // hl * EPS < p
+  uint64_t r;
+  if (__builtin_add_overflow(t0, t1, &r)) r += EPS;     // carried 2^64 == EPS; cannot wrap again
+  return canonical(r);
 }
 
 GL_HD void mul_full(uint64_t a, uint64_t b, uint64_t& lo, uint64_t& hi) {
@@ -72,33 +67,27 @@ GL_HD void mul_full(uint64_t a, uint64_t b, uint64_t& lo, uint64_t& hi) {
 #endif
 }
 
+// a * b mod p for any 64-bit a, b; canonical result.
 GL_HD uint64_t mul(uint64_t a, uint64_t b) {
   uint64_t lo, hi;
   mul_full(a, b, lo, hi);
   return reduce128(lo, hi);
 }
 
-// x * 2^s for 0 <= s < 96 using V = x << s = v0 + v1*2^64 + v2*2^96 with
-// V == v0 + v1*(2^32-1) - v2 (mod p).
+// x * 2^s mod p for 0 <= s < 96 and any 64-bit x; canonical result.
+//  s < 64 : x << s is the 128-bit (x << s, x >> (64 - s)), reduced as above.
+//  s >= 64: x << s = v1 * 2^64 + v2 * 2^96 with v1 = (x << (s - 64)) mod 2^32 and
+//           v2 = x >> (96 - s), == v1 * EPS - v2 (mod p); v1 * EPS < p.
 GL_HD uint64_t shl96(uint64_t x, uint32_t s) {
-  uint64_t v0, v1, v2;
-  if (s == 0) return x;
-  if (s < 64) {
-    v0 = x << s;
-    uint64_t hi = x >> (64 - s);
-    v1 = hi & EPS;
-    v2 = hi >> 32;
-  } else {
-    v0 = 0;
-    v1 = (x << (s - 64)) & EPS;
-    v2 = x >> (96 - s);
-  }
-  return add(sub(v0, v2), (v1 << 32) - v1);
+  if (s == 0) return canonical(x);
+  if (s < 64) return reduce128(x << s, x >> (64 - s));
+  uint64_t v1 = (x << (s - 64)) & EPS;
+  uint64_t v2 = canonical(x >> (96 - s));
+  return sub((v1 << 32) - v1, v2);
 }
 
-// x * 2^s for any 0 <= s < 192 (2^96 == -1), s known per call site.
+// x * 2^s for 0 <= s < 192 (2^96 == -1); s uniform or per lane.
 GL_HD uint64_t mul_pow2(uint64_t x, uint32_t s) {
-  s %= 192;
   if (s >= 96) return neg(shl96(x, s - 96));
   return shl96(x, s);
 }
@@ -114,22 +103,6 @@ GL_HD uint64_t mul_pow2c(uint64_t x) {
   } else {
     return shl96(x, (uint32_t)s);
   }
-}
-
-// Branch-free runtime x * 2^s for 0 <= s < 192 where s varies across lanes.
-GL_HD uint64_t mul_pow2_rt(uint64_t x, uint32_t s) {
-  uint32_t negf = s >= 96;
-  uint32_t s2 = negf ? s - 96 : s;           // 0..95
-  uint32_t lo_sh = s2 & 63;                  // valid when s2 < 64
-  uint64_t v0 = (s2 < 64) ? (x << lo_sh) : 0;
-  uint64_t hi_a = (s2 == 0) ? 0 : (x >> ((64 - s2) & 63));           // s2 in 1..63
-  uint64_t v1a = hi_a & EPS, v2a = hi_a >> 32;
-  uint64_t v1b = (x << ((s2 - 64) & 63)) & EPS;                       // s2 in 64..95
-  uint64_t v2b = x >> ((96 - s2) & 63);
-  uint64_t v1 = (s2 < 64) ? v1a : v1b;
-  uint64_t v2 = (s2 < 64) ? v2a : v2b;
-  uint64_t r = add(sub(v0, v2), (v1 << 32) - v1);
-  return negf ? neg(r) : r;
 }
 
 GL_HD uint64_t pow(uint64_t a, uint64_t e) {

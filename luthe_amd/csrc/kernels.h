@@ -43,13 +43,13 @@ struct BrArgs {
 };
 
 hipError_t launch_ntt(int mode, const uint64_t* a, const uint64_t* bn, uint64_t* out, size_t npoly,
-                      const uint64_t* tw, const uint64_t* itw, hipStream_t s);
+                      const uint64_t* tab, hipStream_t s);
 hipError_t launch_bsk_prepare(const uint64_t* src, uint64_t* dst, int n_lwe, int rows, int K1, int is_ntt,
-                              const uint64_t* tw, hipStream_t s);
+                              const uint64_t* tab, hipStream_t s);
 hipError_t launch_keyswitch(const uint64_t* base, const PbsJob* jobs, int nct, const uint32_t* ksk, uint16_t* lwe2n,
                             const KsArgs& a, int ks_level, hipStream_t s);
 hipError_t launch_blind_rotate(const uint64_t* bsk, const uint64_t* luts, const uint16_t* lwe2n, const PbsJob* jobs,
-                               int nct, uint64_t* out_base, const uint64_t* tw, const uint64_t* itw, const BrArgs& a,
+                               int nct, uint64_t* out_base, const uint64_t* tab, const BrArgs& a,
                                int pbs_level, int grid_override, hipStream_t s);
 int blind_rotate_occupancy(int pbs_level);
 hipError_t launch_modmul_peak(uint64_t* out, int blocks, int iters, hipStream_t s);

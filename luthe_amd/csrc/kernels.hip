@@ -20,18 +20,22 @@ namespace lk {
 // ------------------------------------------------------------------------------------
 // NTT kernels: NP polynomials per 256-thread workgroup.
 // ------------------------------------------------------------------------------------
+// tab: twiddle tables (ntt2048.h TABLE_WORDS layout).
 template <int NP>
-__device__ __forceinline__ void fwd_passes(uint64_t (*v)[8], uint64_t* lds, const uint64_t* tw, int t) {
+__device__ __forceinline__ void fwd_passes(uint64_t (*v)[8], uint64_t* lds, const uint64_t* tab, int t) {
   const int r = t & 63, m0 = t >> 6, k = t >> 3, j = t & 7;
+  const int m0u = wave_uniform(m0);
+  const uint64_t* tw = tab + TW_OFF;
+  const uint64_t* c64 = tab + C64_OFF;
 #pragma unroll
-  for (int p = 0; p < NP; p++) fwd_a1_compute(v[p], m0);
+  for (int p = 0; p < NP; p++) fwd_a1_compute(v[p], m0u);
 #pragma unroll
   for (int p = 0; p < NP; p++) fwd_a1_store(v[p], lds + p * N, r, m0);
   __syncthreads();
 #pragma unroll
   for (int p = 0; p < NP; p++) {
     fwd_a2_load(v[p], lds + p * N, r, m0);
-    fwd_a2_compute(v[p], r, m0, tw);
+    fwd_a2_compute(v[p], r, m0u, tw);
   }
   __syncthreads();
 #pragma unroll
@@ -40,7 +44,7 @@ __device__ __forceinline__ void fwd_passes(uint64_t (*v)[8], uint64_t* lds, cons
 #pragma unroll
   for (int p = 0; p < NP; p++) {
     fwd_c1_load(v[p], lds + p * N, k, j);
-    fwd_c1_compute(v[p], j);
+    fwd_c1_compute(v[p], j, c64);
   }
   __syncthreads();
 #pragma unroll
@@ -56,10 +60,13 @@ __device__ __forceinline__ void fwd_passes(uint64_t (*v)[8], uint64_t* lds, cons
 // Entry: v in NTT domain (thread (k, w1) layout), LDS free for writing only after a
 // barrier that orders the previous readers.  Exit: v[p][m1] canonical coefficients.
 template <int NP>
-__device__ __forceinline__ void inv_passes(uint64_t (*v)[8], uint64_t* lds, const uint64_t* itw, int t) {
+__device__ __forceinline__ void inv_passes(uint64_t (*v)[8], uint64_t* lds, const uint64_t* tab, int t) {
   const int r = t & 63, m0 = t >> 6, k = t >> 3, j = t & 7;
+  const int m0u = wave_uniform(m0);
+  const uint64_t* itw = tab + ITW_OFF;
+  const uint64_t* ic64 = tab + IC64_OFF;
 #pragma unroll
-  for (int p = 0; p < NP; p++) inv_c2_compute(v[p], j);
+  for (int p = 0; p < NP; p++) inv_c2_compute(v[p], j, ic64);
   __syncthreads();
 #pragma unroll
   for (int p = 0; p < NP; p++) inv_c2_store(v[p], lds + p * N, k, j);
@@ -76,7 +83,7 @@ __device__ __forceinline__ void inv_passes(uint64_t (*v)[8], uint64_t* lds, cons
 #pragma unroll
   for (int p = 0; p < NP; p++) {
     inv_a2_load(v[p], lds + p * N, r, m0);
-    inv_a2_compute(v[p], m0);
+    inv_a2_compute(v[p], m0u);
   }
   __syncthreads();
 #pragma unroll
@@ -85,7 +92,7 @@ __device__ __forceinline__ void inv_passes(uint64_t (*v)[8], uint64_t* lds, cons
 #pragma unroll
   for (int p = 0; p < NP; p++) {
     inv_a1_load(v[p], lds + p * N, r, m0);
-    inv_a1_compute(v[p], m0);
+    inv_a1_compute(v[p], m0u);
   }
 }
 
@@ -93,7 +100,7 @@ __device__ __forceinline__ void inv_passes(uint64_t (*v)[8], uint64_t* lds, cons
 template <int NP, int MODE>
 __global__ __launch_bounds__(256) void k_ntt(const uint64_t* __restrict__ a, const uint64_t* __restrict__ bn,
                                              uint64_t* __restrict__ out, size_t npoly,
-                                             const uint64_t* __restrict__ tw, const uint64_t* __restrict__ itw) {
+                                             const uint64_t* __restrict__ tab) {
   __shared__ uint64_t lds[NP * N];
   const int t = threadIdx.x;
   const int r = t & 63, m0 = t >> 6, k = t >> 3, w1 = t & 7;
@@ -113,7 +120,7 @@ __global__ __launch_bounds__(256) void k_ntt(const uint64_t* __restrict__ a, con
 #pragma unroll
       for (int m1 = 0; m1 < 8; m1++) v[p][m1] = a[poly * N + 64 * (m0 + 4 * m1) + r];
     }
-    fwd_passes<NP>(v, lds, tw, t);
+    fwd_passes<NP>(v, lds, tab, t);
   }
   if (MODE == 0) {
 #pragma unroll
@@ -132,7 +139,7 @@ __global__ __launch_bounds__(256) void k_ntt(const uint64_t* __restrict__ a, con
       for (int w0 = 0; w0 < 8; w0++) v[p][w0] = gl::mul(v[p][w0], bn[poly * N + 64 * k + w1 + 8 * w0]);
     }
   }
-  inv_passes<NP>(v, lds, itw, t);
+  inv_passes<NP>(v, lds, tab, t);
 #pragma unroll
   for (int p = 0; p < NP; p++) {
     if (base + p >= npoly) break;
@@ -147,7 +154,7 @@ __global__ __launch_bounds__(256) void k_ntt(const uint64_t* __restrict__ a, con
 // 8 values per (gl, j) with fully coalesced loads.
 // ------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_bsk_prepare(const uint64_t* __restrict__ src, uint64_t* __restrict__ dst,
-                                                     int rows, int K1, int is_ntt, const uint64_t* __restrict__ tw) {
+                                                     int rows, int K1, int is_ntt, const uint64_t* __restrict__ tab) {
   __shared__ uint64_t lds[N];
   const int t = threadIdx.x;
   const int r = t & 63, m0 = t >> 6, k = t >> 3, w1 = t & 7;
@@ -160,7 +167,7 @@ __global__ __launch_bounds__(256) void k_bsk_prepare(const uint64_t* __restrict_
   } else {
 #pragma unroll
     for (int m1 = 0; m1 < 8; m1++) v[0][m1] = src[poly * N + 64 * (m0 + 4 * m1) + r];
-    fwd_passes<1>(v, lds, tw, t);
+    fwd_passes<1>(v, lds, tab, t);
   }
 #pragma unroll
   for (int w0 = 0; w0 < 8; w0++)
@@ -245,8 +252,7 @@ __global__ __launch_bounds__(256) void k_blind_rotate(const uint64_t* __restrict
                                                       const uint16_t* __restrict__ lwe2n,
                                                       const PbsJob* __restrict__ jobs, int nct,
                                                       uint64_t* __restrict__ out_base,
-                                                      const uint64_t* __restrict__ tw,
-                                                      const uint64_t* __restrict__ itw, BrArgs a) {
+                                                      const uint64_t* __restrict__ tab, BrArgs a) {
   constexpr int NPF = K1 * L;
   __shared__ uint64_t lds[NPF * N];
   const int t = threadIdx.x;
@@ -281,9 +287,9 @@ __global__ __launch_bounds__(256) void k_blind_rotate(const uint64_t* __restrict
 #pragma unroll
       for (int g = 0; g < K1; g++) pbs::rotate_decompose<L>(lds + g * N, acc[g], tt, rot, a.base_log, &v[g * L]);
       __syncthreads();
-      fwd_passes<NPF>(v, lds, tw, tt);
+      fwd_passes<NPF>(v, lds, tab, tt);
       pbs::mac<K1, L>(v, bsk_i, tt);
-      inv_passes<K1>(v, lds, itw, tt);
+      inv_passes<K1>(v, lds, tab, tt);
 #pragma unroll
       for (int g = 0; g < K1; g++)
 #pragma unroll
@@ -379,22 +385,22 @@ __global__ __launch_bounds__(256) void k_modmul_peak(uint64_t* out, uint64_t see
 // Host launchers
 // ------------------------------------------------------------------------------------
 hipError_t launch_ntt(int mode, const uint64_t* a, const uint64_t* bn, uint64_t* out, size_t npoly,
-                      const uint64_t* tw, const uint64_t* itw, hipStream_t s) {
+                      const uint64_t* tab, hipStream_t s) {
   constexpr int NP = 2;
   if (npoly == 0) return hipSuccess;
   dim3 grid((unsigned)((npoly + NP - 1) / NP));
   if (mode == 0)
-    k_ntt<NP, 0><<<grid, 256, 0, s>>>(a, bn, out, npoly, tw, itw);
+    k_ntt<NP, 0><<<grid, 256, 0, s>>>(a, bn, out, npoly, tab);
   else if (mode == 1)
-    k_ntt<NP, 1><<<grid, 256, 0, s>>>(a, bn, out, npoly, tw, itw);
+    k_ntt<NP, 1><<<grid, 256, 0, s>>>(a, bn, out, npoly, tab);
   else
-    k_ntt<NP, 2><<<grid, 256, 0, s>>>(a, bn, out, npoly, tw, itw);
+    k_ntt<NP, 2><<<grid, 256, 0, s>>>(a, bn, out, npoly, tab);
   return hipGetLastError();
 }
 
 hipError_t launch_bsk_prepare(const uint64_t* src, uint64_t* dst, int n_lwe, int rows, int K1, int is_ntt,
-                              const uint64_t* tw, hipStream_t s) {
-  k_bsk_prepare<<<dim3((unsigned)((size_t)n_lwe * rows * K1)), 256, 0, s>>>(src, dst, rows, K1, is_ntt, tw);
+                              const uint64_t* tab, hipStream_t s) {
+  k_bsk_prepare<<<dim3((unsigned)((size_t)n_lwe * rows * K1)), 256, 0, s>>>(src, dst, rows, K1, is_ntt, tab);
   return hipGetLastError();
 }
 
@@ -426,17 +432,17 @@ static int br_max_grid() {
 }
 
 hipError_t launch_blind_rotate(const uint64_t* bsk, const uint64_t* luts, const uint16_t* lwe2n, const PbsJob* jobs,
-                               int nct, uint64_t* out_base, const uint64_t* tw, const uint64_t* itw, const BrArgs& a,
+                               int nct, uint64_t* out_base, const uint64_t* tab, const BrArgs& a,
                                int pbs_level, int grid_override, hipStream_t s) {
   if (nct == 0) return hipSuccess;
   if (pbs_level == 1) {
     int g = grid_override > 0 ? grid_override : br_max_grid<2, 1>();
     g = g < nct ? g : nct;
-    k_blind_rotate<2, 1><<<dim3(g), 256, 0, s>>>(bsk, luts, lwe2n, jobs, nct, out_base, tw, itw, a);
+    k_blind_rotate<2, 1><<<dim3(g), 256, 0, s>>>(bsk, luts, lwe2n, jobs, nct, out_base, tab, a);
   } else if (pbs_level == 2) {
     int g = grid_override > 0 ? grid_override : br_max_grid<2, 2>();
     g = g < nct ? g : nct;
-    k_blind_rotate<2, 2><<<dim3(g), 256, 0, s>>>(bsk, luts, lwe2n, jobs, nct, out_base, tw, itw, a);
+    k_blind_rotate<2, 2><<<dim3(g), 256, 0, s>>>(bsk, luts, lwe2n, jobs, nct, out_base, tab, a);
   } else {
     return hipErrorInvalidValue;
   }

@@ -50,8 +50,7 @@ struct luthe_ctx {
   uint64_t* d_luts = nullptr;
   uint64_t* d_bsk = nullptr;
   uint32_t* d_ksk = nullptr;
-  uint64_t* d_tw = nullptr;
-  uint64_t* d_itw = nullptr;
+  uint64_t* d_tab = nullptr;  // NTT twiddle tables (ntt2048.h layout)
   bool bsk_ok = false, ksk_ok = false;
   // scratch (grow-only)
   uint16_t* d_lwe2n = nullptr;
@@ -185,7 +184,7 @@ int run_pbs(luthe_ctx* c, const uint64_t* in_base, const PbsJob* d_jobs, int n, 
   lk::KsArgs ka{(int)(p.glwe_k * p.poly_size), (int)p.lwe_dim, c->lwe_stride, p.ks_base_log, p.ks_mod_log, 12};
   HIPCHK(c, lk::launch_keyswitch(in_base, d_jobs, n, c->d_ksk, c->d_lwe2n, ka, (int)p.ks_level, s));
   lk::BrArgs ba{(int)p.lwe_dim, c->lwe_stride, p.pbs_base_log};
-  HIPCHK(c, lk::launch_blind_rotate(c->d_bsk, c->d_luts, c->d_lwe2n, d_jobs, n, out_base, c->d_tw, c->d_itw, ba,
+  HIPCHK(c, lk::launch_blind_rotate(c->d_bsk, c->d_luts, c->d_lwe2n, d_jobs, n, out_base, c->d_tab, ba,
                                     (int)p.pbs_level, c->br_grid_override, s));
   c->cnt.pbs += (uint64_t)n;
   c->cnt.pbs_batches += 1;
@@ -247,12 +246,10 @@ int luthe_create(const luthe_params* p, int device, luthe_ctx** out) {
   if ((rc = dev_alloc(c, (void**)&c->d_luts, (size_t)p->lut_slots * c->lut_len * 8))) return bail(rc);
   if (hipMemset(c->d_mem, 0, rows * c->blen * 8) != hipSuccess) return bail(LUTHE_ERR_HIP);
   if (hipMemset(c->d_luts, 0, (size_t)p->lut_slots * c->lut_len * 8) != hipSuccess) return bail(LUTHE_ERR_HIP);
-  std::vector<uint64_t> tw(2048), itw(2048);
-  ntt::make_twist_tables(tw.data(), itw.data());
-  if ((rc = dev_alloc(c, (void**)&c->d_tw, 2048 * 8))) return bail(rc);
-  if ((rc = dev_alloc(c, (void**)&c->d_itw, 2048 * 8))) return bail(rc);
-  if (hipMemcpy(c->d_tw, tw.data(), 2048 * 8, hipMemcpyHostToDevice) != hipSuccess) return bail(LUTHE_ERR_HIP);
-  if (hipMemcpy(c->d_itw, itw.data(), 2048 * 8, hipMemcpyHostToDevice) != hipSuccess) return bail(LUTHE_ERR_HIP);
+  std::vector<uint64_t> tab(ntt::TABLE_WORDS);
+  ntt::make_twist_tables(tab.data());
+  if ((rc = dev_alloc(c, (void**)&c->d_tab, tab.size() * 8))) return bail(rc);
+  if (hipMemcpy(c->d_tab, tab.data(), tab.size() * 8, hipMemcpyHostToDevice) != hipSuccess) return bail(LUTHE_ERR_HIP);
   *out = c;
   return LUTHE_OK;
 }
@@ -263,7 +260,7 @@ int luthe_destroy(luthe_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto& kv : c->tickets)
     if (kv.second.done) (void)hipEventDestroy(kv.second.done);
-  void* bufs[] = {c->d_mem, c->d_luts, c->d_bsk, c->d_ksk, c->d_tw, c->d_itw, c->d_lwe2n, c->d_jobs, c->d_stage};
+  void* bufs[] = {c->d_mem, c->d_luts, c->d_bsk, c->d_ksk, c->d_tab, c->d_lwe2n, c->d_jobs, c->d_stage};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (c->h_stage) (void)hipHostFree(c->h_stage);
@@ -354,7 +351,7 @@ int luthe_upload_bsk(luthe_ctx* c, const uint64_t* bsk, int layout) {
   if (rc) return rc;
   hipError_t e = hipMemcpyAsync(tmp, bsk, elems * 8, hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess)
-    e = lk::launch_bsk_prepare(tmp, c->d_bsk, (int)p.lwe_dim, rows, K1, layout == LUTHE_BSK_NTT, c->d_tw, c->stream);
+    e = lk::launch_bsk_prepare(tmp, c->d_bsk, (int)p.lwe_dim, rows, K1, layout == LUTHE_BSK_NTT, c->d_tab, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   (void)hipFree(tmp);
   c->dev_bytes -= elems * 8;
@@ -442,7 +439,7 @@ int luthe_blind_rotate_batch(luthe_ctx* c, const uint16_t* d_lwe2n, uint64_t* d_
   HIPCHK(c, lk::launch_make_jobs(c->d_jobs, d_gid, (int)n, log_lut_nb, c->prm.lut_slots, s));
   const luthe_params& p = c->prm;
   lk::BrArgs ba{(int)p.lwe_dim, (int)p.lwe_dim + 1, p.pbs_base_log};
-  HIPCHK(c, lk::launch_blind_rotate(c->d_bsk, c->d_luts, d_lwe2n, c->d_jobs, (int)n, d_out, c->d_tw, c->d_itw, ba,
+  HIPCHK(c, lk::launch_blind_rotate(c->d_bsk, c->d_luts, d_lwe2n, c->d_jobs, (int)n, d_out, c->d_tab, ba,
                                     (int)p.pbs_level, c->br_grid_override, s));
   return LUTHE_OK;
 }
@@ -464,17 +461,17 @@ int luthe_pbs_batch(luthe_ctx* c, const uint64_t* d_in, uint64_t* d_out, const u
 int luthe_ntt_polymul_batch(luthe_ctx* c, const uint64_t* d_a, const uint64_t* d_bn, uint64_t* d_out, size_t npoly,
                             void* stream) {
   if (!c || (npoly && (!d_a || !d_bn || !d_out))) return LUTHE_ERR_ARG;
-  HIPCHK(c, lk::launch_ntt(2, d_a, d_bn, d_out, npoly, c->d_tw, c->d_itw, pick_stream(c, stream)));
+  HIPCHK(c, lk::launch_ntt(2, d_a, d_bn, d_out, npoly, c->d_tab, pick_stream(c, stream)));
   return LUTHE_OK;
 }
 int luthe_ntt_forward_batch(luthe_ctx* c, const uint64_t* d_in, uint64_t* d_out, size_t npoly, void* stream) {
   if (!c || (npoly && (!d_in || !d_out))) return LUTHE_ERR_ARG;
-  HIPCHK(c, lk::launch_ntt(0, d_in, nullptr, d_out, npoly, c->d_tw, c->d_itw, pick_stream(c, stream)));
+  HIPCHK(c, lk::launch_ntt(0, d_in, nullptr, d_out, npoly, c->d_tab, pick_stream(c, stream)));
   return LUTHE_OK;
 }
 int luthe_ntt_inverse_batch(luthe_ctx* c, const uint64_t* d_in, uint64_t* d_out, size_t npoly, void* stream) {
   if (!c || (npoly && (!d_in || !d_out))) return LUTHE_ERR_ARG;
-  HIPCHK(c, lk::launch_ntt(1, d_in, nullptr, d_out, npoly, c->d_tw, c->d_itw, pick_stream(c, stream)));
+  HIPCHK(c, lk::launch_ntt(1, d_in, nullptr, d_out, npoly, c->d_tab, pick_stream(c, stream)));
   return LUTHE_OK;
 }
 

@@ -31,7 +31,7 @@
 //   T (k, r)      = 64 k + (r ^ 8 (k & 7))
 //   U (k, w1, r0) = 64 k + 8 (w1 ^ (k & 7)) + (r0 ^ w1)
 //
-// Each pass is split into load / compute / store so that a kernel can put a barrier
+// Field values stay canonical (gl64.h).  Each pass is split into load / compute / store so that a kernel can put a barrier
 // between all loads and all stores of the same buffer (in-place LDS reuse), and a
 // sequential CPU emulation of the 256 threads executes the identical code.
 #pragma once
@@ -81,14 +81,33 @@ GL_HD void dft4(uint64_t* x) {
   x[3] = gl::sub(a1, b1);
 }
 
+// Wave-uniform copy of a value that is uniform within each 64-lane wave (the A-pass
+// index m0 = t >> 6): lets shifts by it compile to scalar-controlled code.
+GL_HD int wave_uniform(int x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_readfirstlane(x);
+#else
+  return x;
+#endif
+}
+
 // ---------------------------------------------------------------- forward passes
-// A1: in regs v[m1] = p[64 (m0 + 4 m1) + r]
-GL_HD void fwd_a1_compute(uint64_t* v, int m0) {
-#pragma unroll
-  for (int m1 = 0; m1 < 8; m1++) v[m1] = gl::mul_pow2_rt(v[m1], 3 * (m0 + 4 * m1));
+// A1: in regs v[m1] = p[64 (m0 + 4 m1) + r].  psi^(m0 + 4 m1) Omega^(k1 m0) is split into
+// a compile-time pre-twist psi^(4 m1) = 2^(12 m1) and a post-twist
+// psi^m0 Omega^(k1 m0) = 2^(3 m0 (1 + 2 k1)) that is uniform per wave.
+GL_HD void fwd_a1_compute(uint64_t* v, int m0u) {
+  v[1] = gl::mul_pow2c<12>(v[1]);
+  v[2] = gl::mul_pow2c<24>(v[2]);
+  v[3] = gl::mul_pow2c<36>(v[3]);
+  v[4] = gl::mul_pow2c<48>(v[4]);
+  v[5] = gl::mul_pow2c<60>(v[5]);
+  v[6] = gl::mul_pow2c<72>(v[6]);
+  v[7] = gl::mul_pow2c<84>(v[7]);
   dft8<24>(v);
+  if (m0u != 0) {
 #pragma unroll
-  for (int k1 = 1; k1 < 8; k1++) v[k1] = gl::mul_pow2_rt(v[k1], 6 * k1 * m0);
+    for (int k1 = 0; k1 < 8; k1++) v[k1] = gl::mul_pow2(v[k1], (uint32_t)(3 * m0u * (1 + 2 * k1)));
+  }
 }
 GL_HD void fwd_a1_store(const uint64_t* v, uint64_t* lds, int r, int m0) {
 #pragma unroll
@@ -118,15 +137,15 @@ GL_HD void fwd_a2_store(const uint64_t* v, uint64_t* lds, int r, int q) {
 #pragma unroll
     for (int k0 = 0; k0 < 4; k0++) lds[idx_t(2 * q + h + 8 * k0, r)] = v[h * 4 + k0];
 }
-// C1
+// C1: omega^(w1 r0) with r0 per lane, from the 64-entry table c64[8 w1 + r0].
 GL_HD void fwd_c1_load(uint64_t* v, const uint64_t* lds, int k, int r0) {
 #pragma unroll
   for (int r1 = 0; r1 < 8; r1++) v[r1] = lds[idx_t(k, r0 + 8 * r1)];
 }
-GL_HD void fwd_c1_compute(uint64_t* v, int r0) {
+GL_HD void fwd_c1_compute(uint64_t* v, int r0, const uint64_t* c64) {
   dft8<24>(v);
 #pragma unroll
-  for (int w1 = 1; w1 < 8; w1++) v[w1] = gl::mul_pow2_rt(v[w1], 3 * w1 * r0);
+  for (int w1 = 1; w1 < 8; w1++) v[w1] = gl::mul(v[w1], c64[8 * w1 + r0]);
 }
 GL_HD void fwd_c1_store(const uint64_t* v, uint64_t* lds, int k, int r0) {
 #pragma unroll
@@ -140,10 +159,11 @@ GL_HD void fwd_c2_load(uint64_t* v, const uint64_t* lds, int k, int w1) {
 GL_HD void fwd_c2_compute(uint64_t* v) { dft8<24>(v); }  // v[w0] = Z[64k + w1 + 8 w0]
 
 // ---------------------------------------------------------------- inverse passes
-GL_HD void inv_c2_compute(uint64_t* v, int w1) {
+// IC2: omega^(-w1 r0) with w1 per lane, from ic64[8 w1 + r0].
+GL_HD void inv_c2_compute(uint64_t* v, int w1, const uint64_t* ic64) {
   dft8<168>(v);
 #pragma unroll
-  for (int r0 = 1; r0 < 8; r0++) v[r0] = gl::mul_pow2_rt(v[r0], (192 - 3 * w1 * r0) % 192);
+  for (int r0 = 1; r0 < 8; r0++) v[r0] = gl::mul(v[r0], ic64[8 * w1 + r0]);
 }
 GL_HD void inv_c2_store(const uint64_t* v, uint64_t* lds, int k, int w1) {
 #pragma unroll
@@ -168,13 +188,16 @@ GL_HD void inv_a2_load(uint64_t* v, const uint64_t* lds, int r, int q) {
 #pragma unroll
     for (int k0 = 0; k0 < 4; k0++) v[h * 4 + k0] = lds[idx_t(2 * q + h + 8 * k0, r)];
 }
-GL_HD void inv_a2_compute(uint64_t* v, int q) {
+// IA2: Omega^(-k1 m0) = 2^(-6 k1 m0), uniform per wave (k1 = 2q + h).
+GL_HD void inv_a2_compute(uint64_t* v, int qu) {
 #pragma unroll
   for (int h = 0; h < 2; h++) {
     dft4<144>(v + h * 4);
-    int k1 = 2 * q + h;
+    int k1 = 2 * qu + h;
+    if (k1 != 0) {
 #pragma unroll
-    for (int m0 = 1; m0 < 4; m0++) v[h * 4 + m0] = gl::mul_pow2_rt(v[h * 4 + m0], (192 - 6 * k1 * m0) % 192);
+      for (int m0 = 1; m0 < 4; m0++) v[h * 4 + m0] = gl::mul_pow2(v[h * 4 + m0], (uint32_t)(192 - 6 * k1 * m0));
+    }
   }
 }
 GL_HD void inv_a2_store(const uint64_t* v, uint64_t* lds, int r, int q) {
@@ -187,16 +210,25 @@ GL_HD void inv_a1_load(uint64_t* v, const uint64_t* lds, int r, int m0) {
 #pragma unroll
   for (int k1 = 0; k1 < 8; k1++) v[k1] = lds[idx_y1(k1, m0, r)];
 }
-// Output v[m1] = p[64 (m0 + 4 m1) + r], canonical.
-GL_HD void inv_a1_compute(uint64_t* v, int m0) {
+// Output v[m1] = p[64 (m0 + 4 m1) + r] (canonical); psi^-(m0 + 4 m1) uniform per wave.
+GL_HD void inv_a1_compute(uint64_t* v, int m0u) {
   dft8<168>(v);
 #pragma unroll
-  for (int m1 = 0; m1 < 8; m1++) v[m1] = gl::canonical(gl::mul_pow2_rt(v[m1], (192 - 3 * (m0 + 4 * m1)) % 192));
+  for (int m1 = 0; m1 < 8; m1++) {
+    uint32_t s = (uint32_t)((192 - 3 * (m0u + 4 * m1)) % 192);
+    v[m1] = gl::mul_pow2(v[m1], s);
+  }
 }
 
-// Twist tables (host side): tw[k*64 + r] = zeta^((2k+1) r), itw = zeta^(-(2k+1) r) / 2048,
+// Twiddle tables (host side), one buffer of TABLE_WORDS u64:
+//   [0, 2048)    tw [k*64 + r] = zeta^((2k+1) r)
+//   [2048, 4096) itw[k*64 + r] = zeta^(-(2k+1) r) / 2048
+//   [4096, 4160) c64 [8 w1 + r0] = 8^(w1 r0)
+//   [4160, 4224) ic64[8 w1 + r0] = 8^(-w1 r0)
 // zeta = OMG_2_32^(2^20) (sw/ntt/ntt_gf64.sage:44,69-75 power_of_rou(4096, 1)).
-inline void make_twist_tables(uint64_t* tw, uint64_t* itw) {
+constexpr int TW_OFF = 0, ITW_OFF = 2048, C64_OFF = 4096, IC64_OFF = 4160, TABLE_WORDS = 4224;
+
+inline void make_twist_tables(uint64_t* tab) {
   const uint64_t OMG_2_32 = 16334397945464290598ull;
   uint64_t zeta = gl::pow(OMG_2_32, 1ull << 20);
   uint64_t izeta = gl::inv(zeta);
@@ -204,8 +236,14 @@ inline void make_twist_tables(uint64_t* tw, uint64_t* itw) {
   for (int k = 0; k < 32; k++)
     for (int r = 0; r < 64; r++) {
       uint64_t e = (uint64_t)(2 * k + 1) * (uint64_t)r;
-      tw[k * 64 + r] = gl::pow(zeta, e);
-      itw[k * 64 + r] = gl::canonical(gl::mul(gl::pow(izeta, e), inv_n));
+      tab[TW_OFF + k * 64 + r] = gl::pow(zeta, e);
+      tab[ITW_OFF + k * 64 + r] = gl::mul(gl::pow(izeta, e), inv_n);
+    }
+  uint64_t i8 = gl::inv(8);
+  for (int w1 = 0; w1 < 8; w1++)
+    for (int r0 = 0; r0 < 8; r0++) {
+      tab[C64_OFF + 8 * w1 + r0] = gl::pow(8, (uint64_t)(w1 * r0));
+      tab[IC64_OFF + 8 * w1 + r0] = gl::pow(i8, (uint64_t)(w1 * r0));
     }
 }
 

@@ -11,8 +11,12 @@ using namespace ntt;
 
 namespace {
 struct Tables {
-  uint64_t tw[2048], itw[2048];
-  Tables() { make_twist_tables(tw, itw); }
+  uint64_t tab[TABLE_WORDS];
+  const uint64_t* tw = tab + TW_OFF;
+  const uint64_t* itw = tab + ITW_OFF;
+  const uint64_t* c64 = tab + C64_OFF;
+  const uint64_t* ic64 = tab + IC64_OFF;
+  Tables() { make_twist_tables(tab); }
 };
 const Tables& tables() {
   static Tables t;
@@ -40,7 +44,7 @@ void emu_ntt_fwd(const uint64_t* in, uint64_t* out) {
   for (int t = 0; t < T; t++) fwd_a2_store(v[t], lds.data(), t & 63, t >> 6);
   for (int t = 0; t < T; t++) {
     fwd_c1_load(v[t], lds.data(), t >> 3, t & 7);
-    fwd_c1_compute(v[t], t & 7);
+    fwd_c1_compute(v[t], t & 7, tb.c64);
   }
   for (int t = 0; t < T; t++) fwd_c1_store(v[t], lds.data(), t >> 3, t & 7);
   for (int t = 0; t < T; t++) {
@@ -59,7 +63,7 @@ void emu_ntt_inv(const uint64_t* in, uint64_t* out) {
   for (int t = 0; t < T; t++) {
     int k = t >> 3, w1 = t & 7;
     for (int w0 = 0; w0 < 8; w0++) v[t][w0] = in[64 * k + w1 + 8 * w0];
-    inv_c2_compute(v[t], w1);
+    inv_c2_compute(v[t], w1, tb.ic64);
   }
   for (int t = 0; t < T; t++) inv_c2_store(v[t], lds.data(), t >> 3, t & 7);
   for (int t = 0; t < T; t++) {
@@ -83,7 +87,6 @@ void emu_ntt_inv(const uint64_t* in, uint64_t* out) {
 uint64_t emu_gl_mul(uint64_t a, uint64_t b) { return gl::mul(a, b); }
 uint64_t emu_gl_add(uint64_t a, uint64_t b) { return gl::add(a, b); }
 uint64_t emu_gl_sub(uint64_t a, uint64_t b) { return gl::sub(a, b); }
-uint64_t emu_gl_mul_pow2_rt(uint64_t a, uint32_t s) { return gl::mul_pow2_rt(a, s); }
 uint64_t emu_gl_mul_pow2(uint64_t a, uint32_t s) { return gl::mul_pow2(a, s); }
 uint64_t emu_gl_reduce128(uint64_t lo, uint64_t hi) { return gl::reduce128(lo, hi); }
 uint64_t emu_modswitch(uint64_t e) { return gl::modswitch_p_to_q(e); }
@@ -140,7 +143,7 @@ void blind_rotate(int n, const uint64_t* bsk_int, const uint64_t* lut, const uin
     for (int t = 0; t < T; t++)
       for (int p = 0; p < NPF; p++) {
         fwd_c1_load(v[t][p], lds.data() + p * N, t >> 3, t & 7);
-        fwd_c1_compute(v[t][p], t & 7);
+        fwd_c1_compute(v[t][p], t & 7, tb.c64);
       }
     for (int t = 0; t < T; t++)
       for (int p = 0; p < NPF; p++) fwd_c1_store(v[t][p], lds.data() + p * N, t >> 3, t & 7);
@@ -151,7 +154,7 @@ void blind_rotate(int n, const uint64_t* bsk_int, const uint64_t* lut, const uin
         fwd_c2_compute(v[t][p]);
       }
       pbs::mac<K1, L>(v[t], bsk_i, t);
-      for (int j = 0; j < K1; j++) inv_c2_compute(v[t][j], t & 7);
+      for (int j = 0; j < K1; j++) inv_c2_compute(v[t][j], t & 7, tb.ic64);
     }
     for (int t = 0; t < T; t++)
       for (int j = 0; j < K1; j++) inv_c2_store(v[t][j], lds.data() + j * N, t >> 3, t & 7);
