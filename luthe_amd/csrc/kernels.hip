@@ -246,8 +246,10 @@ __global__ __launch_bounds__(256) void k_keyswitch(const uint64_t* __restrict__ 
 // b, b + gridDim.x, ...  The CMUX order i = 0..n-1 is fixed per ciphertext, so the
 // result does not depend on batch composition.
 // ------------------------------------------------------------------------------------
+// Resident workgroups per CU requested from the register allocator: 32 KB of LDS per
+// workgroup at L = 1 allows 4 (4 waves per SIMD at <= 128 VGPRs); L = 2 needs 64 KB.
 template <int K1, int L>
-__global__ __launch_bounds__(256) void k_blind_rotate(const uint64_t* __restrict__ bsk,
+__global__ __launch_bounds__(256, (L == 1 ? 4 : 2)) void k_blind_rotate(const uint64_t* __restrict__ bsk,
                                                       const uint64_t* __restrict__ luts,
                                                       const uint16_t* __restrict__ lwe2n,
                                                       const PbsJob* __restrict__ jobs, int nct,
@@ -363,9 +365,22 @@ __global__ void k_make_jobs(PbsJob* jobs, const uint32_t* gid, int n, uint32_t l
 }
 
 // ------------------------------------------------------------------------------------
-// Throughput of the Goldilocks modular multiply at full occupancy: 8 independent
-// dependency chains per thread.  Denominator of the blind rotation's VALU roofline.
+// Throughput of a minimal Goldilocks modular multiply (64x64 -> 128-bit product with
+// four 32-bit multiply-adds, then the 2^64 = 2^32 - 1, 2^96 = -1 fold with no final
+// canonicalisation) at full occupancy, 8 independent chains per thread.  Denominator of
+// the blind rotation's VALU roofline; deliberately independent of the kernels' own
+// primitives so the denominator does not move when they change.
 // ------------------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t modmul_min(uint64_t a, uint64_t b) {
+  uint64_t lo, hi;
+  gl::mul_full(a, b, lo, hi);
+  uint64_t hh = hi >> 32, hl = hi & gl::EPS;
+  uint64_t t0;
+  if (__builtin_sub_overflow(lo, hh, &t0)) t0 -= gl::EPS;
+  uint64_t t1 = (hl << 32) - hl, r;
+  if (__builtin_add_overflow(t0, t1, &r)) r += gl::EPS;
+  return r;
+}
 __global__ __launch_bounds__(256) void k_modmul_peak(uint64_t* out, uint64_t seed, int iters) {
   uint64_t x[8];
   const uint64_t y = seed | 0x8000000000000001ull;
@@ -373,7 +388,7 @@ __global__ __launch_bounds__(256) void k_modmul_peak(uint64_t* out, uint64_t see
   for (int j = 0; j < 8; j++) x[j] = seed * (uint64_t)(threadIdx.x + 977 * j + 1) + blockIdx.x;
   for (int it = 0; it < iters; it++) {
 #pragma unroll
-    for (int j = 0; j < 8; j++) x[j] = gl::mul(x[j], y);
+    for (int j = 0; j < 8; j++) x[j] = modmul_min(x[j], y);
   }
   uint64_t s = 0;
 #pragma unroll
