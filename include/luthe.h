@@ -155,6 +155,17 @@ int luthe_register_iop(luthe_ctx* ctx, uint8_t opcode, uint8_t max_align, const 
  * hpu_dop_fmt.h:148-229, then as luthe_submit_dops. */
 int luthe_submit_iop(luthe_ctx* ctx, const uint32_t* words, size_t n, uint64_t* ticket);
 
+/* Host-only utilities (no device needed).  Validate a concrete DOp stream and return
+ * each DOp's dependency level (-1 for SYNC) in `levels`; result = number of levels or
+ * a negative error.  The schedule is the one luthe_submit_dops executes. */
+int luthe_dop_schedule(const uint32_t* words, size_t n, uint32_t regfile_size, uint32_t ct_slots,
+                       uint32_t lut_slots, int32_t* levels);
+/* Translate one IOp against a DOp template program (parse_iop + patch_dop of
+ * ucore_fw.c:135-220, hpu_dop_fmt.h:148-229) without executing: out receives n_tpl
+ * concrete DOp words.  Returns 0 or a negative error. */
+int luthe_iop_translate(const uint32_t* iop_words, size_t n_iop, const uint32_t* tpl, size_t n_tpl,
+                        uint32_t ct_slots, uint32_t* out);
+
 /* ---------------------------------------------------------------- batched hot path */
 
 /* Batched PBS on caller device memory: for c < n, d_out[c*lut_nb + j] = PBS of d_in[c]

@@ -97,6 +97,8 @@ def lib():
         "luthe_set_br_grid": (i32, [vp, i32]),
         "luthe_get_br_grid": (i32, [vp]),
         "luthe_microbench_modmul": (i32, [vp, C.POINTER(C.c_double)]),
+        "luthe_dop_schedule": (i32, [vp, sz, u32, u32, u32, vp]),
+        "luthe_iop_translate": (i32, [vp, sz, vp, sz, u32, vp]),
         "luthe_keygen_secret": (i32, [P_, u64, vp, vp]),
         "luthe_keygen_bsk": (i32, [P_, u64, vp, vp, vp, i32]),
         "luthe_keygen_ksk": (i32, [P_, u64, vp, vp, vp]),
@@ -448,6 +450,27 @@ def lut_encode_many(params, tables, log_lut_nb):
     if rc != 0:
         raise LutheError(rc, "lut_encode_many")
     return out
+
+
+def dop_schedule(words, regfile_size=64, ct_slots=4096, lut_slots=256):
+    """Dependency levels of a concrete DOp stream (host only).  Returns (nlevels, levels)."""
+    w = np.ascontiguousarray(words, np.uint32)
+    lv = np.zeros(max(w.size, 1), np.int32)
+    rc = lib().luthe_dop_schedule(_vp(w), w.size, regfile_size, ct_slots, lut_slots, _vp(lv))
+    if rc < 0:
+        raise LutheError(rc, "dop_schedule")
+    return rc, lv[:w.size]
+
+
+def iop_translate(iop_words, template, ct_slots=4096):
+    """Patch a DOp template program for one IOp (host only)."""
+    iw = np.ascontiguousarray(iop_words, np.uint32)
+    tp = np.ascontiguousarray(template, np.uint32)
+    out = np.zeros(max(tp.size, 1), np.uint32)
+    rc = lib().luthe_iop_translate(_vp(iw), iw.size, _vp(tp), tp.size, ct_slots, _vp(out))
+    if rc != 0:
+        raise LutheError(rc, "iop_translate")
+    return out[:tp.size]
 
 
 def delta(params):

@@ -602,6 +602,35 @@ int luthe_wait(luthe_ctx* c, uint64_t ticket, int64_t timeout_us) {
   return LUTHE_OK;
 }
 
+int luthe_dop_schedule(const uint32_t* words, size_t n, uint32_t regs, uint32_t slots, uint32_t luts,
+                       int32_t* levels) {
+  if ((!words && n) || (!levels && n)) return LUTHE_ERR_ARG;
+  if (regs < 1 || regs > 128 || slots > 65536 || luts < 1 || luts > 4096) return LUTHE_ERR_ARG;
+  std::vector<dop::Dop> prog;
+  std::string why;
+  int rc = dop::decode_stream(words, n, regs, slots, luts, prog, why);
+  if (rc) return rc;
+  std::vector<int> lv;
+  int nlev = dop::schedule_levels(prog, regs, lv);
+  for (size_t i = 0; i < n; i++) levels[i] = lv[i];
+  return nlev;
+}
+
+int luthe_iop_translate(const uint32_t* iop_words, size_t n_iop, const uint32_t* tpl, size_t n_tpl,
+                        uint32_t ct_slots, uint32_t* out) {
+  if (!iop_words || n_iop == 0 || (!tpl && n_tpl) || (!out && n_tpl) || ct_slots == 0 || ct_slots > 65536)
+    return LUTHE_ERR_ARG;
+  dop::IopBundle iop;
+  std::string why;
+  int rc = dop::parse_iop(iop_words, n_iop, iop, why);
+  if (rc) return rc;
+  std::vector<uint32_t> patched;
+  rc = dop::patch_program(std::vector<uint32_t>(tpl, tpl + n_tpl), iop, ct_slots, patched, why);
+  if (rc) return rc;
+  for (size_t i = 0; i < n_tpl; i++) out[i] = patched[i];
+  return LUTHE_OK;
+}
+
 // ---------------------------------------------------------------- IOp path
 int luthe_register_iop(luthe_ctx* c, uint8_t opcode, uint8_t max_align, const uint32_t* dops, size_t n) {
   if (!c || (!dops && n)) return LUTHE_ERR_ARG;

@@ -490,6 +490,10 @@ void ref_decompose(uint64_t x, uint32_t w, uint32_t b, uint32_t L, int32_t* digi
   }
 }
 
+void ref_decompose_batch(const uint64_t* x, size_t n, uint32_t w, uint32_t b, uint32_t L, int32_t* digits) {
+  for (size_t i = 0; i < n; i++) ref_decompose(x[i], w, b, L, digits + i * L);
+}
+
 /* ==================================================================================== */
 /* Modulus switches                                                                       */
 /* ==================================================================================== */

@@ -79,6 +79,7 @@ void ref_negacyclic_mul(uint32_t N, const uint64_t* a, const uint64_t* b, uint64
 /* ---- decomposition (Appendix: closest representable + balanced signed digits) ----
  * digits[0] is the LSB level, digits[L-1] the MSB level. */
 void ref_decompose(uint64_t x, uint32_t w, uint32_t base_log, uint32_t level, int32_t* digits);
+void ref_decompose_batch(const uint64_t* x, size_t n, uint32_t w, uint32_t base_log, uint32_t level, int32_t* digits);
 
 /* ---- modulus switches ---- */
 uint64_t ref_modsw_p_to_2_64(uint64_t e);                  /* mod_switch_to_2powerN, PRECISION_W=96 */

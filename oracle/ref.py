@@ -47,6 +47,7 @@ def lib():
             ("ref_ntt_polymul_batch", C.c_int, [u32, u32p, u32, sz, u64p, u64p, u64p, C.c_int]),
             ("ref_negacyclic_mul", None, [u32, u64p, u64p, u64p]),
             ("ref_decompose", None, [u64, u32, u32, u32, i32p]),
+            ("ref_decompose_batch", None, [u64p, sz, u32, u32, u32, i32p]),
             ("ref_modsw_p_to_2_64", u64, [u64]),
             ("ref_ks_body_switch", u32, [u64, u32]),
             ("ref_ks_to_2n", u32, [u32, u32, u32]),
@@ -142,6 +143,13 @@ def decompose(x, w, base_log, level):
     d = np.zeros(level, np.int32)
     lib().ref_decompose(int(x), w, base_log, level, d)
     return [int(v) for v in d]
+
+
+def decompose_batch(x, w, base_log, level):
+    x = _u64(x)
+    out = np.zeros((x.size, level), np.int32)
+    lib().ref_decompose_batch(x, x.size, w, base_log, level, out)
+    return out
 
 
 def modsw(e):

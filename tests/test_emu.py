@@ -1,0 +1,113 @@
+"""The HIP kernels' per-thread code (ntt2048.h, pbs_core.h, gl64.h), executed by a
+sequential CPU emulation of the 256-thread workgroup (tests/emu/kernel_emu.cpp),
+against the oracle.  Checks index mappings, LDS layouts, twiddles and arithmetic of
+the device code without a GPU."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from oracle import ref
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+P = ref.P
+rng = np.random.default_rng(777)
+
+
+@pytest.fixture(scope="module")
+def emu():
+    path = os.path.join(HERE, "emu", "libkernel_emu.so")
+    if not os.path.exists(path):
+        import subprocess
+        subprocess.check_call(["make", "-s", "-C", os.path.join(HERE, "emu")])
+    E = C.CDLL(path)
+    u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+    E.emu_ntt_fwd.argtypes = [u64p, u64p]
+    E.emu_ntt_inv.argtypes = [u64p, u64p]
+    for f in ("emu_gl_add", "emu_gl_sub", "emu_gl_mul", "emu_gl_reduce128", "emu_gl_mul_pow2", "emu_modswitch"):
+        getattr(E, f).restype = C.c_uint64
+    E.emu_gl_mul_pow2.argtypes = [C.c_uint64, C.c_uint32]
+    E.emu_modswitch.argtypes = [C.c_uint64]
+    for f in ("emu_gl_add", "emu_gl_sub", "emu_gl_mul", "emu_gl_reduce128"):
+        getattr(E, f).argtypes = [C.c_uint64, C.c_uint64]
+    return E
+
+
+def test_device_arithmetic(emu):
+    L = ref.lib()
+    canon = [0, 1, 2, P - 1, P - 2, 2**32, 2**32 - 1, 2**63] + [int(v) for v in rng.integers(0, P, 3000, dtype=np.uint64)]
+    anyv = [2**64 - 1, 2**64 - 2, P, P + 5] + canon
+    for a, b in zip(canon[:1500], canon[1500:3000]):
+        assert emu.emu_gl_add(a, b) == (a + b) % P
+        assert emu.emu_gl_sub(a, b) == (a - b) % P
+    for a, b in zip(anyv[:1500], anyv[1500:3000]):
+        assert emu.emu_gl_mul(a, b) == (a * b) % P
+        assert emu.emu_gl_reduce128(a, b) == (a + (b << 64)) % P
+    for x in anyv[:400]:
+        for s in (0, 1, 3, 12, 24, 31, 32, 33, 48, 63, 64, 65, 72, 84, 93, 95, 96, 120, 144, 168, 191):
+            assert emu.emu_gl_mul_pow2(x, s) == (x << s) % P
+    for e in canon:
+        assert emu.emu_modswitch(e) == ref.modsw(e)
+
+
+def test_kernel_ntt_matches_oracle(emu):
+    for trial in range(4):
+        a = rng.integers(0, P, 2048, dtype=np.uint64)
+        if trial == 1:
+            a[:] = P - 1
+        if trial == 2:
+            a[:] = 0
+            a[3] = 1
+        f = np.empty_like(a)
+        emu.emu_ntt_fwd(a, f)
+        assert np.array_equal(f, ref.ntt(a))
+        back = np.empty_like(a)
+        emu.emu_ntt_inv(f, back)
+        assert np.array_equal(back, a)
+        x = rng.integers(0, P, 2048, dtype=np.uint64)
+        xi = np.empty_like(x)
+        emu.emu_ntt_inv(x, xi)
+        assert np.array_equal(xi, ref.intt(x))
+
+
+@pytest.mark.parametrize("L,base_log", [(1, 23), (2, 15)])
+def test_kernel_blind_rotation_matches_oracle(emu, L, base_log):
+    n, K1, N = 5, 2, 2048
+    bsk = rng.integers(0, P, (n, K1, L, K1, N), dtype=np.uint64)
+    lut = rng.integers(0, 2**64, (K1, N), dtype=np.uint64)
+    a2n = rng.integers(0, 4096, n + 1, dtype=np.uint16)
+    a2n[:4] = [0, 2047, 2048, 4095]
+    prm = ref.params(lwe_dim=n, pbs_base_log=base_log, pbs_level=L)
+    exp = ref.blind_rotate(prm, ref.bsk_to_ntt(prm, bsk.reshape(-1)), lut.reshape(-1), a2n)
+    bint = np.empty(n * 8 * K1 * L * K1 * 256, np.uint64)
+    vp = C.c_void_p
+    emu.emu_bsk_to_internal(C.c_int(n), C.c_int(L), np.ascontiguousarray(bsk.reshape(-1)).ctypes.data_as(vp),
+                            bint.ctypes.data_as(vp))
+    acc = np.empty(K1 * N, np.uint64)
+    assert emu.emu_blind_rotate(C.c_int(n), C.c_int(L), bint.ctypes.data_as(vp),
+                                np.ascontiguousarray(lut.reshape(-1)).ctypes.data_as(vp),
+                                a2n.ctypes.data_as(vp), C.c_uint32(base_log), acc.ctypes.data_as(vp)) == 0
+    assert np.array_equal(acc, exp)
+
+
+def test_kernel_decomposition_and_extract(emu):
+    vp = C.c_void_p
+    emu.emu_decompose.argtypes = [C.c_uint64, C.c_uint32, C.c_int, vp]
+    xs = [int(v) for v in rng.integers(0, 2**64, 3000, dtype=np.uint64)] + [0, 2**63, 2**64 - 1, 2**40, 2**41 - 1]
+    for Lv, b in [(1, 23), (2, 15), (5, 3), (3, 7), (4, 2)]:
+        d = np.zeros(Lv, np.int64)
+        for x in xs:
+            emu.emu_decompose(x, b, Lv, d.ctypes.data_as(vp))
+            assert list(d) == ref.decompose(x, 64, b, Lv)
+    emu.emu_extract_coef.restype = C.c_uint64
+    emu.emu_extract_coef.argtypes = [vp, C.c_int, C.c_int, C.c_uint32, C.c_uint32]
+    acc = rng.integers(0, 2**64, 2 * 2048, dtype=np.uint64)
+    for b in (0, 1, 2047, 2048, 3000, 4095):
+        for log_nb in (0, 2, 3):
+            exp = ref.sample_extract(1, 2048, acc, b, log_nb)
+            for lut in range(1 << log_nb):
+                offs = lut * (2048 >> log_nb)
+                for m in (0, 1, 100, 2047):
+                    assert emu.emu_extract_coef(acc.ctypes.data_as(vp), 0, m, b, offs) == int(exp[lut][m])
+                assert emu.emu_extract_coef(acc.ctypes.data_as(vp), 1, 0, b, offs) == int(exp[lut][2048])
