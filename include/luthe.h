@@ -197,6 +197,26 @@ int luthe_ntt_polymul_batch(luthe_ctx* ctx, const uint64_t* d_a, const uint64_t*
 int luthe_ntt_forward_batch(luthe_ctx* ctx, const uint64_t* d_in, uint64_t* d_out, size_t npoly, void* stream);
 int luthe_ntt_inverse_batch(luthe_ctx* ctx, const uint64_t* d_in, uint64_t* d_out, size_t npoly, void* stream);
 
+/* Batched ALU DOp over contiguous BLWE arrays: d_out[c] = op(d_a[c], d_b[c]) with the
+ * semantics of pea_alu_core.sv:55-160 (opcode = 6-bit DOp code, d_b unused by the
+ * immediate forms).  Replaces the ALU processing element pe_alu for batch hosts. */
+int luthe_alu_batch(luthe_ctx* ctx, uint32_t opcode, const uint64_t* d_a, const uint64_t* d_b, uint64_t* d_out,
+                    uint32_t mul_factor, uint32_t msg_cst, size_t n, void* stream);
+
+/* Pre-size the per-context batch scratch for batches of up to max_batch ciphertexts,
+ * so later batch calls allocate nothing (required before graph capture). */
+int luthe_reserve(luthe_ctx* ctx, size_t max_batch);
+
+/* HIP graph capture of batch calls issued on the context stream (stream argument NULL):
+ * begin, issue luthe_pbs_batch / luthe_alu_batch / NTT calls, end; then replay with
+ * luthe_graph_launch.  DOp submissions cannot be captured.  Shapes and pointers are
+ * fixed at capture time. */
+typedef struct luthe_graph luthe_graph;
+int luthe_capture_begin(luthe_ctx* ctx);
+int luthe_capture_end(luthe_ctx* ctx, luthe_graph** out);
+int luthe_graph_launch(luthe_ctx* ctx, luthe_graph* g);
+int luthe_graph_destroy(luthe_graph* g);
+
 /* Synchronise the context stream. */
 int luthe_sync(luthe_ctx* ctx);
 /* The context's HIP stream (hipStream_t), on which DOp submissions and NULL-stream

@@ -98,6 +98,12 @@ def lib():
         "luthe_get_br_grid": (i32, [vp]),
         "luthe_microbench_modmul": (i32, [vp, C.POINTER(C.c_double)]),
         "luthe_dop_schedule": (i32, [vp, sz, u32, u32, u32, vp]),
+        "luthe_alu_batch": (i32, [vp, u32, vp, vp, vp, u32, u32, sz, vp]),
+        "luthe_reserve": (i32, [vp, sz]),
+        "luthe_capture_begin": (i32, [vp]),
+        "luthe_capture_end": (i32, [vp, C.POINTER(vp)]),
+        "luthe_graph_launch": (i32, [vp, vp]),
+        "luthe_graph_destroy": (i32, [vp]),
         "luthe_iop_translate": (i32, [vp, sz, vp, sz, u32, vp]),
         "luthe_keygen_secret": (i32, [P_, u64, vp, vp]),
         "luthe_keygen_bsk": (i32, [P_, u64, vp, vp, vp, i32]),
@@ -208,6 +214,10 @@ class DeviceBuffer:
 def _vp(a):
     if isinstance(a, DeviceBuffer):
         return a.ptr
+    if isinstance(a, C.c_void_p):
+        return a
+    if isinstance(a, int):
+        return C.c_void_p(a)
     return a.ctypes.data_as(C.c_void_p)
 
 
@@ -334,6 +344,21 @@ class Context:
         return v.value
 
     # ---- batched device-memory entry points
+    def alu_batch_dev(self, opcode, d_a, d_b, d_out, n, mul_factor=0, msg_cst=0, stream=None):
+        self._chk(lib().luthe_alu_batch(self._h, opcode, _vp(d_a), _vp(d_b) if d_b is not None else None,
+                                        _vp(d_out), mul_factor, msg_cst, n, stream), "alu_batch")
+
+    def reserve(self, max_batch):
+        self._chk(lib().luthe_reserve(self._h, max_batch), "reserve")
+
+    def capture_begin(self):
+        self._chk(lib().luthe_capture_begin(self._h), "capture_begin")
+
+    def capture_end(self):
+        g = C.c_void_p()
+        self._chk(lib().luthe_capture_end(self._h, C.byref(g)), "capture_end")
+        return Graph(self, g)
+
     def pbs_batch_dev(self, d_in, d_out, d_gid, n, log_lut_nb=0, stream=None):
         self._chk(lib().luthe_pbs_batch(self._h, _vp(d_in), _vp(d_out), _vp(d_gid), log_lut_nb, n, stream),
                   "pbs_batch")
@@ -391,6 +416,27 @@ class Context:
         self._chk(lib().luthe_ntt_inverse_batch(self._h, d_a.ptr, d_o.ptr, a.shape[0], None), "ntt_inverse")
         self.sync()
         return d_o.numpy()
+
+
+class Graph:
+    """A captured sequence of batch calls, replayed on the context stream."""
+
+    def __init__(self, ctx, handle):
+        self.ctx, self._g = ctx, handle
+
+    def launch(self):
+        self.ctx._chk(lib().luthe_graph_launch(self.ctx.handle, self._g), "graph_launch")
+
+    def destroy(self):
+        if self._g:
+            lib().luthe_graph_destroy(self._g)
+            self._g = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------- client-side helpers

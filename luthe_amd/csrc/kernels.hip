@@ -347,6 +347,28 @@ __global__ __launch_bounds__(256) void k_alu(uint64_t* __restrict__ base, const 
   base[(size_t)op.dst_row * blen + e] = z;
 }
 
+__global__ __launch_bounds__(256) void k_alu_batch(uint32_t opcode, const uint64_t* __restrict__ a,
+                                                   const uint64_t* __restrict__ b, uint64_t* __restrict__ out,
+                                                   uint32_t mul_factor, uint32_t msg_cst, int blen) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= blen) return;
+  const size_t off = (size_t)blockIdx.y * blen + e;
+  const uint64_t x = a[off];
+  const uint64_t y = (opcode == 0x01 || opcode == 0x02 || opcode == 0x05) ? b[off] : 0;
+  const uint64_t cst = (e == blen - 1) ? ((uint64_t)(msg_cst & 31u) << 59) : 0;
+  uint64_t z;
+  switch (opcode) {
+    case 0x01: z = x + y; break;
+    case 0x02: z = x - y; break;
+    case 0x05: z = x * (uint64_t)(mul_factor & 31u) + y; break;
+    case 0x09: z = x + cst; break;
+    case 0x0A: z = x - cst; break;
+    case 0x0B: z = cst - x; break;
+    default: z = x * (uint64_t)(msg_cst & 31u); break;
+  }
+  out[off] = z;
+}
+
 __global__ __launch_bounds__(256) void k_copy(uint64_t* __restrict__ base, const CopyOp* __restrict__ ops, int blen) {
   const CopyOp op = ops[blockIdx.y];
   const int e = blockIdx.x * 256 + threadIdx.x;
@@ -471,6 +493,14 @@ int blind_rotate_occupancy(int pbs_level) {
 hipError_t launch_alu(uint64_t* base, const AluOp* ops, int nops, int blen, hipStream_t s) {
   if (nops == 0) return hipSuccess;
   k_alu<<<dim3((unsigned)((blen + 255) / 256), (unsigned)nops), 256, 0, s>>>(base, ops, blen);
+  return hipGetLastError();
+}
+
+hipError_t launch_alu_batch(uint32_t opcode, const uint64_t* a, const uint64_t* b, uint64_t* out, uint32_t mul_factor,
+                            uint32_t msg_cst, int n, int blen, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  k_alu_batch<<<dim3((unsigned)((blen + 255) / 256), (unsigned)n), 256, 0, s>>>(opcode, a, b, out, mul_factor, msg_cst,
+                                                                                  blen);
   return hipGetLastError();
 }
 

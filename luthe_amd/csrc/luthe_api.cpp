@@ -38,6 +38,11 @@ struct Ticket {
   hipEvent_t done = nullptr;
 };
 
+struct luthe_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
 struct luthe_ctx {
   luthe_params prm{};
   int device = 0;
@@ -456,6 +461,63 @@ int luthe_pbs_batch(luthe_ctx* c, const uint64_t* d_in, uint64_t* d_out, const u
   if (rc) return rc;
   HIPCHK(c, lk::launch_make_jobs(c->d_jobs, d_gid, (int)n, log_lut_nb, c->prm.lut_slots, s));
   return run_pbs(c, d_in, c->d_jobs, (int)n, d_out, s);
+}
+
+int luthe_alu_batch(luthe_ctx* c, uint32_t opcode, const uint64_t* d_a, const uint64_t* d_b, uint64_t* d_out,
+                    uint32_t mul_factor, uint32_t msg_cst, size_t n, void* stream) {
+  if (!c || (n && (!d_a || !d_out))) return LUTHE_ERR_ARG;
+  bool two = opcode == dop::OP_ADD || opcode == dop::OP_SUB || opcode == dop::OP_MAC;
+  bool one = opcode == dop::OP_ADDS || opcode == dop::OP_SUBS || opcode == dop::OP_SSUB || opcode == dop::OP_MULS;
+  if (!two && !one) return fail(c, LUTHE_ERR_DOP, "opcode 0x%02x is not an ALU DOp", opcode);
+  if (two && n && !d_b) return LUTHE_ERR_ARG;
+  if (n > (size_t)1 << 28) return fail(c, LUTHE_ERR_ARG, "batch too large");
+  HIPCHK(c, lk::launch_alu_batch(opcode, d_a, d_b, d_out, mul_factor, msg_cst, (int)n, (int)c->blen,
+                                 pick_stream(c, stream)));
+  c->cnt.alu += n;
+  return LUTHE_OK;
+}
+
+int luthe_reserve(luthe_ctx* c, size_t max_batch) {
+  if (!c) return LUTHE_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  int rc = ensure_jobs(c, max_batch);
+  if (rc) return rc;
+  return ensure_lwe2n(c, max_batch);
+}
+
+int luthe_capture_begin(luthe_ctx* c) {
+  if (!c) return LUTHE_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  return LUTHE_OK;
+}
+
+int luthe_capture_end(luthe_ctx* c, luthe_graph** out) {
+  if (!c || !out) return LUTHE_ERR_ARG;
+  *out = nullptr;
+  luthe_graph* g = new luthe_graph();
+  hipError_t e = hipStreamEndCapture(c->stream, &g->graph);
+  if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    luthe_graph_destroy(g);
+    return fail(c, LUTHE_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+  }
+  *out = g;
+  return LUTHE_OK;
+}
+
+int luthe_graph_launch(luthe_ctx* c, luthe_graph* g) {
+  if (!c || !g || !g->exec) return LUTHE_ERR_ARG;
+  HIPCHK(c, hipGraphLaunch(g->exec, c->stream));
+  return LUTHE_OK;
+}
+
+int luthe_graph_destroy(luthe_graph* g) {
+  if (!g) return LUTHE_ERR_ARG;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
+  return LUTHE_OK;
 }
 
 int luthe_ntt_polymul_batch(luthe_ctx* c, const uint64_t* d_a, const uint64_t* d_bn, uint64_t* d_out, size_t npoly,

@@ -1,0 +1,195 @@
+"""GPU: config-3 parameters (PBS level 2, base 2^15), Many-LUT PBS, batched ALU,
+HIP-graph capture of an uneven batch program, IOp submission and error paths --
+all bit-exact against the oracle through the C ABI."""
+import os
+
+import numpy as np
+import pytest
+
+import luthe_amd as la
+from luthe_amd import dop
+
+pytestmark = pytest.mark.gpu
+THREADS = min(16, os.cpu_count() or 1)
+
+
+@pytest.fixture(scope="module")
+def cfg3(oracle):
+    """Config 3 parameters: n=724, k=1, N=2048, PBS base 2^15 level 2, KS 2^3 x 5."""
+    prm = la.default_params(pbs_base_log=15, pbs_level=2, lut_slots=512)
+    ctx = la.Context(prm, device=0)
+    keys = la.Keys(prm, seed=303)
+    ctx.upload_bsk(keys.bsk)
+    ctx.upload_ksk(keys.ksk)
+    rp = oracle.params(pbs_base_log=15, pbs_level=2)
+    bsk_ntt = oracle.bsk_to_ntt(rp, keys.bsk.reshape(-1), threads=THREADS)
+    yield prm, ctx, keys, rp, bsk_ntt
+    ctx.close()
+
+
+def test_pbs_level2_vs_oracle(cfg3, oracle):
+    print("case: PBS L=2 B=2^15, 12 ciphertexts, per-ciphertext LUTs", flush=True)
+    prm, ctx, keys, rp, bsk_ntt = cfg3
+    rng = np.random.default_rng(31)
+    tables = rng.integers(0, 16, (12, 16), dtype=np.uint8)
+    luts = np.stack([la.lut_encode(prm, t) for t in tables])
+    ctx.write_lut(0, luts)
+    msgs = rng.integers(0, 16, 12)
+    cts = keys.encrypt_msg(msgs, seed=5)
+    gids = np.arange(12, dtype=np.uint32)
+    got = ctx.pbs_batch(cts, gids)
+    exp = oracle.pbs_batch(rp, bsk_ntt, keys.ksk, luts, gids, cts, threads=THREADS)
+    assert np.array_equal(got, exp)
+    assert (keys.decrypt_msg(got) == tables[gids, msgs]).mean() >= 0.75
+
+
+@pytest.mark.parametrize("log_nb", [1, 2, 3])
+def test_many_lut_vs_oracle(cfg3, oracle, log_nb):
+    print(f"case: Many-LUT log_lut_nb={log_nb}", flush=True)
+    prm, ctx, keys, rp, bsk_ntt = cfg3
+    rng = np.random.default_rng(40 + log_nb)
+    nb, tlen = 1 << log_nb, 16 >> log_nb
+    tabs = rng.integers(0, 16, (nb, tlen), dtype=np.uint8)
+    lut = la.lut_encode_many(prm, tabs, log_nb)
+    ctx.write_lut(100, lut)
+    msgs = rng.integers(0, tlen, 4)
+    cts = keys.encrypt_msg(msgs, seed=log_nb)
+    got = ctx.pbs_batch(cts, np.full(4, 100, np.uint32), log_lut_nb=log_nb)
+    assert got.shape == (4 * nb, prm.blen)
+    for c in range(4):
+        _, a2n = oracle.keyswitch(rp, cts[c], keys.ksk)
+        acc = oracle.blind_rotate(rp, bsk_ntt, lut, a2n)
+        exp = oracle.sample_extract(1, 2048, acc, int(a2n[-1]), log_nb)
+        assert np.array_equal(got[c * nb:(c + 1) * nb], exp), c
+    dec = keys.decrypt_msg(got).reshape(4, nb)
+    assert (dec == tabs[:, msgs].T).mean() >= 0.75
+
+
+def test_alu_batch_vs_oracle(cfg3, oracle):
+    print("case: ALU batch, all opcodes", flush=True)
+    prm, ctx, _, _, _ = cfg3
+    rng = np.random.default_rng(50)
+    n = 33
+    a = rng.integers(0, 2**64, (n, prm.blen), dtype=np.uint64)
+    b = rng.integers(0, 2**64, (n, prm.blen), dtype=np.uint64)
+    da, db = la.DeviceBuffer.from_numpy(a), la.DeviceBuffer.from_numpy(b)
+    dout = la.DeviceBuffer((n, prm.blen), np.uint64)
+    for op in (dop.ADD, dop.SUB, dop.MAC, dop.ADDS, dop.SUBS, dop.SSUB, dop.MULS):
+        ctx.alu_batch_dev(op, da, db, dout, n, mul_factor=19, msg_cst=0x15)
+        ctx.sync()
+        got = dout.numpy()
+        for c in (0, 17, 32):
+            exp = oracle.alu(op, a[c], b[c], mul_factor=19, msg_cst=0x15)
+            assert np.array_equal(got[c], exp), hex(op)
+
+
+def test_uneven_batches_graph_replay(cfg3, oracle):
+    """Config-3 shape at reduced size: batches of uneven sizes, per-ciphertext LUTs, an
+    ADD and a MAC before each PBS, captured once as a HIP graph and replayed three times
+    with eager work in between; every replay equals the eager result, which equals the
+    oracle on sampled ciphertexts."""
+    print("case: uneven batches captured as a HIP graph", flush=True)
+    prm, ctx, keys, rp, bsk_ntt = cfg3
+    rng = np.random.default_rng(60)
+    sizes = [1, 37, 5, 120, 64, 3]
+    total = sum(sizes)
+    tables = rng.integers(0, 16, (64, 16), dtype=np.uint8)
+    luts = np.stack([la.lut_encode(prm, t) for t in tables])
+    ctx.write_lut(200, luts)
+    ma, mb = rng.integers(0, 4, total), rng.integers(0, 4, total)
+    A, B = keys.encrypt_msg(ma, seed=61), keys.encrypt_msg(mb, seed=62)
+    gids = (200 + rng.integers(0, 64, total)).astype(np.uint32)
+    dA, dB = la.DeviceBuffer.from_numpy(A), la.DeviceBuffer.from_numpy(B)
+    dT, dU = la.DeviceBuffer((total, prm.blen), np.uint64), la.DeviceBuffer((total, prm.blen), np.uint64)
+    dG = la.DeviceBuffer.from_numpy(gids)
+    dO = la.DeviceBuffer((total, prm.blen), np.uint64)
+    ctx.reserve(max(sizes))
+    blen8 = prm.blen * 8
+
+    def program():
+        off = 0
+        for s in sizes:
+            a_p, b_p = dA.ptr.value + off * blen8, dB.ptr.value + off * blen8
+            t_p, u_p, o_p = dT.ptr.value + off * blen8, dU.ptr.value + off * blen8, dO.ptr.value + off * blen8
+            g_p = dG.ptr.value + off * 4
+            ctx.alu_batch_dev(dop.ADD, la.C.c_void_p(a_p), la.C.c_void_p(b_p), la.C.c_void_p(t_p), s)
+            ctx.alu_batch_dev(dop.MAC, la.C.c_void_p(t_p), la.C.c_void_p(b_p), la.C.c_void_p(u_p), s, mul_factor=1)
+            ctx.pbs_batch_dev(la.C.c_void_p(u_p), la.C.c_void_p(o_p), la.C.c_void_p(g_p), s)
+            off += s
+
+    program()
+    ctx.sync()
+    eager = dO.numpy()
+    ctx.capture_begin()
+    program()
+    g = ctx.capture_end()
+    for rep in range(3):
+        dO.zero()
+        g.launch()
+        ctx.sync()
+        assert np.array_equal(dO.numpy(), eager), f"replay {rep}"
+        # eager work between replays
+        ctx.pbs_batch(A[:3], gids[:3])
+    g.destroy()
+    for c in (0, 1, 40, total - 1):
+        t = oracle.alu(dop.ADD, A[c], B[c])
+        u = oracle.alu(dop.MAC, t, B[c], mul_factor=1)
+        exp = oracle.pbs_batch(rp, bsk_ntt, keys.ksk, luts, np.array([gids[c] - 200], np.uint32), u.reshape(1, -1),
+                               threads=1)[0]
+        assert np.array_equal(eager[c], exp), c
+
+
+def test_iop_submission(cfg3, oracle):
+    """An IOp with a registered DOp template program behaves like the patched DOps."""
+    print("case: IOp path", flush=True)
+    prm, ctx, keys, rp, bsk_ntt = cfg3
+    rng = np.random.default_rng(70)
+    lut = la.lut_encode(prm, (np.arange(16) * 7 % 16).astype(np.uint8))
+    ctx.write_lut(300, lut)
+    # template: dst block j = PBS(src0 block j + src1 block j) for 2 blocks
+    tpl = []
+    for j in range(2):
+        tpl += [dop.ld(0, dop.tslot(0, j), dop.MEM_SRC), dop.ld(1, dop.tslot(1, j), dop.MEM_SRC),
+                dop.add(2, 0, 1), dop.pbs(3, 2, 300, flush=(j == 1)), dop.st(dop.tslot(0, j), 3, dop.MEM_DST)]
+    ctx.register_iop(0x20, 2, tpl)
+    x, y = keys.encrypt_msg([1, 2], seed=71), keys.encrypt_msg([3, 0], seed=72)
+    ctx.write_ct(500, x)
+    ctx.write_ct(600, y)
+    t = ctx.submit_iop(dop.iop_words(0x20, [(700, 2)], [(500, 2), (600, 2)]))
+    ctx.wait(t)
+    out = ctx.read_ct(700, 2)
+    for j in range(2):
+        s = oracle.alu(dop.ADD, x[j], y[j])
+        exp = oracle.pbs_batch(rp, bsk_ntt, keys.ksk, lut.reshape(1, -1), np.zeros(1, np.uint32), s.reshape(1, -1),
+                               threads=1)[0]
+        assert np.array_equal(out[j], exp)
+    assert ctx.counters()["iops"] == 1
+
+
+def test_error_paths(oracle):
+    print("case: error paths", flush=True)
+    prm = la.default_params(lwe_dim=8)
+    ctx = la.Context(prm, device=0)
+    cts = np.zeros((2, prm.blen), np.uint64)
+    with pytest.raises(la.LutheError) as e:          # keys missing
+        ctx.pbs_batch(cts, np.zeros(2, np.uint32))
+    assert e.value.code == -7
+    with pytest.raises(la.LutheError) as e:          # unknown ticket
+        ctx.wait(12345)
+    assert e.value.code == -7
+    with pytest.raises(la.LutheError) as e:          # malformed stream: nothing runs
+        ctx.submit_dops([dop.ld(0, 1), 0x3F << 26])
+    assert e.value.code == -5
+    with pytest.raises(la.LutheError) as e:          # slot out of range
+        ctx.write_ct(4096, cts[:1])
+    assert e.value.code == -6
+    # a LUT id beyond the LUT memory is clamped, never read out of bounds
+    keys = la.Keys(prm, seed=1)
+    ctx.upload_bsk(keys.bsk)
+    ctx.upload_ksk(keys.ksk)
+    lut = la.lut_encode(prm, np.arange(16, dtype=np.uint8))
+    ctx.write_lut(prm.lut_slots - 1, lut)
+    ok = ctx.pbs_batch(keys.encrypt_msg([3]), np.array([prm.lut_slots - 1], np.uint32))
+    clamped = ctx.pbs_batch(keys.encrypt_msg([3]), np.array([999999], np.uint32))
+    assert np.array_equal(ok, clamped)
+    ctx.close()
