@@ -66,6 +66,85 @@ def bsk_bytes(p):
     return p.lwe_dim * k1 * p.pbs_level * k1 * p.poly_size * 8
 
 
+def run_config1(la, hip, ctx, peak, npoly=4096, reps=5):
+    """Config 1: NTT -> pointwise multiply by an NTT-domain set -> INTT, 4096 x 2048."""
+    rng = np.random.default_rng(1)
+    a = rng.integers(0, la.P, (npoly, 2048), dtype=np.uint64)
+    bn = rng.integers(0, la.P, (npoly, 2048), dtype=np.uint64)
+    da, db = la.DeviceBuffer.from_numpy(a), la.DeviceBuffer.from_numpy(bn)
+    do = la.DeviceBuffer((npoly, 2048), np.uint64)
+    tm = Timer(hip, ctx.stream)
+    la.lib().luthe_ntt_polymul_batch(ctx.handle, da.ptr, db.ptr, do.ptr, npoly, None)
+    ctx.sync()
+    tm.start()
+    for _ in range(reps):
+        la.lib().luthe_ntt_polymul_batch(ctx.handle, da.ptr, db.ptr, do.ptr, npoly, None)
+    ms = tm.stop_ms() / reps
+    work = npoly * (2 * 11264 + 2048)
+    nbytes = 3 * npoly * 2048 * 8
+    return {"workload": f"NTT, pointwise x NTT-domain operand, INTT: {npoly} polynomials x 2048, uniform < p",
+            "ms_per_batch": round(ms, 4), "polys_per_s": round(npoly / (ms * 1e-3), 1),
+            "hbm_gbps": round(nbytes / (ms * 1e-3) / 1e9, 1), "hbm_frac": round(nbytes / (ms * 1e-3) / 8e12, 4),
+            "gmodmul_per_s": round(work / (ms * 1e-3) / 1e9, 1), "valu_frac": round(work / (ms * 1e-3) / 1e9 / peak, 4)}
+
+
+def run_config3(la, hip, seed=3):
+    """Config 3: 64 batches, sizes uniform in [1, 4096]; per batch an ADD and a MAC then a
+    PBS with a per-ciphertext LUT (random 16-entry table); PBS level 2, base 2^15; the
+    whole 64-batch program captured as one HIP graph and replayed."""
+    prm = la.default_params(pbs_base_log=15, pbs_level=2, lut_slots=4096)
+    rng = np.random.default_rng(seed)
+    sizes = [int(x) for x in rng.integers(1, 4097, 64)]
+    total = sum(sizes)
+    log(f"bench: config 3, {total} PBS in 64 batches, keygen L=2")
+    keys = la.Keys(prm, seed=5678)
+    ctx = la.Context(prm, device=0)
+    ctx.upload_bsk(keys.bsk)
+    ctx.upload_ksk(keys.ksk)
+    tables = np.random.default_rng(seed + 1).integers(0, 16, (4096, 16), dtype=np.uint8)
+    luts = np.stack([la.lut_encode(prm, t) for t in tables])
+    ctx.write_lut(0, luts)
+    ma, mb = rng.integers(0, 4, total), rng.integers(0, 4, total)
+    dA = la.DeviceBuffer.from_numpy(keys.encrypt_msg(ma, seed=11))
+    dB = la.DeviceBuffer.from_numpy(keys.encrypt_msg(mb, seed=12))
+    dT = la.DeviceBuffer((total, prm.blen), np.uint64)
+    dU = la.DeviceBuffer((total, prm.blen), np.uint64)
+    dO = la.DeviceBuffer((total, prm.blen), np.uint64)
+    gids = np.concatenate([np.arange(s, dtype=np.uint32) for s in sizes])
+    dG = la.DeviceBuffer.from_numpy(gids)
+    ctx.reserve(max(sizes))
+    b8 = prm.blen * 8
+
+    def program():
+        off = 0
+        for s in sizes:
+            a_p, b_p, t_p = dA.ptr.value + off * b8, dB.ptr.value + off * b8, dT.ptr.value + off * b8
+            u_p, o_p, g_p = dU.ptr.value + off * b8, dO.ptr.value + off * b8, dG.ptr.value + off * 4
+            ctx.alu_batch_dev(0x01, a_p, b_p, t_p, s)
+            ctx.alu_batch_dev(0x05, t_p, b_p, u_p, s, mul_factor=2)
+            ctx.pbs_batch_dev(u_p, o_p, g_p, s)
+            off += s
+
+    ctx.capture_begin()
+    program()
+    g = ctx.capture_end()
+    tm = Timer(hip, ctx.stream)
+    g.launch()          # warm
+    ctx.sync()
+    tm.start()
+    g.launch()
+    ms = tm.stop_ms()
+    out = dO.numpy()
+    exp = tables[gids, 2 * (ma + mb) + mb]   # MAC: (a + b) * 2 + b
+    ok = float((keys.decrypt_msg(out) == exp).mean())
+    g.destroy()
+    ctx.close()
+    return {"workload": "64 uneven batches (sizes uniform in [1,4096]), ADD+MAC then PBS with per-ciphertext "
+                        "LUTs, PBS 2^15 x 2, one HIP graph",
+            "pbs": total, "ms": round(ms, 2), "pbs_per_s": round(total / (ms * 1e-3), 1),
+            "decrypt_success": round(ok, 4)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,6 +155,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--br-grid", type=int, default=0, help="blind-rotation workgroups (0 = occupancy x CUs)")
+    ap.add_argument("--no-extra", action="store_true", help="skip the config-1 and config-3 measurements")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="only run a device-to-device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -185,6 +265,12 @@ def main():
                                          "+ (k+1)^2 L x N + (k+1) N)",
                      "bsk_stream_gbps": round(bsk_bytes(prm) / (br_ms * 1e-3) / 1e9, 2)},
     }
+
+    if not args.no_extra:
+        out["config1_ntt"] = run_config1(la, hip, ctx, peak)
+        log(f"bench: config 1 {out['config1_ntt']}")
+        out["config3_uneven"] = run_config3(la, hip)
+        log(f"bench: config 3 {out['config3_uneven']}")
 
     if not args.no_cpu_baseline:
         from oracle import ref  # CPU baseline leg only
