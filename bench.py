@@ -8,6 +8,9 @@ LUT; inputs resident in HBM before the timed region.  Beside it:
   * the blind-rotation kernel's own duration (HIP events on the engine stream) and its
     roofline: algorithmic modmul-equivalents per launch / duration against the measured
     full-occupancy rate of the same Goldilocks modmul primitive;
+  * config 1 (batched NTT core), config 3 (uneven graph-captured batches, PBS level 2)
+    and config 4 (serving: Poisson arrivals of radix IOps through the continuous-batching
+    server, PBS/s with p50/p99 latency per IOp);
   * cpu_baseline: the C oracle (oracle/luthe_ref.c, OpenMP over ciphertexts) timed on a
     bounded sample of the same batch; its outputs are compared bit for bit with the GPU's.
 
@@ -145,6 +148,82 @@ def run_config3(la, hip, seed=3):
             "decrypt_success": round(ok, 4)}
 
 
+def _pct(v, q):
+    return float(np.percentile(np.asarray(v), q, method="nearest")) if len(v) else None
+
+
+def run_config4(la, ctx, keys, rates=(8, 20, 40, 60), seconds=5.0, burst=96, seed=4):
+    """Config 4: serving.  Poisson arrivals (seeded) of radix IOps drawn uniformly from
+    {ADD, CMP_GT, MUL} x {8, 16, 32} blocks of 2 bits, through the continuous-batching
+    server (HBM arena admission, one PBS cohort per step); latency = submit -> result on
+    the host.  Plus a burst of `burst` requests submitted at once (saturation PBS/s)."""
+    prm = ctx.params
+    mix = [(op, nb) for op in (la.IOP_ADD, la.IOP_CMP_GT, la.IOP_MUL) for nb in (8, 16, 32)]
+    npbs = {k: la.iop_program(*k)[0].npbs for k in mix}
+    rng = np.random.default_rng(seed)
+    pool_a = keys.encrypt_msg(rng.integers(0, 4, 32), seed=41)
+    pool_b = keys.encrypt_msg(rng.integers(0, 4, 32), seed=42)
+    name = {la.IOP_ADD: "ADD", la.IOP_CMP_GT: "CMP_GT", la.IOP_MUL: "MUL"}
+    mean_pbs = float(np.mean([npbs[k] for k in mix]))
+    res = {"workload": "Poisson arrivals of radix IOps, uniform over {ADD, CMP_GT, MUL} x {8, 16, 32} blocks x 2 "
+                       "bits, config-0 parameters, keys resident, continuous batching",
+           "pbs_per_iop": {f"{name[o]}{n}": npbs[(o, n)] for o, n in mix}, "mean_pbs_per_request": mean_pbs}
+    srv = la.Server(ctx)
+    res["cohort_max"] = srv.stats()["cohort_max"]
+    for k in mix:  # warm every program once
+        srv.wait(srv.submit(k[0], pool_a[:k[1]], pool_b[:k[1]]))
+
+    def summarize(infos, wall, st):
+        lat = [i["latency_us"] / 1e3 for _, _, i in infos]
+        by = {}
+        for op, nb, i in infos:
+            by.setdefault(f"{name[op]}{nb}", []).append(i["latency_us"] / 1e3)
+        total_pbs = sum(npbs[(op, nb)] for op, nb, _ in infos)
+        return {"requests": len(infos), "wall_s": round(wall, 3), "pbs": total_pbs,
+                "pbs_per_s": round(total_pbs / wall, 1), "iops_per_s": round(len(infos) / wall, 2),
+                "p50_ms": round(_pct(lat, 50), 2), "p99_ms": round(_pct(lat, 99), 2),
+                "per_iop_ms": {k: {"n": len(v), "p50": round(_pct(v, 50), 2), "p99": round(_pct(v, 99), 2)}
+                               for k, v in sorted(by.items())},
+                "cohorts": st["cohorts"], "mean_cohort": round(st["pbs"] / max(1, st["cohorts"]), 1),
+                "cohort_fill_hist": st["fill_hist"], "arena_rows_peak": st["arena_rows_peak"]}
+
+    # burst: all at once
+    kinds = rng.integers(0, len(mix), burst)
+    srv.stats(reset=True)
+    t0 = time.perf_counter()
+    ids = [(mix[k][0], mix[k][1], srv.submit(mix[k][0], pool_a[:mix[k][1]], pool_b[:mix[k][1]])) for k in kinds]
+    infos = [(op, nb, srv.wait(rid)[1]) for op, nb, rid in ids]
+    res["burst"] = summarize(infos, time.perf_counter() - t0, srv.stats())
+    log(f"bench: config 4 burst of {burst}: {res['burst']['pbs_per_s']} PBS/s, "
+        f"p50 {res['burst']['p50_ms']} ms, p99 {res['burst']['p99_ms']} ms")
+    res["rates"] = []
+    for rate in rates:
+        r2 = np.random.default_rng(seed * 1000 + rate)
+        n = max(8, int(rate * seconds))
+        gaps = r2.exponential(1.0 / rate, n)
+        kinds = r2.integers(0, len(mix), n)
+        srv.stats(reset=True)
+        ids = []
+        t0 = time.perf_counter()
+        t = t0
+        for i in range(n):
+            t += gaps[i]
+            dt = t - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            op, nb = mix[kinds[i]]
+            ids.append((op, nb, srv.submit(op, pool_a[:nb], pool_b[:nb])))
+        infos = [(op, nb, srv.wait(rid)[1]) for op, nb, rid in ids]
+        row = summarize(infos, time.perf_counter() - t0, srv.stats())
+        row["offered_iops_per_s"] = rate
+        row["offered_load"] = round(rate * mean_pbs / max(1.0, res["burst"]["pbs_per_s"]), 3)
+        res["rates"].append(row)
+        log(f"bench: config 4 rate {rate}/s: {row['pbs_per_s']} PBS/s, p50 {row['p50_ms']} ms, "
+            f"p99 {row['p99_ms']} ms, mean cohort {row['mean_cohort']}")
+    srv.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,7 +234,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--br-grid", type=int, default=0, help="blind-rotation workgroups (0 = occupancy x CUs)")
-    ap.add_argument("--no-extra", action="store_true", help="skip the config-1 and config-3 measurements")
+    ap.add_argument("--no-extra", action="store_true", help="skip the config-1, -3 and -4 measurements")
+    ap.add_argument("--serve-seconds", type=float, default=5.0, help="arrival window per rate (config 4)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="only run a device-to-device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -271,6 +351,7 @@ def main():
         log(f"bench: config 1 {out['config1_ntt']}")
         out["config3_uneven"] = run_config3(la, hip)
         log(f"bench: config 3 {out['config3_uneven']}")
+        out["config4_serving"] = run_config4(la, ctx, keys, seconds=args.serve_seconds)
 
     if not args.no_cpu_baseline:
         from oracle import ref  # CPU baseline leg only

@@ -41,7 +41,8 @@ enum {
   LUTHE_ERR_RANGE = -6,       /* register, slot or LUT id out of range */
   LUTHE_ERR_STATE = -7,       /* keys not uploaded, unknown ticket, ... */
   LUTHE_ERR_TIMEOUT = -8,     /* luthe_wait timed out */
-  LUTHE_ERR_NODEV = -9        /* no usable gfx950 device */
+  LUTHE_ERR_NODEV = -9,       /* no usable gfx950 device */
+  LUTHE_ERR_BUSY = -10        /* admission queue full, or the context is owned by a server */
 };
 
 /* TFHE parameter set.  Replaces the compile-time parameter packages
@@ -260,6 +261,102 @@ int luthe_lut_encode(const luthe_params* p, const uint8_t* table, uint32_t table
  * 2^(msg+carry-log_lut_nb) values; function j lands on output lut j. */
 int luthe_lut_encode_many(const luthe_params* p, const uint8_t* tables, uint32_t table_len, uint32_t log_lut_nb,
                           uint64_t* glwe);
+
+/* ---------------------------------------------------------------- radix IOp programs
+ * Builder-written IOp -> DOp-level programs on 2-bit blocks (2 message + 2 carry bits),
+ * the role of the tfhe-rs "fw" translation programs that feed the reference firmware
+ * (fw/ublaze/src/ucore_fw.c:54-128; not part of the checkout).  Operands are radix
+ * integers of nblocks blocks, least significant block first.  Host-only calls. */
+enum {
+  LUTHE_IOP_ADD = 0,        /* (a + b) mod 4^n, carry-lookahead (Sklansky prefix over carry states) */
+  LUTHE_IOP_CMP_GT = 1,     /* one block: a > b (comparison tree, docs/debug.md:284-350,430-490) */
+  LUTHE_IOP_MUL = 2,        /* (a * b) mod 4^n: partial products, column reduction, carry-lookahead add */
+  LUTHE_IOP_ADD_RIPPLE = 3, /* (a + b) mod 4^n, ripple carry with the Many-LUT msg/carry split */
+  LUTHE_IOP_COUNT = 4
+};
+#define LUTHE_IOP_LUTS 11 /* logical LUTs the programs use (luthe_iop_lut) */
+
+typedef struct luthe_iop_info {
+  uint32_t nblocks, rows, nodes, npbs, nalu, depth, nout, a_row0, b_row0;
+} luthe_iop_info;
+
+/* One node: is_pbs ? dst..dst+2^log_lut_nb-1 = PBS(src0, lut) : dst = ALU(opcode, src0, src1, mul_factor).
+ * Rows are single-assignment indices into the request workspace. */
+typedef struct luthe_iop_node {
+  uint8_t is_pbs, opcode, lut, log_lut_nb;
+  uint32_t dst, src0, src1, mul_factor;
+} luthe_iop_node;
+
+/* Shape of a program, and optionally its nodes (topological order) and output rows. */
+int luthe_iop_program(uint32_t op, uint32_t nblocks, luthe_iop_info* info, luthe_iop_node* nodes, size_t nodes_cap,
+                      uint32_t* out_rows, size_t out_cap);
+/* Table of logical LUT `lut` on 4-bit inputs (Many-LUT: 2^log_lut_nb tables of 16 >> log_lut_nb entries). */
+int luthe_iop_lut(uint32_t lut, uint8_t* table16, uint32_t* log_lut_nb);
+/* Plaintext evaluation of a program on block values (PBS = table lookup): the host check of its logic. */
+int luthe_iop_eval_plain(uint32_t op, uint32_t nblocks, const uint32_t* a, const uint32_t* b, uint32_t* out);
+
+/* ---------------------------------------------------------------- serving engine
+ * Continuous batching of radix IOps.  Replaces the backend-to-HPU IOp queue and ack
+ * (ucore_fw.c:54-232), the instruction scheduler's out-of-order issue
+ * (hw/module/instruction_scheduler/rtl/instruction_scheduler.sv) and the BPIP batch
+ * former with fill threshold and timeout (pep_sequencer.sv:584-620, bpip registers at
+ * hpu_regif_core_cfg_1in3.toml:240-260).  A server thread owns the context stream:
+ * while a server is attached, use only luthe_server_* calls on that context (DOp
+ * submissions return LUTHE_ERR_BUSY).  Each request runs in an HBM workspace admitted
+ * from an arena; each step launches the ready PBS of all admitted requests as one cohort.
+ * Results do not depend on which requests share a cohort. */
+typedef struct luthe_server luthe_server;
+
+typedef struct luthe_server_config {
+  uint64_t arena_bytes;       /* HBM for request workspaces (admission budget); default 4 GiB */
+  uint64_t pinned_bytes;      /* pinned host staging for operands/results; default 256 MiB */
+  uint32_t cohort_max;        /* PBS per cohort; 0 = blind-rotation grid width (one wave) */
+  uint32_t cohort_min;        /* launch a cohort once this many PBS are ready ... (default 1) */
+  uint32_t cohort_timeout_us; /* ... or when the oldest ready PBS has waited this long */
+  uint32_t pipeline_depth;    /* steps in flight on the stream, 1..4 (default 2) */
+  uint32_t lut_base;          /* first of LUTHE_IOP_LUTS LUT slots written at creation;
+                                 0xFFFFFFFF = the top LUTHE_IOP_LUTS slots */
+  uint32_t max_inflight;      /* admitted requests at once; 0 = limited by the arena only */
+  uint32_t queue_max;         /* submitted-but-not-admitted requests; beyond: LUTHE_ERR_BUSY */
+} luthe_server_config;
+
+/* Counters of the mockup's per-IOp report (docs/debug.md:149-170: PBS issued, batches,
+ * by_timeout, usage) plus admission and latency figures since creation or the last reset. */
+typedef struct luthe_server_stats {
+  uint64_t submitted, admitted, completed, rejected;
+  uint64_t steps, cohorts, cohorts_by_timeout, pbs, alu, alu_launches;
+  uint64_t fill_hist[8];      /* cohorts by fill: bucket floor(8 * size / cohort_max), full = 7 */
+  uint64_t arena_rows, arena_rows_used, arena_rows_peak;
+  uint32_t cohort_max, lut_base, pending, inflight;
+  double latency_p50_us, latency_p99_us, latency_max_us; /* submit -> result on the host */
+} luthe_server_stats;
+
+typedef struct luthe_request_info {
+  uint32_t op, nblocks, npbs, depth, cohorts; /* cohorts = PBS cohorts the request took part in */
+  int32_t status;
+  double queue_us;   /* submit -> admission */
+  double latency_us; /* submit -> result on the host */
+} luthe_request_info;
+
+void luthe_server_config_default(luthe_server_config* cfg);
+/* Requires uploaded keys and msg_bits = carry_bits = 2; writes the program LUTs. */
+int luthe_server_create(luthe_ctx* ctx, const luthe_server_config* cfg, luthe_server** out);
+/* Completes all submitted requests, then stops.  Results not collected are dropped. */
+int luthe_server_destroy(luthe_server* srv);
+const char* luthe_server_last_error(luthe_server* srv);
+/* Queue IOp `op` on a, b (host, nblocks BLWE each; copied before return).  `out` (host,
+ * nblocks BLWE, or 1 for CMP_GT) is written before the request reports done and must stay
+ * valid until then.  LUTHE_ERR_NOMEM: larger than the arena or staging; LUTHE_ERR_BUSY:
+ * queue full. */
+int luthe_server_submit(luthe_server* srv, uint32_t op, uint32_t nblocks, const uint64_t* a, const uint64_t* b,
+                        uint64_t* out, uint64_t* id);
+/* 1 = done (request released, info filled), 0 = pending, < 0 = error. */
+int luthe_server_poll(luthe_server* srv, uint64_t id, luthe_request_info* info);
+/* Wait for a request (timeout_us < 0: no limit); releases it. */
+int luthe_server_wait(luthe_server* srv, uint64_t id, int64_t timeout_us, luthe_request_info* info);
+/* Launch the ready PBS at the next step even below cohort_min (the PBS_F flush, docs/dop.md:108-111). */
+int luthe_server_flush(luthe_server* srv);
+int luthe_server_get_stats(luthe_server* srv, luthe_server_stats* out, int reset);
 
 #ifdef __cplusplus
 }

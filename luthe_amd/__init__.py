@@ -18,7 +18,10 @@ _HIP = None
 
 P = 0xFFFFFFFF00000001
 ERRORS = {0: "OK", -1: "ARG", -2: "PARAMS", -3: "HIP", -4: "NOMEM", -5: "DOP", -6: "RANGE",
-          -7: "STATE", -8: "TIMEOUT", -9: "NODEV"}
+          -7: "STATE", -8: "TIMEOUT", -9: "NODEV", -10: "BUSY"}
+IOP_ADD, IOP_CMP_GT, IOP_MUL, IOP_ADD_RIPPLE = 0, 1, 2, 3
+IOP_NAMES = {IOP_ADD: "ADD", IOP_CMP_GT: "CMP_GT", IOP_MUL: "MUL", IOP_ADD_RIPPLE: "ADD_RIPPLE"}
+IOP_LUTS = 11
 BSK_COEF, BSK_NTT = 0, 1
 
 
@@ -54,6 +57,41 @@ class Params(C.Structure):
 class Counters(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("pbs", "pbs_batches", "pbs_max_batch", "alu", "mem", "dops",
                                           "iops", "tickets", "levels")]
+
+
+class IopInfo(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("nblocks", "rows", "nodes", "npbs", "nalu", "depth", "nout",
+                                          "a_row0", "b_row0")]
+
+
+class IopNode(C.Structure):
+    _fields_ = [("is_pbs", C.c_uint8), ("opcode", C.c_uint8), ("lut", C.c_uint8), ("log_lut_nb", C.c_uint8),
+                ("dst", C.c_uint32), ("src0", C.c_uint32), ("src1", C.c_uint32), ("mul_factor", C.c_uint32)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("arena_bytes", C.c_uint64), ("pinned_bytes", C.c_uint64), ("cohort_max", C.c_uint32),
+                ("cohort_min", C.c_uint32), ("cohort_timeout_us", C.c_uint32), ("pipeline_depth", C.c_uint32),
+                ("lut_base", C.c_uint32), ("max_inflight", C.c_uint32), ("queue_max", C.c_uint32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = ([(n, C.c_uint64) for n in ("submitted", "admitted", "completed", "rejected", "steps", "cohorts",
+                                           "cohorts_by_timeout", "pbs", "alu", "alu_launches")]
+                + [("fill_hist", C.c_uint64 * 8)]
+                + [(n, C.c_uint64) for n in ("arena_rows", "arena_rows_used", "arena_rows_peak")]
+                + [(n, C.c_uint32) for n in ("cohort_max", "lut_base", "pending", "inflight")]
+                + [(n, C.c_double) for n in ("latency_p50_us", "latency_p99_us", "latency_max_us")])
+
+    def as_dict(self):
+        d = {n: getattr(self, n) for n, _ in self._fields_ if n != "fill_hist"}
+        d["fill_hist"] = list(self.fill_hist)
+        return d
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("op", "nblocks", "npbs", "depth", "cohorts")] + [
+        ("status", C.c_int32), ("queue_us", C.c_double), ("latency_us", C.c_double)]
 
 
 def lib():
@@ -114,6 +152,18 @@ def lib():
         "luthe_decode": (u32, [P_, u64]),
         "luthe_lut_encode": (i32, [P_, vp, u32, vp]),
         "luthe_lut_encode_many": (i32, [P_, vp, u32, u32, vp]),
+        "luthe_iop_program": (i32, [u32, u32, C.POINTER(IopInfo), vp, sz, vp, sz]),
+        "luthe_iop_lut": (i32, [u32, vp, C.POINTER(u32)]),
+        "luthe_iop_eval_plain": (i32, [u32, u32, vp, vp, vp]),
+        "luthe_server_config_default": (None, [C.POINTER(ServerConfig)]),
+        "luthe_server_create": (i32, [vp, C.POINTER(ServerConfig), C.POINTER(vp)]),
+        "luthe_server_destroy": (i32, [vp]),
+        "luthe_server_last_error": (C.c_char_p, [vp]),
+        "luthe_server_submit": (i32, [vp, u32, u32, vp, vp, vp, C.POINTER(u64)]),
+        "luthe_server_poll": (i32, [vp, u64, C.POINTER(RequestInfo)]),
+        "luthe_server_wait": (i32, [vp, u64, C.c_int64, C.POINTER(RequestInfo)]),
+        "luthe_server_flush": (i32, [vp]),
+        "luthe_server_get_stats": (i32, [vp, C.POINTER(ServerStats), i32]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -254,7 +304,9 @@ class Context:
 
     def close(self):
         if self._h:
-            lib().luthe_destroy(self._h)
+            rc = lib().luthe_destroy(self._h)
+            if rc == -7:  # a server is still attached: keep the context alive
+                self._chk(rc, "luthe_destroy")
             self._h = C.c_void_p()
 
     def __del__(self):
@@ -521,3 +573,122 @@ def iop_translate(iop_words, template, ct_slots=4096):
 
 def delta(params):
     return lib().luthe_delta(C.byref(params))
+
+
+# ---------------------------------------------------------------- radix IOp programs
+def iop_program(op, nblocks):
+    """(info, nodes, out_rows) of a builder-written radix IOp program (host only)."""
+    info = IopInfo()
+    rc = lib().luthe_iop_program(op, nblocks, C.byref(info), None, 0, None, 0)
+    if rc != 0:
+        raise LutheError(rc, "iop_program")
+    nodes = (IopNode * info.nodes)()
+    outs = np.zeros(info.nout, np.uint32)
+    rc = lib().luthe_iop_program(op, nblocks, C.byref(info), C.cast(nodes, C.c_void_p), info.nodes, _vp(outs),
+                                 outs.size)
+    if rc != 0:
+        raise LutheError(rc, "iop_program")
+    return info, nodes, outs
+
+
+def iop_lut(lut):
+    """(table of 16 entries, log_lut_nb) of a program LUT."""
+    t = np.zeros(16, np.uint8)
+    lognb = C.c_uint32()
+    rc = lib().luthe_iop_lut(lut, _vp(t), C.byref(lognb))
+    if rc != 0:
+        raise LutheError(rc, "iop_lut")
+    return t, lognb.value
+
+
+def iop_eval_plain(op, a_blocks, b_blocks):
+    a = np.ascontiguousarray(a_blocks, np.uint32)
+    b = np.ascontiguousarray(b_blocks, np.uint32)
+    out = np.zeros(1 if op == IOP_CMP_GT else a.size, np.uint32)
+    rc = lib().luthe_iop_eval_plain(op, a.size, _vp(a), _vp(b), _vp(out))
+    if rc != 0:
+        raise LutheError(rc, "iop_eval_plain")
+    return out
+
+
+def to_blocks(x, nblocks):
+    """Radix decomposition of an integer into 2-bit blocks, least significant first."""
+    return np.array([(int(x) >> (2 * i)) & 3 for i in range(nblocks)], np.uint32)
+
+
+def from_blocks(blocks):
+    return sum(int(v) << (2 * i) for i, v in enumerate(blocks))
+
+
+# ---------------------------------------------------------------- serving engine
+class Server:
+    """Continuous-batching server of radix IOps attached to a Context (see luthe.h)."""
+
+    def __init__(self, ctx, **cfg):
+        self.ctx = ctx
+        self.cfg = ServerConfig()
+        lib().luthe_server_config_default(C.byref(self.cfg))
+        for k, v in cfg.items():
+            setattr(self.cfg, k, v)
+        self._h = C.c_void_p()
+        rc = lib().luthe_server_create(ctx.handle, C.byref(self.cfg), C.byref(self._h))
+        if rc != 0:
+            ctx._chk(rc, "luthe_server_create")
+        self._outs = {}
+        self.blen = ctx.params.blen
+
+    def _chk(self, rc, what):
+        if rc < 0:
+            msg = lib().luthe_server_last_error(self._h)
+            raise LutheError(rc, f"{what}: {msg.decode() if msg else ''}")
+        return rc
+
+    def submit(self, op, a, b):
+        """Queue IOp `op` on BLWE arrays a, b of shape (nblocks, blen).  Returns the id."""
+        a = np.ascontiguousarray(a, np.uint64)
+        b = np.ascontiguousarray(b, np.uint64)
+        assert a.shape == b.shape and a.ndim == 2 and a.shape[1] == self.blen
+        nb = a.shape[0]
+        out = np.zeros((1 if op == IOP_CMP_GT else nb, self.blen), np.uint64)
+        rid = C.c_uint64()
+        self._chk(lib().luthe_server_submit(self._h, op, nb, _vp(a), _vp(b), _vp(out), C.byref(rid)), "submit")
+        self._outs[rid.value] = out
+        return rid.value
+
+    def poll(self, rid):
+        """None while pending, else (out, info dict)."""
+        info = RequestInfo()
+        rc = self._chk(lib().luthe_server_poll(self._h, rid, C.byref(info)), "poll")
+        if rc == 0:
+            return None
+        return self._outs.pop(rid), {n: getattr(info, n) for n, _ in info._fields_}
+
+    def wait(self, rid, timeout_us=-1):
+        info = RequestInfo()
+        self._chk(lib().luthe_server_wait(self._h, rid, timeout_us, C.byref(info)), "wait")
+        return self._outs.pop(rid), {n: getattr(info, n) for n, _ in info._fields_}
+
+    def flush(self):
+        self._chk(lib().luthe_server_flush(self._h), "flush")
+
+    def stats(self, reset=False):
+        st = ServerStats()
+        self._chk(lib().luthe_server_get_stats(self._h, C.byref(st), 1 if reset else 0), "stats")
+        return st.as_dict()
+
+    def close(self):
+        if self._h:
+            lib().luthe_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,73 @@
+// Context internals shared by the C ABI translation units (luthe_api.cpp, serve.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/luthe.h"
+#include "kernels.h"
+
+struct Ticket {
+  hipEvent_t done = nullptr;
+};
+
+struct luthe_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+struct luthe_ctx {
+  luthe_params prm{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  size_t blen = 0;           // k*N + 1
+  size_t lut_len = 0;        // (k+1)*N
+  // device memories
+  uint64_t* d_mem = nullptr;  // [regfile_size + ct_slots] BLWE rows
+  uint64_t* d_luts = nullptr;
+  uint64_t* d_bsk = nullptr;
+  uint32_t* d_ksk = nullptr;
+  uint64_t* d_tab = nullptr;  // NTT twiddle tables (ntt2048.h layout)
+  bool bsk_ok = false, ksk_ok = false;
+  // scratch (grow-only)
+  uint16_t* d_lwe2n = nullptr;
+  size_t lwe2n_cap = 0;
+  int lwe_stride = 0;
+  lk::PbsJob* d_jobs = nullptr;
+  size_t jobs_cap = 0;
+  uint8_t* d_stage = nullptr;  // per-submission descriptors
+  size_t stage_cap = 0;
+  uint8_t* h_stage = nullptr;  // pinned mirror
+  size_t h_stage_cap = 0;
+  hipEvent_t stage_copied = nullptr;
+  uint64_t dev_bytes = 0;
+  luthe_counters cnt{};
+  uint64_t next_ticket = 1;
+  std::map<uint64_t, Ticket> tickets;
+  std::map<uint32_t, std::vector<uint32_t>> iop_table;  // key = max_align << 8 | opcode
+  int br_grid_override = 0;
+  luthe_server* server = nullptr;  // attached serving engine (owns the stream while set)
+};
+
+namespace lx {
+
+// Records the message on the context and returns `code`.
+int fail(luthe_ctx* c, int code, const char* fmt, ...);
+int dev_alloc(luthe_ctx* c, void** p, size_t bytes);
+int ensure_lwe2n(luthe_ctx* c, size_t n);
+int ensure_jobs(luthe_ctx* c, size_t n);
+// Key switch + blind rotation + sample extract of n jobs whose descriptors are on the
+// device; rows are relative to in_base / out_base.
+int run_pbs(luthe_ctx* c, const uint64_t* in_base, const lk::PbsJob* d_jobs, int n, uint64_t* out_base,
+            hipStream_t s);
+
+}  // namespace lx
+
+#define HIPCHK(c, expr)                                                                    \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess) return lx::fail((c), LUTHE_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)

@@ -17,6 +17,8 @@ using namespace ntt;
 
 namespace lk {
 
+constexpr int MAX_GRID_Y = 65535;  // launches split their per-op grid dimension at this
+
 // ------------------------------------------------------------------------------------
 // NTT kernels: NP polynomials per 256-thread workgroup.
 // ------------------------------------------------------------------------------------
@@ -443,16 +445,23 @@ hipError_t launch_bsk_prepare(const uint64_t* src, uint64_t* dst, int n_lwe, int
 
 hipError_t launch_keyswitch(const uint64_t* base, const PbsJob* jobs, int nct, const uint32_t* ksk, uint16_t* lwe2n,
                             const KsArgs& a, int ks_level, hipStream_t s) {
-  if (nct == 0) return hipSuccess;
-  dim3 grid((unsigned)((a.n_lwe + 1 + 255) / 256), (unsigned)((nct + KS_CT - 1) / KS_CT));
-  switch (ks_level) {
+  if (ks_level < 1 || ks_level > 8) return hipErrorInvalidValue;
+  // grid.y is limited to 65535: chunks of at most 65535 * KS_CT ciphertexts
+  for (int c0 = 0; c0 < nct; c0 += MAX_GRID_Y * KS_CT) {
+    const int nc = (nct - c0) < MAX_GRID_Y * KS_CT ? (nct - c0) : MAX_GRID_Y * KS_CT;
+    dim3 grid((unsigned)((a.n_lwe + 1 + 255) / 256), (unsigned)((nc + KS_CT - 1) / KS_CT));
+    const PbsJob* jb = jobs + c0;
+    uint16_t* out = lwe2n + (size_t)c0 * a.lwe_stride;
+    switch (ks_level) {
 #define KS_CASE(L) \
-  case L: k_keyswitch<L><<<grid, 256, 0, s>>>(base, jobs, nct, ksk, lwe2n, a); break;
-    KS_CASE(1) KS_CASE(2) KS_CASE(3) KS_CASE(4) KS_CASE(5) KS_CASE(6) KS_CASE(7) KS_CASE(8)
+  case L: k_keyswitch<L><<<grid, 256, 0, s>>>(base, jb, nc, ksk, out, a); break;
+      KS_CASE(1) KS_CASE(2) KS_CASE(3) KS_CASE(4) KS_CASE(5) KS_CASE(6) KS_CASE(7) KS_CASE(8)
 #undef KS_CASE
-    default: return hipErrorInvalidValue;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
   }
-  return hipGetLastError();
+  return hipSuccess;
 }
 
 template <int K1, int L>
@@ -491,23 +500,36 @@ int blind_rotate_occupancy(int pbs_level) {
 }
 
 hipError_t launch_alu(uint64_t* base, const AluOp* ops, int nops, int blen, hipStream_t s) {
-  if (nops == 0) return hipSuccess;
-  k_alu<<<dim3((unsigned)((blen + 255) / 256), (unsigned)nops), 256, 0, s>>>(base, ops, blen);
-  return hipGetLastError();
+  for (int o = 0; o < nops; o += MAX_GRID_Y) {
+    const int n = (nops - o) < MAX_GRID_Y ? (nops - o) : MAX_GRID_Y;
+    k_alu<<<dim3((unsigned)((blen + 255) / 256), (unsigned)n), 256, 0, s>>>(base, ops + o, blen);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 hipError_t launch_alu_batch(uint32_t opcode, const uint64_t* a, const uint64_t* b, uint64_t* out, uint32_t mul_factor,
                             uint32_t msg_cst, int n, int blen, hipStream_t s) {
-  if (n == 0) return hipSuccess;
-  k_alu_batch<<<dim3((unsigned)((blen + 255) / 256), (unsigned)n), 256, 0, s>>>(opcode, a, b, out, mul_factor, msg_cst,
-                                                                                  blen);
-  return hipGetLastError();
+  for (int o = 0; o < n; o += MAX_GRID_Y) {
+    const int m = (n - o) < MAX_GRID_Y ? (n - o) : MAX_GRID_Y;
+    const size_t off = (size_t)o * blen;
+    k_alu_batch<<<dim3((unsigned)((blen + 255) / 256), (unsigned)m), 256, 0, s>>>(
+        opcode, a + off, b ? b + off : nullptr, out + off, mul_factor, msg_cst, blen);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 hipError_t launch_copy(uint64_t* base, const CopyOp* ops, int nops, int blen, hipStream_t s) {
-  if (nops == 0) return hipSuccess;
-  k_copy<<<dim3((unsigned)((blen + 255) / 256), (unsigned)nops), 256, 0, s>>>(base, ops, blen);
-  return hipGetLastError();
+  for (int o = 0; o < nops; o += MAX_GRID_Y) {
+    const int n = (nops - o) < MAX_GRID_Y ? (nops - o) : MAX_GRID_Y;
+    k_copy<<<dim3((unsigned)((blen + 255) / 256), (unsigned)n), 256, 0, s>>>(base, ops + o, blen);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 hipError_t launch_modmul_peak(uint64_t* out, int blocks, int iters, hipStream_t s) {

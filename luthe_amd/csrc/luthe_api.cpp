@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "../../include/luthe.h"
+#include "ctx.h"
 #include "dop.h"
 #include "kernels.h"
 #include "ntt2048.h"
@@ -34,49 +35,7 @@ using lk::AluOp;
 using lk::CopyOp;
 using lk::PbsJob;
 
-struct Ticket {
-  hipEvent_t done = nullptr;
-};
-
-struct luthe_graph {
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-};
-
-struct luthe_ctx {
-  luthe_params prm{};
-  int device = 0;
-  hipStream_t stream = nullptr;
-  std::string err;
-  size_t blen = 0;           // k*N + 1
-  size_t lut_len = 0;        // (k+1)*N
-  // device memories
-  uint64_t* d_mem = nullptr;  // [regfile_size + ct_slots] BLWE rows
-  uint64_t* d_luts = nullptr;
-  uint64_t* d_bsk = nullptr;
-  uint32_t* d_ksk = nullptr;
-  uint64_t* d_tab = nullptr;  // NTT twiddle tables (ntt2048.h layout)
-  bool bsk_ok = false, ksk_ok = false;
-  // scratch (grow-only)
-  uint16_t* d_lwe2n = nullptr;
-  size_t lwe2n_cap = 0;
-  int lwe_stride = 0;
-  PbsJob* d_jobs = nullptr;
-  size_t jobs_cap = 0;
-  uint8_t* d_stage = nullptr;  // per-submission descriptors
-  size_t stage_cap = 0;
-  uint8_t* h_stage = nullptr;  // pinned mirror
-  size_t h_stage_cap = 0;
-  hipEvent_t stage_copied = nullptr;
-  uint64_t dev_bytes = 0;
-  luthe_counters cnt{};
-  uint64_t next_ticket = 1;
-  std::map<uint64_t, Ticket> tickets;
-  std::map<uint32_t, std::vector<uint32_t>> iop_table;  // key = max_align << 8 | opcode
-  int br_grid_override = 0;
-};
-
-namespace {
+namespace lx {
 
 int fail(luthe_ctx* c, int code, const char* fmt, ...) {
   if (c) {
@@ -89,12 +48,6 @@ int fail(luthe_ctx* c, int code, const char* fmt, ...) {
   }
   return code;
 }
-
-#define HIPCHK(c, expr)                                                                    \
-  do {                                                                                     \
-    hipError_t e_ = (expr);                                                                \
-    if (e_ != hipSuccess) return fail((c), LUTHE_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
-  } while (0)
 
 int dev_alloc(luthe_ctx* c, void** p, size_t bytes) {
   hipError_t e = hipMalloc(p, bytes);
@@ -197,7 +150,9 @@ int run_pbs(luthe_ctx* c, const uint64_t* in_base, const PbsJob* d_jobs, int n, 
   return LUTHE_OK;
 }
 
-}  // namespace
+}  // namespace lx
+
+using namespace lx;
 
 // ===================================================================================
 extern "C" {
@@ -261,6 +216,7 @@ int luthe_create(const luthe_params* p, int device, luthe_ctx** out) {
 
 int luthe_destroy(luthe_ctx* c) {
   if (!c) return LUTHE_ERR_ARG;
+  if (c->server) return fail(c, LUTHE_ERR_STATE, "destroy the attached server first");
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto& kv : c->tickets)
@@ -487,6 +443,7 @@ int luthe_reserve(luthe_ctx* c, size_t max_batch) {
 
 int luthe_capture_begin(luthe_ctx* c) {
   if (!c) return LUTHE_ERR_ARG;
+  if (c->server) return fail(c, LUTHE_ERR_BUSY, "context stream is owned by an attached server");
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   return LUTHE_OK;
@@ -539,6 +496,7 @@ int luthe_ntt_inverse_batch(luthe_ctx* c, const uint64_t* d_in, uint64_t* d_out,
 
 // ---------------------------------------------------------------- DOp stream
 static int execute_program(luthe_ctx* c, const std::vector<dop::Dop>& prog, uint64_t* ticket) {
+  if (c->server) return fail(c, LUTHE_ERR_BUSY, "context stream is owned by an attached server");
   const luthe_params& p = c->prm;
   const uint32_t R = p.regfile_size;
   std::vector<int> level;
@@ -622,6 +580,7 @@ static int execute_program(luthe_ctx* c, const std::vector<dop::Dop>& prog, uint
 
 int luthe_submit_dops(luthe_ctx* c, const uint32_t* words, size_t n, uint64_t* ticket) {
   if (!c || (!words && n)) return LUTHE_ERR_ARG;
+  if (c->server) return fail(c, LUTHE_ERR_BUSY, "context stream is owned by an attached server");
   std::vector<dop::Dop> prog;
   std::string why;
   int rc = dop::decode_stream(words, n, c->prm.regfile_size, c->prm.ct_slots, c->prm.lut_slots, prog, why);
@@ -705,6 +664,7 @@ int luthe_register_iop(luthe_ctx* c, uint8_t opcode, uint8_t max_align, const ui
 
 int luthe_submit_iop(luthe_ctx* c, const uint32_t* words, size_t n, uint64_t* ticket) {
   if (!c || !words || n == 0) return LUTHE_ERR_ARG;
+  if (c->server) return fail(c, LUTHE_ERR_BUSY, "context stream is owned by an attached server");
   dop::IopBundle iop;
   std::string why;
   int rc = dop::parse_iop(words, n, iop, why);
