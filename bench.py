@@ -322,8 +322,14 @@ def main():
         tm.start()
         ctx.pbs_batch_dev(d_in, d_out, d_gid, 1)
         plain_ms.append(tm.stop_ms())
-    plain = 1000.0 / float(np.median(plain_ms))
-    log(f"bench: plain path {np.median(plain_ms):.2f} ms per PBS -> {plain:.1f} PBS/s")
+    plain = 1000.0 / float(np.median(plain_ms)) if plain_ms else None
+    if plain:
+        log(f"bench: plain path {np.median(plain_ms):.2f} ms per PBS -> {plain:.1f} PBS/s")
+    traffic = None
+    tpath = os.path.join(HERE, "profiles", "k_blind_rotate_8192_hbm_traffic.json")
+    if os.path.exists(tpath) and args.batch == 8192:
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
 
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "PBS/s", "n_gpus": 1, "steps": args.steps,
@@ -335,15 +341,16 @@ def main():
                    "batch": args.batch, "lwe_dim": prm.lwe_dim, "poly_size": prm.poly_size,
                    "pbs_base_log": prm.pbs_base_log, "pbs_level": prm.pbs_level,
                    "ks_base_log": prm.ks_base_log, "ks_level": prm.ks_level, "br_workgroups": ctx.br_grid()},
-        "plain_pbs_per_s": round(plain, 2),
-        "batched_over_plain": round(value / plain, 1),
+        "plain_pbs_per_s": round(plain, 2) if plain else None,
+        "batched_over_plain": round(value / plain, 1) if plain else None,
         "decrypt_success": decrypt_ok,
         "kernels_ms": {"blind_rotate": round(br_ms, 3), "keyswitch": round(ks_ms, 3)},
         "roofline": {"bound": "valu", "achieved": round(achieved, 2), "peak": round(peak, 2),
-                     "unit": "Gmodmul/s", "frac": round(achieved / peak, 4), "traffic": None,
+                     "unit": "Gmodmul/s", "frac": round(achieved / peak, 4), "traffic": traffic,
                      "algorithmic_work": "modmul-equivalents per launch = batch x n x ((k+1)(L+1) x 11264 "
                                          "+ (k+1)^2 L x N + (k+1) N)",
-                     "bsk_stream_gbps": round(bsk_bytes(prm) / (br_ms * 1e-3) / 1e9, 2)},
+                     "bsk_consumed_gbps": round(args.batch * bsk_bytes(prm) / (br_ms * 1e-3) / 1e9, 1),
+                     "traffic_gbps": round(traffic / (br_ms * 1e-3) / 1e9, 2) if traffic else None},
     }
 
     if not args.no_extra:
