@@ -262,6 +262,76 @@ int luthe_lut_encode(const luthe_params* p, const uint8_t* table, uint32_t table
 int luthe_lut_encode_many(const luthe_params* p, const uint8_t* tables, uint32_t table_len, uint32_t log_lut_nb,
                           uint64_t* glwe);
 
+/* ---------------------------------------------------------------- HPU memory images (host only)
+ * Conversions between this engine's layouts and the HPU's HBM images, so keys, LUTs and
+ * ciphertexts laid out for the FPGA load directly (and results can be written back).
+ *  - BSK: per CMUX i, output column j, stage iteration s < N/chunk, input row g, gadget
+ *    level l (least significant first), `chunk` = PSI*R consecutive NTT-domain
+ *    coefficients, the polynomial in the HW NTT order bsk_order(ntt_cut)
+ *    (hw/module/hpu/simu/scripts/gen_stimuli.py:108-136, sw/ntt/ntt_gf64_bsk_order.py:9-35;
+ *    the header of bsk_manager.sv notes the reversed (R,N) order).
+ *  - LUT: body polynomials only (the mask of a trivial GLWE is zero), N u64 per gid
+ *    (pep_load_glwe.sv:9,83,204).
+ *  - Ciphertexts: pem_pc pseudo-channel regions of count * slot_bytes bytes; slot s at
+ *    s * slot_bytes (page aligned, pem_common_param_pkg.sv:24-36); register-file word w
+ *    sends coefficients [w*RC + p*RC/PC, w*RC + (p+1)*RC/PC) to channel p; the body is
+ *    the first u64 of one extra AXI word of channel 0 (pem_load.sv:90-156).
+ * The KSK image (ksk_if block-column format) is not converted. */
+typedef struct luthe_hpu_layout {
+  uint32_t axi_bytes;    /* AXI4 data width of the ciphertext channels in bytes (8..64) */
+  uint32_t pem_pc;       /* PEM_PC: ciphertext pseudo-channels, 1 or 2 */
+  uint32_t regf_coef_nb; /* REGF_COEF_NB: coefficients per register-file word */
+  uint32_t bsk_chunk;    /* PSI * R of the HW NTT: coefficients per BSK image chunk */
+  uint32_t ntt_ncut;     /* radix stages of the HW NTT (1..8) */
+  uint32_t ntt_cut[8];   /* log2 radix per stage, summing to log2 N */
+} luthe_hpu_layout;
+
+/* order[pos] = natural NTT-domain index (evaluation at zeta^(2m+1)) held at position pos
+ * by an NTT split as `cut`; [5,6] is this engine's NTT-domain order. */
+int luthe_hpu_bsk_order(const uint32_t* cut, uint32_t ncut, uint32_t* order);
+size_t luthe_hpu_bsk_image_words(const luthe_params* p);
+/* image <-> LUTHE_BSK_NTT layout ([n][k+1][L][k+1][N], ntt() order of cut [5,6]). */
+int luthe_hpu_bsk_from_image(const luthe_params* p, const luthe_hpu_layout* lay, const uint64_t* image,
+                             uint64_t* bsk_ntt);
+int luthe_hpu_bsk_to_image(const luthe_params* p, const luthe_hpu_layout* lay, const uint64_t* bsk_ntt,
+                           uint64_t* image);
+/* Upload a BSK given as an HPU image (converted on the host, then luthe_upload_bsk). */
+int luthe_upload_bsk_hpu_image(luthe_ctx* ctx, const luthe_hpu_layout* lay, const uint64_t* image);
+/* CT_MEM_BYTES: bytes per ciphertext slot and pseudo-channel (0 if the layout is invalid). */
+size_t luthe_hpu_ct_slot_bytes(const luthe_params* p, const luthe_hpu_layout* lay);
+int luthe_hpu_ct_to_image(const luthe_params* p, const luthe_hpu_layout* lay, const uint64_t* blwe, uint32_t count,
+                          uint8_t* image);
+int luthe_hpu_ct_from_image(const luthe_params* p, const luthe_hpu_layout* lay, const uint8_t* image, uint32_t count,
+                            uint64_t* blwe);
+/* LUT bodies (count x N) -> full GLWEs (count x (k+1)N, zero mask). */
+int luthe_hpu_lut_from_image(const luthe_params* p, const uint64_t* bodies, uint32_t count, uint64_t* glwe);
+int luthe_write_lut_hpu_image(luthe_ctx* ctx, uint32_t gid, const uint64_t* bodies, uint32_t count);
+
+/* ---------------------------------------------------------------- trace
+ * Instruction-scheduler trace records (isc_trace_t, isc_common_param_pkg.sv:35-48):
+ * timestamp | DOp word | query command | pool state (sync_id, rd/wr/issue locks, vld,
+ * rd_pdg, pdg) | pe_reserved, packed most significant field first into
+ * luthe_trace_record_bits() bits and stored as 32-bit words, least significant word
+ * first (trace_manager.sv:245-264).  The DOp path of a context logs REFILL when a DOp is
+ * accepted, ISSUE when its dependency level is launched and RETIRE when its submission
+ * completes; timestamps are microseconds since luthe_trace_enable (the HPU counts clock
+ * cycles), pe_reserved of a PBS RETIRE holds the number of PBS launched with it. */
+enum { LUTHE_TRC_NONE = 0, LUTHE_TRC_RDUNLOCK = 1, LUTHE_TRC_RETIRE = 2, LUTHE_TRC_REFILL = 3, LUTHE_TRC_ISSUE = 4 };
+typedef struct luthe_trace_event {
+  uint32_t timestamp;
+  uint32_t insn;
+  uint8_t cmd, sync_id, rd_lock, wr_lock, issue_lock, vld, rd_pdg, pdg;
+  uint32_t pe_reserved;
+} luthe_trace_event;
+int luthe_trace_record_bits(uint32_t lwe_dim);
+/* Returns the words per record (> 0) or an error. */
+int luthe_trace_pack(uint32_t lwe_dim, const luthe_trace_event* ev, size_t n, uint32_t* words);
+int luthe_trace_unpack(uint32_t lwe_dim, const uint32_t* words, size_t n, luthe_trace_event* ev);
+/* Keep the last `capacity` events of the DOp path (0 = off; the HPU's trace buffer wraps). */
+int luthe_trace_enable(luthe_ctx* ctx, size_t capacity);
+/* Copy out up to cap events, oldest first, and drop them from the log. */
+int luthe_trace_read(luthe_ctx* ctx, luthe_trace_event* out, size_t cap, size_t* n);
+
 /* ---------------------------------------------------------------- radix IOp programs
  * Builder-written IOp -> DOp-level programs on 2-bit blocks (2 message + 2 carry bits),
  * the role of the tfhe-rs "fw" translation programs that feed the reference firmware

@@ -69,6 +69,32 @@ class IopNode(C.Structure):
                 ("dst", C.c_uint32), ("src0", C.c_uint32), ("src1", C.c_uint32), ("mul_factor", C.c_uint32)]
 
 
+class HpuLayout(C.Structure):
+    _fields_ = [("axi_bytes", C.c_uint32), ("pem_pc", C.c_uint32), ("regf_coef_nb", C.c_uint32),
+                ("bsk_chunk", C.c_uint32), ("ntt_ncut", C.c_uint32), ("ntt_cut", C.c_uint32 * 8)]
+
+
+def hpu_layout(axi_bytes=64, pem_pc=2, regf_coef_nb=32, bsk_chunk=64, ntt_cut=(5, 6)):
+    lay = HpuLayout()
+    lay.axi_bytes, lay.pem_pc, lay.regf_coef_nb, lay.bsk_chunk = axi_bytes, pem_pc, regf_coef_nb, bsk_chunk
+    lay.ntt_ncut = len(ntt_cut)
+    for i, c in enumerate(ntt_cut):
+        lay.ntt_cut[i] = c
+    return lay
+
+
+class TraceEvent(C.Structure):
+    _fields_ = [("timestamp", C.c_uint32), ("insn", C.c_uint32)] + [
+        (n, C.c_uint8) for n in ("cmd", "sync_id", "rd_lock", "wr_lock", "issue_lock", "vld", "rd_pdg", "pdg")] + [
+        ("pe_reserved", C.c_uint32)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+TRC_NONE, TRC_RDUNLOCK, TRC_RETIRE, TRC_REFILL, TRC_ISSUE = 0, 1, 2, 3, 4
+
+
 class ServerConfig(C.Structure):
     _fields_ = [("arena_bytes", C.c_uint64), ("pinned_bytes", C.c_uint64), ("cohort_max", C.c_uint32),
                 ("cohort_min", C.c_uint32), ("cohort_timeout_us", C.c_uint32), ("pipeline_depth", C.c_uint32),
@@ -152,6 +178,21 @@ def lib():
         "luthe_decode": (u32, [P_, u64]),
         "luthe_lut_encode": (i32, [P_, vp, u32, vp]),
         "luthe_lut_encode_many": (i32, [P_, vp, u32, u32, vp]),
+        "luthe_hpu_bsk_order": (i32, [vp, u32, vp]),
+        "luthe_hpu_bsk_image_words": (sz, [P_]),
+        "luthe_hpu_bsk_from_image": (i32, [P_, C.POINTER(HpuLayout), vp, vp]),
+        "luthe_hpu_bsk_to_image": (i32, [P_, C.POINTER(HpuLayout), vp, vp]),
+        "luthe_upload_bsk_hpu_image": (i32, [vp, C.POINTER(HpuLayout), vp]),
+        "luthe_hpu_ct_slot_bytes": (sz, [P_, C.POINTER(HpuLayout)]),
+        "luthe_hpu_ct_to_image": (i32, [P_, C.POINTER(HpuLayout), vp, u32, vp]),
+        "luthe_hpu_ct_from_image": (i32, [P_, C.POINTER(HpuLayout), vp, u32, vp]),
+        "luthe_hpu_lut_from_image": (i32, [P_, vp, u32, vp]),
+        "luthe_write_lut_hpu_image": (i32, [vp, u32, vp, u32]),
+        "luthe_trace_record_bits": (i32, [u32]),
+        "luthe_trace_pack": (i32, [u32, vp, sz, vp]),
+        "luthe_trace_unpack": (i32, [u32, vp, sz, vp]),
+        "luthe_trace_enable": (i32, [vp, sz]),
+        "luthe_trace_read": (i32, [vp, vp, sz, C.POINTER(sz)]),
         "luthe_iop_program": (i32, [u32, u32, C.POINTER(IopInfo), vp, sz, vp, sz]),
         "luthe_iop_lut": (i32, [u32, vp, C.POINTER(u32)]),
         "luthe_iop_eval_plain": (i32, [u32, u32, vp, vp, vp]),
@@ -396,6 +437,23 @@ class Context:
         return v.value
 
     # ---- batched device-memory entry points
+    def trace_enable(self, capacity):
+        self._chk(lib().luthe_trace_enable(self._h, capacity), "trace_enable")
+
+    def trace_read(self, cap=1 << 16):
+        buf = (TraceEvent * cap)()
+        n = C.c_size_t()
+        self._chk(lib().luthe_trace_read(self._h, C.cast(buf, C.c_void_p), cap, C.byref(n)), "trace_read")
+        return [buf[i].as_dict() for i in range(n.value)]
+
+    def upload_bsk_hpu_image(self, layout, image):
+        img = np.ascontiguousarray(image, np.uint64)
+        self._chk(lib().luthe_upload_bsk_hpu_image(self._h, C.byref(layout), _vp(img)), "upload_bsk_hpu_image")
+
+    def write_lut_hpu_image(self, gid, bodies):
+        b = np.ascontiguousarray(bodies, np.uint64).reshape(-1, self.params.poly_size)
+        self._chk(lib().luthe_write_lut_hpu_image(self._h, gid, _vp(b), b.shape[0]), "write_lut_hpu_image")
+
     def alu_batch_dev(self, opcode, d_a, d_b, d_out, n, mul_factor=0, msg_cst=0, stream=None):
         self._chk(lib().luthe_alu_batch(self._h, opcode, _vp(d_a), _vp(d_b) if d_b is not None else None,
                                         _vp(d_out), mul_factor, msg_cst, n, stream), "alu_batch")
@@ -692,3 +750,90 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------- HPU memory images (host only)
+def hpu_bsk_order(cut):
+    c = np.ascontiguousarray(cut, np.uint32)
+    out = np.zeros(1 << int(c.sum()), np.uint32)
+    rc = lib().luthe_hpu_bsk_order(_vp(c), c.size, _vp(out))
+    if rc != 0:
+        raise LutheError(rc, "hpu_bsk_order")
+    return out
+
+
+def hpu_bsk_to_image(params, layout, bsk_ntt):
+    src = np.ascontiguousarray(bsk_ntt, np.uint64)
+    out = np.empty(src.size, np.uint64)
+    rc = lib().luthe_hpu_bsk_to_image(C.byref(params), C.byref(layout), _vp(src), _vp(out))
+    if rc != 0:
+        raise LutheError(rc, "hpu_bsk_to_image")
+    return out
+
+
+def hpu_bsk_from_image(params, layout, image):
+    src = np.ascontiguousarray(image, np.uint64)
+    out = np.empty(src.size, np.uint64)
+    rc = lib().luthe_hpu_bsk_from_image(C.byref(params), C.byref(layout), _vp(src), _vp(out))
+    if rc != 0:
+        raise LutheError(rc, "hpu_bsk_from_image")
+    return out.reshape(params.bsk_shape())
+
+
+def hpu_ct_slot_bytes(params, layout):
+    return lib().luthe_hpu_ct_slot_bytes(C.byref(params), C.byref(layout))
+
+
+def hpu_ct_to_image(params, layout, blwe):
+    b = np.ascontiguousarray(blwe, np.uint64).reshape(-1, params.blen)
+    sb = hpu_ct_slot_bytes(params, layout)
+    if sb == 0:
+        raise LutheError(-1, "invalid layout")
+    out = np.zeros(layout.pem_pc * b.shape[0] * sb, np.uint8)
+    rc = lib().luthe_hpu_ct_to_image(C.byref(params), C.byref(layout), _vp(b), b.shape[0], _vp(out))
+    if rc != 0:
+        raise LutheError(rc, "hpu_ct_to_image")
+    return out
+
+
+def hpu_ct_from_image(params, layout, image, count):
+    img = np.ascontiguousarray(image, np.uint8)
+    out = np.zeros((count, params.blen), np.uint64)
+    rc = lib().luthe_hpu_ct_from_image(C.byref(params), C.byref(layout), _vp(img), count, _vp(out))
+    if rc != 0:
+        raise LutheError(rc, "hpu_ct_from_image")
+    return out
+
+
+def hpu_lut_from_image(params, bodies):
+    b = np.ascontiguousarray(bodies, np.uint64).reshape(-1, params.poly_size)
+    out = np.zeros((b.shape[0], params.lut_len), np.uint64)
+    rc = lib().luthe_hpu_lut_from_image(C.byref(params), _vp(b), b.shape[0], _vp(out))
+    if rc != 0:
+        raise LutheError(rc, "hpu_lut_from_image")
+    return out
+
+
+def trace_pack(lwe_dim, events):
+    n = len(events)
+    ev = (TraceEvent * max(1, n))()
+    for i, e in enumerate(events):
+        for k, v in e.items():
+            setattr(ev[i], k, v)
+    bits = lib().luthe_trace_record_bits(lwe_dim)
+    nw = (bits + 31) // 32
+    out = np.zeros(max(1, n) * nw, np.uint32)
+    rc = lib().luthe_trace_pack(lwe_dim, C.cast(ev, C.c_void_p), n, _vp(out))
+    if rc < 0:
+        raise LutheError(rc, "trace_pack")
+    return out[:n * nw].reshape(n, nw)
+
+
+def trace_unpack(lwe_dim, words):
+    w = np.ascontiguousarray(words, np.uint32)
+    n = w.shape[0]
+    ev = (TraceEvent * max(1, n))()
+    rc = lib().luthe_trace_unpack(lwe_dim, _vp(w), n, C.cast(ev, C.c_void_p))
+    if rc < 0:
+        raise LutheError(rc, "trace_unpack")
+    return [ev[i].as_dict() for i in range(n)]

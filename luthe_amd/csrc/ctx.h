@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <chrono>
+#include <deque>
 #include <map>
 #include <string>
 #include <vector>
@@ -50,6 +52,11 @@ struct luthe_ctx {
   std::map<uint32_t, std::vector<uint32_t>> iop_table;  // key = max_align << 8 | opcode
   int br_grid_override = 0;
   luthe_server* server = nullptr;  // attached serving engine (owns the stream while set)
+  // DOp-path trace (isc_trace_t events), a ring of trace_cap events
+  size_t trace_cap = 0;
+  std::deque<luthe_trace_event> trace;
+  std::chrono::steady_clock::time_point trace_t0;
+  std::map<uint64_t, std::vector<luthe_trace_event>> trace_retire;  // ticket -> RETIRE events
 };
 
 namespace lx {

@@ -150,6 +150,29 @@ int run_pbs(luthe_ctx* c, const uint64_t* in_base, const PbsJob* d_jobs, int n, 
   return LUTHE_OK;
 }
 
+void trace_push(luthe_ctx* c, const luthe_trace_event& e) {
+  if (!c->trace_cap) return;
+  if (c->trace.size() >= c->trace_cap) c->trace.pop_front();
+  c->trace.push_back(e);
+}
+
+uint32_t trace_now(luthe_ctx* c) {
+  return (uint32_t)std::chrono::duration_cast<std::chrono::microseconds>(std::chrono::steady_clock::now() -
+                                                                        c->trace_t0).count();
+}
+
+// RETIRE events of a completed submission.
+void trace_retire(luthe_ctx* c, uint64_t ticket) {
+  auto it = c->trace_retire.find(ticket);
+  if (it == c->trace_retire.end()) return;
+  const uint32_t now = trace_now(c);
+  for (luthe_trace_event e : it->second) {
+    e.timestamp = now;
+    trace_push(c, e);
+  }
+  c->trace_retire.erase(it);
+}
+
 }  // namespace lx
 
 using namespace lx;
@@ -553,6 +576,27 @@ static int execute_program(luthe_ctx* c, const std::vector<dop::Dop>& prog, uint
   int rc = stage_upload(c, blob);
   if (rc) return rc;
   const int blen = (int)c->blen;
+  // trace: REFILL of every DOp now, ISSUE per level as it is launched, RETIRE at completion
+  const uint64_t id = c->next_ticket;
+  std::vector<std::vector<uint32_t>> by_level;
+  if (c->trace_cap) {
+    by_level.resize(nlev);
+    const uint32_t now = trace_now(c);
+    std::vector<luthe_trace_event>& ret = c->trace_retire[id];
+    for (size_t i = 0; i < prog.size(); i++) {
+      luthe_trace_event e{};
+      e.timestamp = now;
+      e.insn = prog[i].word;
+      e.cmd = LUTHE_TRC_REFILL;
+      e.sync_id = (uint8_t)(id & 15);
+      e.vld = 1;
+      trace_push(c, e);
+      if (level[i] >= 0) by_level[level[i]].push_back((uint32_t)i);
+      e.cmd = LUTHE_TRC_RETIRE;
+      e.pe_reserved = (level[i] >= 0 && prog[i].kind == dop::KIND_PBS) ? (uint32_t)pbss[level[i]].size() : 0;
+      ret.push_back(e);
+    }
+  }
   for (int L = 0; L < nlev; L++) {
     const LevelPlan& lp = plan[L];
     if (lp.ncopy)
@@ -565,6 +609,19 @@ static int execute_program(luthe_ctx* c, const std::vector<dop::Dop>& prog, uint
     }
     c->cnt.alu += (uint64_t)lp.nalu;
     c->cnt.mem += (uint64_t)lp.ncopy;
+    if (c->trace_cap) {
+      const uint32_t now = trace_now(c);
+      for (uint32_t i : by_level[L]) {
+        luthe_trace_event e{};
+        e.timestamp = now;
+        e.insn = prog[i].word;
+        e.cmd = LUTHE_TRC_ISSUE;
+        e.sync_id = (uint8_t)(id & 15);
+        e.vld = 1;
+        e.pdg = 1;
+        trace_push(c, e);
+      }
+    }
   }
   c->cnt.levels += (uint64_t)nlev;
   c->cnt.dops += prog.size();
@@ -572,7 +629,7 @@ static int execute_program(luthe_ctx* c, const std::vector<dop::Dop>& prog, uint
   Ticket tk;
   HIPCHK(c, hipEventCreateWithFlags(&tk.done, hipEventDisableTiming));
   HIPCHK(c, hipEventRecord(tk.done, c->stream));
-  uint64_t id = c->next_ticket++;
+  c->next_ticket++;
   c->tickets[id] = tk;
   if (ticket) *ticket = id;
   return LUTHE_OK;
@@ -597,6 +654,7 @@ int luthe_poll(luthe_ctx* c, uint64_t ticket) {
   if (e != hipSuccess) return fail(c, LUTHE_ERR_HIP, "ticket: %s", hipGetErrorString(e));
   (void)hipEventDestroy(it->second.done);
   c->tickets.erase(it);
+  trace_retire(c, ticket);
   return 1;
 }
 
@@ -620,6 +678,25 @@ int luthe_wait(luthe_ctx* c, uint64_t ticket, int64_t timeout_us) {
   }
   (void)hipEventDestroy(it->second.done);
   c->tickets.erase(it);
+  trace_retire(c, ticket);
+  return LUTHE_OK;
+}
+
+int luthe_trace_enable(luthe_ctx* c, size_t capacity) {
+  if (!c) return LUTHE_ERR_ARG;
+  c->trace_cap = capacity;
+  c->trace.clear();
+  c->trace_retire.clear();
+  c->trace_t0 = std::chrono::steady_clock::now();
+  return LUTHE_OK;
+}
+
+int luthe_trace_read(luthe_ctx* c, luthe_trace_event* out, size_t cap, size_t* n) {
+  if (!c || !n || (cap && !out)) return LUTHE_ERR_ARG;
+  size_t k = std::min(cap, c->trace.size());
+  for (size_t i = 0; i < k; i++) out[i] = c->trace[i];
+  c->trace.erase(c->trace.begin(), c->trace.begin() + (ptrdiff_t)k);
+  *n = k;
   return LUTHE_OK;
 }
 
