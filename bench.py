@@ -152,7 +152,7 @@ def _pct(v, q):
     return float(np.percentile(np.asarray(v), q, method="nearest")) if len(v) else None
 
 
-def run_config4(la, ctx, keys, rates=(8, 20, 40, 60), seconds=5.0, burst=96, seed=4):
+def run_config4(la, ctx, keys, rates=(8, 20, 40, 60), seconds=5.0, burst=96, seed=4, cohort_max=0):
     """Config 4: serving.  Poisson arrivals (seeded) of radix IOps drawn uniformly from
     {ADD, CMP_GT, MUL} x {8, 16, 32} blocks of 2 bits, through the continuous-batching
     server (HBM arena admission, one PBS cohort per step); latency = submit -> result on
@@ -168,7 +168,7 @@ def run_config4(la, ctx, keys, rates=(8, 20, 40, 60), seconds=5.0, burst=96, see
     res = {"workload": "Poisson arrivals of radix IOps, uniform over {ADD, CMP_GT, MUL} x {8, 16, 32} blocks x 2 "
                        "bits, config-0 parameters, keys resident, continuous batching",
            "pbs_per_iop": {f"{name[o]}{n}": npbs[(o, n)] for o, n in mix}, "mean_pbs_per_request": mean_pbs}
-    srv = la.Server(ctx)
+    srv = la.Server(ctx, cohort_max=cohort_max)
     res["cohort_max"] = srv.stats()["cohort_max"]
     for k in mix:  # warm every program once
         srv.wait(srv.submit(k[0], pool_a[:k[1]], pool_b[:k[1]]))
@@ -236,6 +236,8 @@ def main():
     ap.add_argument("--br-grid", type=int, default=0, help="blind-rotation workgroups (0 = occupancy x CUs)")
     ap.add_argument("--no-extra", action="store_true", help="skip the config-1, -3 and -4 measurements")
     ap.add_argument("--serve-seconds", type=float, default=5.0, help="arrival window per rate (config 4)")
+    ap.add_argument("--serve-cohort-max", type=int, default=0, help="PBS per cohort (config 4; 0 = server default)")
+    ap.add_argument("--serve-only", action="store_true", help="config 4 only (after the config-2 headline)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="only run a device-to-device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -354,11 +356,13 @@ def main():
     }
 
     if not args.no_extra:
-        out["config1_ntt"] = run_config1(la, hip, ctx, peak)
-        log(f"bench: config 1 {out['config1_ntt']}")
-        out["config3_uneven"] = run_config3(la, hip)
-        log(f"bench: config 3 {out['config3_uneven']}")
-        out["config4_serving"] = run_config4(la, ctx, keys, seconds=args.serve_seconds)
+        if not args.serve_only:
+            out["config1_ntt"] = run_config1(la, hip, ctx, peak)
+            log(f"bench: config 1 {out['config1_ntt']}")
+            out["config3_uneven"] = run_config3(la, hip)
+            log(f"bench: config 3 {out['config3_uneven']}")
+        out["config4_serving"] = run_config4(la, ctx, keys, seconds=args.serve_seconds,
+                                             cohort_max=args.serve_cohort_max)
 
     if not args.no_cpu_baseline:
         from oracle import ref  # CPU baseline leg only

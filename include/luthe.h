@@ -227,6 +227,13 @@ void* luthe_stream(luthe_ctx* ctx);
 /* Blind-rotation launch width (workgroups of the persistent kernel); 0 = occupancy x CUs. */
 int luthe_set_br_grid(luthe_ctx* ctx, int workgroups);
 int luthe_get_br_grid(luthe_ctx* ctx);
+/* Batches of at most max_cohort ciphertexts run the latency variant of the blind rotation
+ * (512 threads per ciphertext, one GLWE polynomial per half); larger ones the throughput
+ * kernel.  Results are identical.  Default: the variant's resident capacity (CUs x its
+ * workgroups per CU: 512 at L=1, 256 at L=2 on MI355X); 0 disables it.  No
+ * reference counterpart (the HPU's PBS latency is fixed by its pipeline). */
+int luthe_set_br_latency_max(luthe_ctx* ctx, int max_cohort);
+int luthe_get_br_latency_max(luthe_ctx* ctx);
 /* Diagnostics: throughput of the Goldilocks modular multiply (64x64 -> 128-bit product
  * plus reduction) at full occupancy, in 10^9 modmul/s; the VALU roofline denominator of
  * the blind rotation.  No reference counterpart (the HPU's rate is fixed by its RTL). */

@@ -160,6 +160,8 @@ def lib():
         "luthe_stream": (vp, [vp]),
         "luthe_set_br_grid": (i32, [vp, i32]),
         "luthe_get_br_grid": (i32, [vp]),
+        "luthe_set_br_latency_max": (i32, [vp, i32]),
+        "luthe_get_br_latency_max": (i32, [vp]),
         "luthe_microbench_modmul": (i32, [vp, C.POINTER(C.c_double)]),
         "luthe_dop_schedule": (i32, [vp, sz, u32, u32, u32, vp]),
         "luthe_alu_batch": (i32, [vp, u32, vp, vp, vp, u32, u32, sz, vp]),

@@ -51,6 +51,7 @@ struct luthe_ctx {
   std::map<uint64_t, Ticket> tickets;
   std::map<uint32_t, std::vector<uint32_t>> iop_table;  // key = max_align << 8 | opcode
   int br_grid_override = 0;
+  int br_latency_max = -1;  // cohorts up to this size use the latency kernel; -1 = its resident capacity
   luthe_server* server = nullptr;  // attached serving engine (owns the stream while set)
   // DOp-path trace (isc_trace_t events), a ring of trace_cap events
   size_t trace_cap = 0;
@@ -70,6 +71,8 @@ int ensure_jobs(luthe_ctx* c, size_t n);
 // device; rows are relative to in_base / out_base.
 int run_pbs(luthe_ctx* c, const uint64_t* in_base, const lk::PbsJob* d_jobs, int n, uint64_t* out_base,
             hipStream_t s);
+// Largest cohort that runs the 512-thread latency kernel.
+int latency_max(luthe_ctx* c);
 
 }  // namespace lx
 

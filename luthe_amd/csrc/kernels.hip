@@ -324,6 +324,102 @@ __global__ __launch_bounds__(256, (L == 1 ? 4 : 2)) void k_blind_rotate(const ui
 }
 
 // ------------------------------------------------------------------------------------
+// Latency variant of the blind rotation for small cohorts: one ciphertext per 512-thread
+// workgroup.  Half h (256 threads, same per-thread mapping as above) owns GLWE
+// polynomial h: its rotation and decomposition, the forward NTTs of its L digit
+// polynomials, output column j = h of the external product and its inverse NTT.  The
+// halves exchange NTT-domain digits through LDS before the product.  Arithmetic is the
+// same canonical Goldilocks arithmetic, so results equal k_blind_rotate's bit for bit.
+// ------------------------------------------------------------------------------------
+template <int K1, int L>
+__global__ __launch_bounds__(512, 1) void k_blind_rotate_lat(const uint64_t* __restrict__ bsk,
+                                                             const uint64_t* __restrict__ luts,
+                                                             const uint16_t* __restrict__ lwe2n,
+                                                             const PbsJob* __restrict__ jobs, int nct,
+                                                             uint64_t* __restrict__ out_base,
+                                                             const uint64_t* __restrict__ tab, BrArgs a) {
+  static_assert(K1 == 2, "one half per GLWE polynomial");
+  constexpr int NPF = K1 * L;
+  __shared__ uint64_t lds[NPF * N];  // acc (K1 polys) / digit NTTs (K1*L) / exchange (K1*L*8*T)
+  const int t2 = threadIdx.x;
+  const int h = t2 >> 8, t = t2 & 255;
+  const int r = t & 63, m0 = t >> 6;
+  const size_t blen = (size_t)(K1 - 1) * N + 1;
+  const size_t bsk_stride = (size_t)8 * NPF * K1 * T;
+  for (int c = blockIdx.x; c < nct; c += gridDim.x) {
+    const PbsJob job = jobs[c];
+    const uint16_t* a2n = lwe2n + (size_t)c * a.lwe_stride;
+    const uint64_t* lut = luts + (size_t)job.gid * K1 * N;
+    uint64_t acc[8];
+#pragma unroll
+    for (int m1 = 0; m1 < 8; m1++) acc[m1] = lut[h * N + 64 * (m0 + 4 * m1) + r];
+    uint32_t rot_next = a2n[0];
+    for (int i = 0; i < a.n_lwe; i++) {
+      const uint32_t rot = rot_next;
+      rot_next = a2n[i + 1];  // i + 1 <= n_lwe: the body slot, always present
+      const uint64_t* bsk_i = bsk + (size_t)i * bsk_stride;
+      int tt = t;
+      asm volatile("" : "+v"(tt));
+      const int rr = tt & 63, mm0 = tt >> 6;
+      // key slice of this half's output column, issued early
+      uint64_t kb[8][NPF];
+#pragma unroll
+      for (int w0 = 0; w0 < 8; w0++)
+#pragma unroll
+        for (int gl_ = 0; gl_ < NPF; gl_++) kb[w0][gl_] = bsk_i[((w0 * NPF + gl_) * K1 + h) * T + tt];
+      __syncthreads();
+#pragma unroll
+      for (int m1 = 0; m1 < 8; m1++) lds[h * N + 64 * (mm0 + 4 * m1) + rr] = acc[m1];
+      __syncthreads();
+      uint64_t v[L][8];
+      pbs::rotate_decompose<L>(lds + h * N, acc, tt, rot, a.base_log, v);
+      __syncthreads();
+      fwd_passes<L>(v, lds + h * L * N, tab, tt);
+      // exchange the digit NTTs: row gl = g * L + l at [(gl * 8 + w0) * T + t]
+      __syncthreads();
+#pragma unroll
+      for (int l = 0; l < L; l++)
+#pragma unroll
+        for (int w0 = 0; w0 < 8; w0++) lds[((h * L + l) * 8 + w0) * T + tt] = v[l][w0];
+      __syncthreads();
+      uint64_t z[1][8];
+#pragma unroll
+      for (int w0 = 0; w0 < 8; w0++) {
+        uint64_t s = 0;
+#pragma unroll
+        for (int gl_ = 0; gl_ < NPF; gl_++) {
+          const uint64_t d = lds[(gl_ * 8 + w0) * T + tt];
+          s = gl::add(s, gl::mul(d, kb[w0][gl_]));
+        }
+        z[0][w0] = s;
+      }
+      inv_passes<1>(z, lds + h * N, tab, tt);
+#pragma unroll
+      for (int m1 = 0; m1 < 8; m1++) acc[m1] += gl::modswitch_p_to_q(z[0][m1]);
+    }
+    // sample extract
+    __syncthreads();
+#pragma unroll
+    for (int m1 = 0; m1 < 8; m1++) lds[h * N + 64 * (m0 + 4 * m1) + r] = acc[m1];
+    __syncthreads();
+    const uint32_t b = a2n[a.n_lwe];
+    const int lut_nb = 1 << job.log_lut_nb;
+    for (int lu = 0; lu < lut_nb; lu++) {
+      const uint32_t offs = (uint32_t)(lu * (N >> job.log_lut_nb));
+      uint64_t* o = out_base + (size_t)(job.out_row + lu) * blen;
+#pragma unroll
+      for (int g = 0; g < K1 - 1; g++)
+#pragma unroll
+        for (int e = 0; e < N / (2 * T); e++) {
+          int m = t2 + 2 * T * e;
+          o[g * N + m] = pbs::extract_coef(lds, g, m, b, offs);
+        }
+      if (t2 == 0) o[(K1 - 1) * N] = pbs::extract_coef(lds, K1 - 1, 0, b, offs);
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------
 // ALU DOps (pea_alu_core.sv:55-160) and LD/ST row copies.
 // ------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_alu(uint64_t* __restrict__ base, const AluOp* __restrict__ ops, int blen) {
@@ -477,10 +573,35 @@ static int br_max_grid() {
   return cached;
 }
 
+template <int K1, int L>
+static int br_lat_max_grid() {
+  static int cached = 0;
+  if (!cached) {
+    int dev = 0, cus = 0, per = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_blind_rotate_lat<K1, L>, 512, 0);
+    cached = (cus > 0 ? cus : 256) * (per > 0 ? per : 1);
+  }
+  return cached;
+}
+
 hipError_t launch_blind_rotate(const uint64_t* bsk, const uint64_t* luts, const uint16_t* lwe2n, const PbsJob* jobs,
                                int nct, uint64_t* out_base, const uint64_t* tab, const BrArgs& a,
-                               int pbs_level, int grid_override, hipStream_t s) {
+                               int pbs_level, int grid_override, hipStream_t s, int latency_max) {
   if (nct == 0) return hipSuccess;
+  if (nct <= latency_max && (pbs_level == 1 || pbs_level == 2)) {
+    if (pbs_level == 1) {
+      int g = br_lat_max_grid<2, 1>();
+      g = g < nct ? g : nct;
+      k_blind_rotate_lat<2, 1><<<dim3(g), 512, 0, s>>>(bsk, luts, lwe2n, jobs, nct, out_base, tab, a);
+    } else {
+      int g = br_lat_max_grid<2, 2>();
+      g = g < nct ? g : nct;
+      k_blind_rotate_lat<2, 2><<<dim3(g), 512, 0, s>>>(bsk, luts, lwe2n, jobs, nct, out_base, tab, a);
+    }
+    return hipGetLastError();
+  }
   if (pbs_level == 1) {
     int g = grid_override > 0 ? grid_override : br_max_grid<2, 1>();
     g = g < nct ? g : nct;
@@ -497,6 +618,10 @@ hipError_t launch_blind_rotate(const uint64_t* bsk, const uint64_t* luts, const 
 
 int blind_rotate_occupancy(int pbs_level) {
   return pbs_level == 1 ? br_max_grid<2, 1>() : br_max_grid<2, 2>();
+}
+
+int blind_rotate_latency_slots(int pbs_level) {
+  return pbs_level == 1 ? br_lat_max_grid<2, 1>() : br_lat_max_grid<2, 2>();
 }
 
 hipError_t launch_alu(uint64_t* base, const AluOp* ops, int nops, int blen, hipStream_t s) {

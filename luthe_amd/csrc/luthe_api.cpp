@@ -132,6 +132,14 @@ int stage_upload(luthe_ctx* c, const std::vector<uint8_t>& bytes) {
 
 hipStream_t pick_stream(luthe_ctx* c, void* s) { return s ? (hipStream_t)s : c->stream; }
 
+int latency_max(luthe_ctx* c) {
+  if (c->br_latency_max < 0) {
+    (void)hipSetDevice(c->device);
+    c->br_latency_max = lk::blind_rotate_latency_slots((int)c->prm.pbs_level);
+  }
+  return c->br_latency_max;
+}
+
 // KS + BR over jobs already on the device.
 int run_pbs(luthe_ctx* c, const uint64_t* in_base, const PbsJob* d_jobs, int n, uint64_t* out_base, hipStream_t s) {
   if (!c->bsk_ok || !c->ksk_ok) return fail(c, LUTHE_ERR_STATE, "keys not uploaded");
@@ -143,7 +151,7 @@ int run_pbs(luthe_ctx* c, const uint64_t* in_base, const PbsJob* d_jobs, int n, 
   HIPCHK(c, lk::launch_keyswitch(in_base, d_jobs, n, c->d_ksk, c->d_lwe2n, ka, (int)p.ks_level, s));
   lk::BrArgs ba{(int)p.lwe_dim, c->lwe_stride, p.pbs_base_log};
   HIPCHK(c, lk::launch_blind_rotate(c->d_bsk, c->d_luts, c->d_lwe2n, d_jobs, n, out_base, c->d_tab, ba,
-                                    (int)p.pbs_level, c->br_grid_override, s));
+                                    (int)p.pbs_level, c->br_grid_override, s, latency_max(c)));
   c->cnt.pbs += (uint64_t)n;
   c->cnt.pbs_batches += 1;
   c->cnt.pbs_max_batch = std::max<uint64_t>(c->cnt.pbs_max_batch, (uint64_t)n);
@@ -276,6 +284,17 @@ int luthe_set_br_grid(luthe_ctx* c, int workgroups) {
   if (!c || workgroups < 0) return LUTHE_ERR_ARG;
   c->br_grid_override = workgroups;
   return LUTHE_OK;
+}
+
+int luthe_set_br_latency_max(luthe_ctx* c, int max_cohort) {
+  if (!c || max_cohort < 0) return LUTHE_ERR_ARG;
+  c->br_latency_max = max_cohort;
+  return LUTHE_OK;
+}
+
+int luthe_get_br_latency_max(luthe_ctx* c) {
+  if (!c) return LUTHE_ERR_ARG;
+  return latency_max(c);
 }
 
 int luthe_get_br_grid(luthe_ctx* c) {
@@ -424,7 +443,7 @@ int luthe_blind_rotate_batch(luthe_ctx* c, const uint16_t* d_lwe2n, uint64_t* d_
   const luthe_params& p = c->prm;
   lk::BrArgs ba{(int)p.lwe_dim, (int)p.lwe_dim + 1, p.pbs_base_log};
   HIPCHK(c, lk::launch_blind_rotate(c->d_bsk, c->d_luts, d_lwe2n, c->d_jobs, (int)n, d_out, c->d_tab, ba,
-                                    (int)p.pbs_level, c->br_grid_override, s));
+                                    (int)p.pbs_level, c->br_grid_override, s, latency_max(c)));
   return LUTHE_OK;
 }
 
