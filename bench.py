@@ -237,7 +237,7 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the config-1, -3 and -4 measurements")
     ap.add_argument("--serve-seconds", type=float, default=5.0, help="arrival window per rate (config 4)")
     ap.add_argument("--serve-cohort-max", type=int, default=0, help="PBS per cohort (config 4; 0 = server default)")
-    ap.add_argument("--serve-only", action="store_true", help="config 4 only (after the config-2 headline)")
+    ap.add_argument("--extras", default="c1,c3,c4", help="extra configurations to run beside the headline")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="only run a device-to-device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -355,12 +355,14 @@ def main():
                      "traffic_gbps": round(traffic / (br_ms * 1e-3) / 1e9, 2) if traffic else None},
     }
 
-    if not args.no_extra:
-        if not args.serve_only:
-            out["config1_ntt"] = run_config1(la, hip, ctx, peak)
-            log(f"bench: config 1 {out['config1_ntt']}")
-            out["config3_uneven"] = run_config3(la, hip)
-            log(f"bench: config 3 {out['config3_uneven']}")
+    extras = set() if args.no_extra else {e.strip() for e in args.extras.split(",") if e.strip()}
+    if "c1" in extras:
+        out["config1_ntt"] = run_config1(la, hip, ctx, peak)
+        log(f"bench: config 1 {out['config1_ntt']}")
+    if "c3" in extras:
+        out["config3_uneven"] = run_config3(la, hip)
+        log(f"bench: config 3 {out['config3_uneven']}")
+    if "c4" in extras:
         out["config4_serving"] = run_config4(la, ctx, keys, seconds=args.serve_seconds,
                                              cohort_max=args.serve_cohort_max)
 
