@@ -372,6 +372,26 @@ int luthe_iop_lut(uint32_t lut, uint8_t* table16, uint32_t* log_lut_nb);
 /* Plaintext evaluation of a program on block values (PBS = table lookup): the host check of its logic. */
 int luthe_iop_eval_plain(uint32_t op, uint32_t nblocks, const uint32_t* a, const uint32_t* b, uint32_t* out);
 
+/* IOp -> DOp template (the translation-table entry the reference firmware patches,
+ * ucore_fw.c:135-220): sources TS[0].i = a, TS[1].i = b, results TD[0].i, spills to heap
+ * slots (TH), PBS on LUT ids lut_base + logical LUT, for a register file of `regs`
+ * registers.  DOps are issued level by level (the last PBS of a level carries the flush
+ * bit, docs/dop.md:108-111); Many-LUT destinations are aligned.  *n = words written
+ * (call with words = NULL to size).  The IOp opcode is the reference's (docs/iop.md:176-196:
+ * ADD 0xE0, CMP_GT 0xC0, MUL 0xE4; ripple ADD uses user opcode 0x60) with alignment
+ * 2 * nblocks bits. */
+typedef struct luthe_iop_dop_stats {
+  uint32_t dops, ld, st, alu, pbs, heap_slots, levels;
+  uint32_t iop_opcode, align_bits;
+} luthe_iop_dop_stats;
+int luthe_iop_program_dops(uint32_t op, uint32_t nblocks, uint32_t regs, uint32_t lut_base, uint32_t* words,
+                           size_t cap, size_t* n, luthe_iop_dop_stats* stats);
+/* Write the programs' LUTs into LUT slots lut_base .. lut_base + LUTHE_IOP_LUTS - 1. */
+int luthe_write_iop_luts(luthe_ctx* ctx, uint32_t lut_base);
+/* luthe_iop_program_dops for the context's register file + luthe_register_iop under the
+ * program's IOp opcode and alignment. */
+int luthe_register_iop_program(luthe_ctx* ctx, uint32_t op, uint32_t nblocks, uint32_t lut_base);
+
 /* ---------------------------------------------------------------- serving engine
  * Continuous batching of radix IOps.  Replaces the backend-to-HPU IOp queue and ack
  * (ucore_fw.c:54-232), the instruction scheduler's out-of-order issue

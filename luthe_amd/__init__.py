@@ -95,6 +95,11 @@ class TraceEvent(C.Structure):
 TRC_NONE, TRC_RDUNLOCK, TRC_RETIRE, TRC_REFILL, TRC_ISSUE = 0, 1, 2, 3, 4
 
 
+class IopDopStats(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("dops", "ld", "st", "alu", "pbs", "heap_slots", "levels", "iop_opcode",
+                                          "align_bits")]
+
+
 class ServerConfig(C.Structure):
     _fields_ = [("arena_bytes", C.c_uint64), ("pinned_bytes", C.c_uint64), ("cohort_max", C.c_uint32),
                 ("cohort_min", C.c_uint32), ("cohort_timeout_us", C.c_uint32), ("pipeline_depth", C.c_uint32),
@@ -197,6 +202,9 @@ def lib():
         "luthe_trace_read": (i32, [vp, vp, sz, C.POINTER(sz)]),
         "luthe_iop_program": (i32, [u32, u32, C.POINTER(IopInfo), vp, sz, vp, sz]),
         "luthe_iop_lut": (i32, [u32, vp, C.POINTER(u32)]),
+        "luthe_iop_program_dops": (i32, [u32, u32, u32, u32, vp, sz, C.POINTER(sz), C.POINTER(IopDopStats)]),
+        "luthe_write_iop_luts": (i32, [vp, u32]),
+        "luthe_register_iop_program": (i32, [vp, u32, u32, u32]),
         "luthe_iop_eval_plain": (i32, [u32, u32, vp, vp, vp]),
         "luthe_server_config_default": (None, [C.POINTER(ServerConfig)]),
         "luthe_server_create": (i32, [vp, C.POINTER(ServerConfig), C.POINTER(vp)]),
@@ -439,6 +447,12 @@ class Context:
         return v.value
 
     # ---- batched device-memory entry points
+    def write_iop_luts(self, lut_base):
+        self._chk(lib().luthe_write_iop_luts(self._h, lut_base), "write_iop_luts")
+
+    def register_iop_program(self, op, nblocks, lut_base):
+        self._chk(lib().luthe_register_iop_program(self._h, op, nblocks, lut_base), "register_iop_program")
+
     def trace_enable(self, capacity):
         self._chk(lib().luthe_trace_enable(self._h, capacity), "trace_enable")
 
@@ -649,6 +663,20 @@ def iop_program(op, nblocks):
     if rc != 0:
         raise LutheError(rc, "iop_program")
     return info, nodes, outs
+
+
+def iop_program_dops(op, nblocks, regs=64, lut_base=0):
+    """(DOp template words, stats dict) of a radix IOp program (host only)."""
+    n = C.c_size_t()
+    st = IopDopStats()
+    rc = lib().luthe_iop_program_dops(op, nblocks, regs, lut_base, None, 0, C.byref(n), C.byref(st))
+    if rc != 0:
+        raise LutheError(rc, "iop_program_dops")
+    w = np.zeros(n.value, np.uint32)
+    rc = lib().luthe_iop_program_dops(op, nblocks, regs, lut_base, _vp(w), w.size, C.byref(n), C.byref(st))
+    if rc != 0:
+        raise LutheError(rc, "iop_program_dops")
+    return w, {k: getattr(st, k) for k, _ in st._fields_}
 
 
 def iop_lut(lut):

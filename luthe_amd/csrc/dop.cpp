@@ -234,7 +234,10 @@ int parse_iop(const uint32_t* w, size_t n, IopBundle& out, std::string& why) {
       uint32_t kind = o >> 30;
       if (kind != (list == 0 ? 1u : 0u)) return why = "IOp operand of the wrong kind", LUTHE_ERR_DOP;
       if (ops.size() >= 16) return why = "too many IOp operands", LUTHE_ERR_DOP;
-      ops.push_back(Operand{o & 0xFFFF, ((o >> 24) & 31) + 1});
+      // hpu_iop_fmt.h:38-45: base_cid, block (blocks per integer), vec_size (integers - 1);
+      // a template addresses blocks 0 .. (vec_size + 1) * block - 1 of the operand
+      const uint32_t blocks = (o >> 16) & 0xFF, vec = ((o >> 24) & 31) + 1;
+      ops.push_back(Operand{o & 0xFFFF, vec * (blocks ? blocks : 1u)});
       if ((o >> 29) & 1) break;  // is_last
     }
   }

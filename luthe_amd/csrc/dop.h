@@ -47,7 +47,7 @@ int check_template(const uint32_t* words, size_t n, uint32_t regs, uint32_t luts
 int schedule_levels(const std::vector<Dop>& prog, uint32_t regs, std::vector<int>& level);
 
 struct Operand {
-  uint32_t cid_ofst = 0, len = 0;
+  uint32_t cid_ofst = 0, len = 0;  // len: blocks the operand spans
 };
 struct IopBundle {
   uint32_t header = 0;

@@ -17,6 +17,9 @@
 #include "iop_programs.h"
 
 #include <algorithm>
+#include <deque>
+
+#include "dop.h"
 
 namespace iop {
 
@@ -271,6 +274,243 @@ bool build(uint32_t op, uint32_t nblocks, Program& out) {
     case OP_MUL: build_mul(B, nblocks); break;
     default: return false;
   }
+  return true;
+}
+
+uint8_t iop_opcode(uint32_t op) {
+  switch (op) {
+    case OP_ADD: return 0xE0;
+    case OP_CMP_GT: return 0xC0;
+    case OP_MUL: return 0xE4;
+    default: return 0x60;  // user range (0x00..0x7f): ripple-carry ADD
+  }
+}
+
+bool emit_dops(const Program& p, uint32_t regs, uint32_t lut_base, std::vector<uint32_t>& words, EmitStats& st) {
+  words.clear();
+  st = EmitStats();
+  if (regs < 4 || regs > 128) return false;
+  const size_t nn = p.nodes.size();
+  // ASAP level of every node (inputs at level 0)
+  std::vector<int> writer(p.rows, -1);
+  for (size_t i = 0; i < nn; i++) {
+    const Node& n = p.nodes[i];
+    for (uint32_t k = 0; k < (n.pbs ? (1u << n.log_lut_nb) : 1u); k++) writer[n.dst + k] = (int)i;
+  }
+  std::vector<uint32_t> lvl(nn, 0);
+  for (size_t i = 0; i < nn; i++) {
+    const Node& n = p.nodes[i];
+    uint32_t l = 0;
+    int w0 = writer[n.src0];
+    if (w0 >= 0) l = std::max(l, lvl[(size_t)w0] + 1);
+    if (!n.pbs) {
+      int w1 = writer[n.src1];
+      if (w1 >= 0) l = std::max(l, lvl[(size_t)w1] + 1);
+    }
+    lvl[i] = l;
+  }
+  std::vector<uint32_t> order(nn);
+  for (size_t i = 0; i < nn; i++) order[i] = (uint32_t)i;
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return lvl[x] < lvl[y]; });
+  // uses of every row, as positions in the issue order
+  std::vector<std::vector<uint32_t>> uses(p.rows);
+  for (uint32_t pos = 0; pos < nn; pos++) {
+    const Node& n = p.nodes[order[pos]];
+    uses[n.src0].push_back(pos);
+    if (!n.pbs && n.src1 != n.src0) uses[n.src1].push_back(pos);
+  }
+  std::vector<int> out_block(p.rows, -1);
+  for (size_t k = 0; k < p.out_rows.size(); k++) out_block[p.out_rows[k]] = (int)k;
+  std::vector<size_t> next_use(p.rows, 0);  // index into uses[row]
+  auto next_pos = [&](uint32_t row) -> uint32_t {
+    return next_use[row] < uses[row].size() ? uses[row][next_use[row]] : 0xFFFFFFFFu;
+  };
+  std::vector<int> reg_of(p.rows, -1), heap_of(p.rows, -1), content(regs, -1);
+  std::vector<uint32_t> pinned(regs, 0);
+  std::deque<uint32_t> free_regs;
+  for (uint32_t r = 0; r < regs; r++) free_regs.push_back(r);
+  std::vector<uint32_t> free_heap;
+  uint32_t heap_top = 0;
+  auto mem_slot = [&](uint32_t row, uint32_t& mode, uint32_t& slot) -> bool {
+    if (row >= p.a_row0 && row < p.a_row0 + p.nblocks) {
+      mode = dop::MEM_SRC, slot = ((0u) << 8) | (row - p.a_row0);
+      return true;
+    }
+    if (row >= p.b_row0 && row < p.b_row0 + p.nblocks) {
+      mode = dop::MEM_SRC, slot = ((1u) << 8) | (row - p.b_row0);
+      return true;
+    }
+    if (heap_of[row] >= 0) {
+      mode = dop::MEM_HEAP, slot = (uint32_t)heap_of[row];
+      return true;
+    }
+    return false;
+  };
+  auto release = [&](uint32_t r) {
+    content[r] = -1;
+    free_regs.push_back(r);
+  };
+  // frees a register holding a value: spill it first if it is still needed and not in memory
+  auto evict = [&](uint32_t r) {
+    const int row = content[r];
+    if (row >= 0) {
+      uint32_t mode, slot;
+      if (next_pos((uint32_t)row) != 0xFFFFFFFFu && !mem_slot((uint32_t)row, mode, slot)) {
+        uint32_t h;
+        if (!free_heap.empty()) {
+          h = free_heap.back();
+          free_heap.pop_back();
+        } else {
+          h = heap_top++;
+        }
+        heap_of[row] = (int)h;
+        words.push_back(dop::enc_mem(dop::OP_ST, r, dop::MEM_HEAP, h));
+        st.st++;
+      }
+      reg_of[row] = -1;
+    }
+    content[r] = -1;
+  };
+  // victim: unpinned register whose value is next used farthest away
+  auto victim = [&](uint32_t align, uint32_t count) -> int {
+    int best = -1;
+    uint64_t best_key = 0;
+    for (uint32_t r0 = 0; r0 + count <= regs; r0 += align) {
+      uint64_t key = ~0ull;
+      bool ok = true;
+      for (uint32_t k = 0; k < count && ok; k++) {
+        if (pinned[r0 + k]) ok = false;
+        else if (content[r0 + k] >= 0) key = std::min<uint64_t>(key, next_pos((uint32_t)content[r0 + k]));
+      }
+      if (ok && (best < 0 || key > best_key)) best = (int)r0, best_key = key;
+    }
+    return best;
+  };
+  // `count` consecutive registers aligned on `align`, evicting as needed
+  auto grab = [&](uint32_t align, uint32_t count) -> int {
+    if (count == 1) {
+      for (size_t i = 0; i < free_regs.size(); i++) {
+        uint32_t r = free_regs[i];
+        if (!pinned[r] && content[r] < 0) {
+          free_regs.erase(free_regs.begin() + (ptrdiff_t)i);
+          return (int)r;
+        }
+      }
+    } else {
+      for (uint32_t r0 = 0; r0 + count <= regs; r0 += align) {
+        bool ok = true;
+        for (uint32_t k = 0; k < count; k++) ok &= !pinned[r0 + k] && content[r0 + k] < 0;
+        if (ok) {
+          for (uint32_t k = 0; k < count; k++)
+            free_regs.erase(std::find(free_regs.begin(), free_regs.end(), r0 + k));
+          return (int)r0;
+        }
+      }
+    }
+    int r0 = victim(align, count);
+    if (r0 < 0) return -1;
+    for (uint32_t k = 0; k < count; k++) {
+      evict((uint32_t)r0 + k);
+      auto it = std::find(free_regs.begin(), free_regs.end(), (uint32_t)r0 + k);
+      if (it != free_regs.end()) free_regs.erase(it);
+    }
+    return r0;
+  };
+  auto load = [&](uint32_t row) -> int {
+    if (reg_of[row] >= 0) return reg_of[row];
+    uint32_t mode, slot;
+    if (!mem_slot(row, mode, slot)) return -1;  // never produced
+    int r = grab(1, 1);
+    if (r < 0) return -1;
+    words.push_back(dop::enc_mem(dop::OP_LD, (uint32_t)r, mode, slot));
+    st.ld++;
+    reg_of[row] = r;
+    content[r] = (int)row;
+    return r;
+  };
+  auto drop_if_dead = [&](uint32_t row) {
+    if (next_pos(row) != 0xFFFFFFFFu) return;
+    if (reg_of[row] >= 0) release((uint32_t)reg_of[row]), reg_of[row] = -1;
+    if (heap_of[row] >= 0) free_heap.push_back((uint32_t)heap_of[row]), heap_of[row] = -1;
+  };
+  auto store_output = [&](uint32_t row, uint32_t r) {
+    if (out_block[row] < 0) return;
+    words.push_back(dop::enc_mem(dop::OP_ST, r, dop::MEM_DST, ((0u) << 8) | (uint32_t)out_block[row]));
+    st.st++;
+    out_block[row] = -1;
+  };
+  size_t last_pbs_word = (size_t)-1;
+  uint32_t cur_level = 0;
+  for (uint32_t pos = 0; pos < nn; pos++) {
+    const Node& n = p.nodes[order[pos]];
+    if (lvl[order[pos]] != cur_level) {
+      if (last_pbs_word != (size_t)-1) words[last_pbs_word] |= 8u << 26;  // flush the level's last PBS
+      last_pbs_word = (size_t)-1;
+      cur_level = lvl[order[pos]];
+    }
+    // sources into registers, pinned for this node
+    int rs0 = load(n.src0);
+    if (rs0 < 0) return false;
+    pinned[(uint32_t)rs0] = 1;
+    int rs1 = rs0;
+    if (!n.pbs) {
+      rs1 = load(n.src1);
+      if (rs1 < 0) return false;
+      pinned[(uint32_t)rs1] = 1;
+    }
+    // consume this use of the sources
+    next_use[n.src0]++;
+    if (!n.pbs && n.src1 != n.src0) next_use[n.src1]++;
+    pinned[(uint32_t)rs0] = 0;
+    pinned[(uint32_t)rs1] = 0;
+    const uint32_t cnt = n.pbs ? (1u << n.log_lut_nb) : 1u;
+    // a dying source register may take the result (no Many-LUT alignment issue for 1 reg)
+    for (uint32_t row : {n.src0, n.src1}) {
+      if (n.pbs && row != n.src0) continue;
+      if (next_pos(row) == 0xFFFFFFFFu && reg_of[row] >= 0 && !(cnt > 1)) {
+        uint32_t r = (uint32_t)reg_of[row];
+        reg_of[row] = -1;
+        content[r] = -1;
+        if (heap_of[row] >= 0) free_heap.push_back((uint32_t)heap_of[row]), heap_of[row] = -1;
+        free_regs.push_back(r);
+      }
+    }
+    pinned[(uint32_t)rs0] = reg_of[n.src0] >= 0 ? 1 : 0;
+    if (!n.pbs && reg_of[n.src1] >= 0) pinned[(uint32_t)rs1] = 1;
+    int rd = grab(cnt, cnt);
+    if (rd < 0) return false;
+    pinned[(uint32_t)rs0] = 0;
+    pinned[(uint32_t)rs1] = 0;
+    if (n.pbs) {
+      words.push_back(dop::enc_pbs((uint32_t)rd, (uint32_t)rs0, lut_base + n.lut, n.log_lut_nb, 0));
+      last_pbs_word = words.size() - 1;
+      st.pbs++;
+    } else {
+      words.push_back(dop::enc_arith(n.opcode, (uint32_t)rd, (uint32_t)rs0, (uint32_t)rs1, n.mul_factor));
+      st.alu++;
+    }
+    for (uint32_t k = 0; k < cnt; k++) {
+      const uint32_t row = n.dst + k;
+      reg_of[row] = rd + (int)k;
+      content[(uint32_t)rd + k] = (int)row;
+      store_output(row, (uint32_t)rd + k);
+      drop_if_dead(row);
+    }
+    drop_if_dead(n.src0);
+    if (!n.pbs) drop_if_dead(n.src1);
+  }
+  if (last_pbs_word != (size_t)-1) words[last_pbs_word] |= 8u << 26;
+  // outputs that no node produced (inputs passed through)
+  for (size_t k = 0; k < p.out_rows.size(); k++) {
+    const uint32_t row = p.out_rows[k];
+    if (out_block[row] < 0) continue;
+    int r = load(row);
+    if (r < 0) return false;
+    store_output(row, (uint32_t)r);
+  }
+  st.dops = (uint32_t)words.size();
+  st.heap_slots = heap_top;
+  st.levels = nn ? cur_level + 1 : 0;
   return true;
 }
 

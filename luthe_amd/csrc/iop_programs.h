@@ -65,4 +65,17 @@ bool build(uint32_t op, uint32_t nblocks, Program& out);
 // host-side check of a program's logic.
 void eval_plain(const Program& p, const uint32_t* a, const uint32_t* b, uint32_t* out);
 
+// DOp template of a program for a register file of `regs` registers (>= 4): sources
+// TS[0].i (a) and TS[1].i (b), results TD[0].i, spills to heap slots; PBS use LUT ids
+// lut_base + logical LUT.  Nodes are issued in as-soon-as-possible level order, evicting
+// the register whose next use is farthest; the last PBS of each level carries the flush
+// bit.  Returns false if regs is too small.
+struct EmitStats {
+  uint32_t dops = 0, ld = 0, st = 0, alu = 0, pbs = 0, heap_slots = 0, levels = 0;
+};
+bool emit_dops(const Program& p, uint32_t regs, uint32_t lut_base, std::vector<uint32_t>& words, EmitStats& st);
+
+// IOp opcode of the reference's IOp table (docs/iop.md:176-196) for each program.
+uint8_t iop_opcode(uint32_t op);
+
 }  // namespace iop

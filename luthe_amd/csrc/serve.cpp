@@ -780,6 +780,64 @@ int luthe_iop_program(uint32_t op, uint32_t nblocks, luthe_iop_info* info, luthe
   return LUTHE_OK;
 }
 
+int luthe_iop_program_dops(uint32_t op, uint32_t nblocks, uint32_t regs, uint32_t lut_base, uint32_t* words,
+                           size_t cap, size_t* n, luthe_iop_dop_stats* stats) {
+  if (op >= LUTHE_IOP_COUNT || nblocks < 1 || nblocks > 64 || !n) return LUTHE_ERR_ARG;
+  if (lut_base + LUTHE_IOP_LUTS > 4096) return LUTHE_ERR_RANGE;
+  iop::Program p;
+  if (!iop::build(op, nblocks, p)) return LUTHE_ERR_ARG;
+  std::vector<uint32_t> w;
+  iop::EmitStats es;
+  if (!iop::emit_dops(p, regs, lut_base, w, es)) return LUTHE_ERR_ARG;
+  *n = w.size();
+  if (stats) {
+    memset(stats, 0, sizeof *stats);
+    stats->dops = es.dops, stats->ld = es.ld, stats->st = es.st, stats->alu = es.alu, stats->pbs = es.pbs;
+    stats->heap_slots = es.heap_slots, stats->levels = es.levels;
+    stats->iop_opcode = iop::iop_opcode(op);
+    stats->align_bits = (op == LUTHE_IOP_CMP_GT) ? 2 * nblocks : 2 * nblocks;
+  }
+  if (words) {
+    if (cap < w.size()) return LUTHE_ERR_ARG;
+    memcpy(words, w.data(), w.size() * sizeof(uint32_t));
+  }
+  return LUTHE_OK;
+}
+
+int luthe_write_iop_luts(luthe_ctx* c, uint32_t lut_base) {
+  if (!c) return LUTHE_ERR_ARG;
+  const luthe_params& p = c->prm;
+  if (p.msg_bits != 2 || p.carry_bits != 2)
+    return lx::fail(c, LUTHE_ERR_PARAMS, "radix programs need 2 message + 2 carry bits");
+  if ((uint64_t)lut_base + LUTHE_IOP_LUTS > p.lut_slots) return lx::fail(c, LUTHE_ERR_RANGE, "LUT slots out of range");
+  std::vector<uint64_t> glwe(c->lut_len);
+  for (uint32_t l = 0; l < iop::LUT_COUNT; l++) {
+    std::vector<uint8_t> t;
+    uint32_t lognb;
+    iop::lut_tables(l, t, lognb);
+    int rc = lognb ? luthe_lut_encode_many(&p, t.data(), 16u >> lognb, lognb, glwe.data())
+                   : luthe_lut_encode(&p, t.data(), 16, glwe.data());
+    if (rc) return lx::fail(c, rc, "LUT encoding");
+    rc = luthe_write_lut(c, lut_base + l, glwe.data(), 1);
+    if (rc) return rc;
+  }
+  return LUTHE_OK;
+}
+
+int luthe_register_iop_program(luthe_ctx* c, uint32_t op, uint32_t nblocks, uint32_t lut_base) {
+  if (!c) return LUTHE_ERR_ARG;
+  size_t n = 0;
+  luthe_iop_dop_stats st;
+  int rc = luthe_iop_program_dops(op, nblocks, c->prm.regfile_size, lut_base, nullptr, 0, &n, &st);
+  if (rc) return lx::fail(c, rc, "IOp %u on %u blocks: no program for %u registers", op, nblocks, c->prm.regfile_size);
+  std::vector<uint32_t> w(n);
+  rc = luthe_iop_program_dops(op, nblocks, c->prm.regfile_size, lut_base, w.data(), w.size(), &n, &st);
+  if (rc) return rc;
+  if (st.heap_slots > c->prm.ct_slots / 2)
+    return lx::fail(c, LUTHE_ERR_RANGE, "program needs %u heap slots", st.heap_slots);
+  return luthe_register_iop(c, st.iop_opcode, (uint8_t)st.align_bits, w.data(), w.size());
+}
+
 int luthe_iop_lut(uint32_t lut, uint8_t* table16, uint32_t* log_lut_nb) {
   if (lut >= LUTHE_IOP_LUTS || !table16 || !log_lut_nb) return LUTHE_ERR_ARG;
   std::vector<uint8_t> t;

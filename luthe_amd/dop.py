@@ -51,15 +51,18 @@ def iop_header(opcode, dst_align, src_align, has_imm=False, fw_mode=0):
         ((1 if has_imm else 0) << 24) | ((fw_mode & 1) << 25)
 
 
-def iop_operand(base_cid, nblocks, kind, is_last, block=0):
-    return (base_cid & 0xFFFF) | ((block & 0xFF) << 16) | (((nblocks - 1) & 31) << 24) | \
+def iop_operand(base_cid, nblocks, kind, is_last, vec=1):
+    """hpu_iop_fmt.h:38-45: base_cid, block = blocks per integer, vec_size = integers - 1."""
+    return (base_cid & 0xFFFF) | ((nblocks & 0xFF) << 16) | (((vec - 1) & 31) << 24) | \
         ((1 if is_last else 0) << 29) | ((kind & 3) << 30)
 
 
-def iop_words(opcode, dsts, srcs, imms=(), nblocks=None):
-    """dsts/srcs: lists of (base_cid, nblocks); imms: list of (value, nblocks)."""
-    align = nblocks if nblocks is not None else max([n for _, n in dsts] + [n for _, n in srcs])
-    w = [iop_header(opcode, max(n for _, n in dsts), max(n for _, n in srcs), bool(imms))]
+def iop_words(opcode, dsts, srcs, imms=(), nblocks=None, align=None):
+    """dsts/srcs: lists of (base_cid, nblocks); imms: list of (value, nblocks).  The header's
+    alignment fields are the operands' block counts unless `align` = (dst_align, src_align)
+    is given (e.g. bit widths, as the programs of luthe_register_iop_program use)."""
+    dal, sal = align if align is not None else (max(n for _, n in dsts), max(n for _, n in srcs))
+    w = [iop_header(opcode, dal, sal, bool(imms))]
     for i, (cid, n) in enumerate(dsts):
         w.append(iop_operand(cid, n, KIND_DST, i == len(dsts) - 1))
     for i, (cid, n) in enumerate(srcs):

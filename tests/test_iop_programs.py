@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 import luthe_amd as la
+from luthe_amd import dop
 
 rng = np.random.default_rng(2024)
 OPS = [la.IOP_ADD, la.IOP_CMP_GT, la.IOP_MUL, la.IOP_ADD_RIPPLE]
@@ -127,3 +128,82 @@ def test_program_argument_errors():
     for op, nb in [(4, 8), (0, 0), (0, 65)]:
         with pytest.raises(la.LutheError):
             la.iop_program(op, nb)
+
+
+# ---------------------------------------------------------------- DOp templates
+def _run_template(words, nb, a_blocks, b_blocks, regs, lut_base, op):
+    """Plaintext interpreter of a DOp template (docs/dop.md semantics, PBS = table lookup)."""
+    tables = {}
+    R = [None] * regs
+    mem = {("S", 0, i): int(a_blocks[i]) for i in range(nb)}
+    mem.update({("S", 1, i): int(b_blocks[i]) for i in range(nb)})
+    out = {}
+    for w in (int(x) for x in words):
+        opc = w >> 26
+        if opc in (0x01, 0x02, 0x05):
+            d, s0, s1, mf = w & 127, (w >> 7) & 127, (w >> 14) & 127, (w >> 21) & 31
+            assert d < regs and R[s0] is not None and R[s1] is not None
+            v = R[s0] * mf + R[s1] if opc == 0x05 else (R[s0] + R[s1] if opc == 0x01 else R[s0] - R[s1])
+            assert 0 <= v <= 15, "padding bit reached"
+            R[d] = v
+        elif opc in (0x20, 0x21):
+            rid, mode, slot = w & 127, (w >> 8) & 3, (w >> 10) & 0xFFFF
+            key = {1: ("H", slot), 2: ("S", slot >> 8, slot & 255), 3: ("D", slot >> 8, slot & 255)}[mode]
+            if opc == 0x20:
+                assert key in mem, key
+                R[rid] = mem[key]
+            else:
+                assert R[rid] is not None
+                mem[key] = R[rid]
+                if mode == 3:
+                    out[slot & 255] = R[rid]
+        elif (opc & 0x34) == 0x30:
+            d, s, gid, lognb = w & 127, (w >> 7) & 127, (w >> 14) & 4095, opc & 3
+            lut = gid - lut_base
+            if lut not in tables:
+                tables[lut] = la.iop_lut(lut)
+            t, tl = tables[lut]
+            assert tl == lognb and d % (1 << lognb) == 0 and d + (1 << lognb) <= regs
+            tlen = 16 >> lognb
+            assert R[s] is not None and R[s] < tlen
+            vals = [int(t[j * tlen + R[s]]) for j in range(1 << lognb)]
+            for j, v in enumerate(vals):
+                R[d + j] = v
+        else:
+            raise AssertionError(f"unexpected DOp {w:#010x}")
+    nout = 1 if op == la.IOP_CMP_GT else nb
+    assert sorted(out) == list(range(nout))
+    return np.array([out[i] for i in range(nout)], np.uint32)
+
+
+@pytest.mark.parametrize("op", OPS)
+@pytest.mark.parametrize("nb,regs", [(1, 4), (2, 64), (4, 6), (8, 64), (8, 8), (16, 64), (16, 12), (32, 64)])
+def test_dop_templates_compute_the_iop(op, nb, regs):
+    lut_base = 100
+    words, st = la.iop_program_dops(op, nb, regs, lut_base)
+    info = la.iop_program(op, nb)[0]
+    assert st["pbs"] == info.npbs and st["alu"] == info.nalu
+    assert st["iop_opcode"] == {la.IOP_ADD: 0xE0, la.IOP_CMP_GT: 0xC0, la.IOP_MUL: 0xE4,
+                                la.IOP_ADD_RIPPLE: 0x60}[op]
+    top = 1 << (2 * nb)
+    cases = [(0, 0), (top - 1, top - 1), (top - 1, 1)]
+    cases += [(int(rng.integers(0, 2**62)) % top, int(rng.integers(0, 2**62)) % top) for _ in range(6)]
+    for a, b in cases:
+        got = _run_template(words, nb, la.to_blocks(a, nb), la.to_blocks(b, nb), regs, lut_base, op)
+        assert np.array_equal(got, _expected(op, a, b, nb)), (a, b)
+    # the template patches (hpu_dop_fmt.h semantics) and validates as a concrete stream
+    ct_slots = 4096
+    iop = dop.iop_words(st["iop_opcode"], [(2000, nb)], [(0, nb), (64, nb)],
+                           align=(st["align_bits"], st["align_bits"]))
+    concrete = la.iop_translate(iop, words, ct_slots)
+    nlev, levels = la.dop_schedule(concrete, regfile_size=max(regs, 4), ct_slots=ct_slots, lut_slots=4096)
+    assert nlev >= info.depth
+    heap = [(int(w) >> 10) & 0xFFFF for w, t in zip(concrete, words)
+            if (int(t) >> 26) in (0x20, 0x21) and ((int(t) >> 8) & 3) == 1]
+    assert (st["heap_slots"] == 0) == (not heap)
+    assert all(s >= ct_slots - st["heap_slots"] for s in heap)
+
+
+def test_dop_templates_reject_tiny_register_files():
+    with pytest.raises(la.LutheError):
+        la.iop_program_dops(la.IOP_ADD, 8, 3)
