@@ -238,6 +238,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=5.0, help="arrival window per rate (config 4)")
     ap.add_argument("--serve-cohort-max", type=int, default=0, help="PBS per cohort (config 4; 0 = server default)")
     ap.add_argument("--extras", default="c1,c3,c4", help="extra configurations to run beside the headline")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (library defaults)")
+    ap.add_argument("--tuning-json", default="", help="where the autotuner writes its choices")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="only run a device-to-device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -270,6 +272,11 @@ def main():
     ctx.upload_ksk(keys.ksk)
     lut = la.lut_encode(prm, np.arange(16, dtype=np.uint8))
     ctx.write_lut(0, lut)
+    tuning = None
+    if not args.no_tune and not args.br_grid:
+        t0 = time.perf_counter()
+        tuning = ctx.autotune(args.batch, args.tuning_json or None)
+        log(f"bench: autotuned in {time.perf_counter() - t0:.1f} s: {tuning}")
     rng = np.random.default_rng(99)
     msgs = rng.integers(0, 16, args.batch)
     cts = keys.encrypt_msg(msgs, seed=4321)
@@ -342,7 +349,8 @@ def main():
                                f"ciphertexts, n=724 k=1 N=2048 PBS 2^23x1 KS 2^3x5 w_ks=21",
                    "batch": args.batch, "lwe_dim": prm.lwe_dim, "poly_size": prm.poly_size,
                    "pbs_base_log": prm.pbs_base_log, "pbs_level": prm.pbs_level,
-                   "ks_base_log": prm.ks_base_log, "ks_level": prm.ks_level, "br_workgroups": ctx.br_grid()},
+                   "ks_base_log": prm.ks_base_log, "ks_level": prm.ks_level, "br_workgroups": ctx.br_grid(),
+                   "tuning": tuning if tuning is not None else ctx.tuning()},
         "plain_pbs_per_s": round(plain, 2) if plain else None,
         "batched_over_plain": round(value / plain, 1) if plain else None,
         "decrypt_success": decrypt_ok,

@@ -234,6 +234,25 @@ int luthe_get_br_grid(luthe_ctx* ctx);
  * reference counterpart (the HPU's PBS latency is fixed by its pipeline). */
 int luthe_set_br_latency_max(luthe_ctx* ctx, int max_cohort);
 int luthe_get_br_latency_max(luthe_ctx* ctx);
+/* Tuning knobs and the autotuner.  luthe_autotune times, with the context's keys and
+ * synthetic inputs, (1) the persistent blind-rotation width (ciphertexts resident per CU,
+ * batches of `batch`), (2) the largest cohort for which the latency variant is faster,
+ * (3) polynomials per NTT workgroup (1, 2, 4) and sets (4) the serving cohort size to one
+ * wave of the chosen width; applies them and writes them to json_path (optional).
+ * luthe_load_tuning applies a saved file (refused if tuned for another PBS level).
+ * No reference counterpart: the HPU's equivalents are build-time parameters
+ * (docs/hpu_parameters.md). */
+typedef struct luthe_tuning {
+  int32_t br_grid;          /* persistent blind-rotation workgroups, 0 = occupancy x CUs */
+  int32_t br_latency_max;   /* largest cohort for the latency variant, -1 = its capacity */
+  int32_t ntt_polys_per_wg; /* 1, 2 or 4 */
+  int32_t cohort_max;       /* serving cohort size, 0 = blind-rotation width */
+} luthe_tuning;
+int luthe_autotune(luthe_ctx* ctx, uint32_t batch, const char* json_path, luthe_tuning* out);
+int luthe_load_tuning(luthe_ctx* ctx, const char* json_path);
+int luthe_get_tuning(luthe_ctx* ctx, luthe_tuning* out);
+int luthe_set_tuning(luthe_ctx* ctx, const luthe_tuning* t);
+
 /* Diagnostics: throughput of the Goldilocks modular multiply (64x64 -> 128-bit product
  * plus reduction) at full occupancy, in 10^9 modmul/s; the VALU roofline denominator of
  * the blind rotation.  No reference counterpart (the HPU's rate is fixed by its RTL). */

@@ -95,6 +95,13 @@ class TraceEvent(C.Structure):
 TRC_NONE, TRC_RDUNLOCK, TRC_RETIRE, TRC_REFILL, TRC_ISSUE = 0, 1, 2, 3, 4
 
 
+class Tuning(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("br_grid", "br_latency_max", "ntt_polys_per_wg", "cohort_max")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 class IopDopStats(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in ("dops", "ld", "st", "alu", "pbs", "heap_slots", "levels", "iop_opcode",
                                           "align_bits")]
@@ -166,6 +173,10 @@ def lib():
         "luthe_set_br_grid": (i32, [vp, i32]),
         "luthe_get_br_grid": (i32, [vp]),
         "luthe_set_br_latency_max": (i32, [vp, i32]),
+        "luthe_autotune": (i32, [vp, u32, C.c_char_p, C.POINTER(Tuning)]),
+        "luthe_load_tuning": (i32, [vp, C.c_char_p]),
+        "luthe_get_tuning": (i32, [vp, C.POINTER(Tuning)]),
+        "luthe_set_tuning": (i32, [vp, C.POINTER(Tuning)]),
         "luthe_get_br_latency_max": (i32, [vp]),
         "luthe_microbench_modmul": (i32, [vp, C.POINTER(C.c_double)]),
         "luthe_dop_schedule": (i32, [vp, sz, u32, u32, u32, vp]),
@@ -447,6 +458,27 @@ class Context:
         return v.value
 
     # ---- batched device-memory entry points
+    def autotune(self, batch, json_path=None):
+        t = Tuning()
+        path = json_path.encode() if json_path else None
+        self._chk(lib().luthe_autotune(self._h, batch, path, C.byref(t)), "autotune")
+        return t.as_dict()
+
+    def load_tuning(self, json_path):
+        self._chk(lib().luthe_load_tuning(self._h, json_path.encode()), "load_tuning")
+
+    def tuning(self):
+        t = Tuning()
+        self._chk(lib().luthe_get_tuning(self._h, C.byref(t)), "get_tuning")
+        return t.as_dict()
+
+    def set_tuning(self, **kw):
+        t = Tuning()
+        self._chk(lib().luthe_get_tuning(self._h, C.byref(t)), "get_tuning")
+        for k, v in kw.items():
+            setattr(t, k, v)
+        self._chk(lib().luthe_set_tuning(self._h, C.byref(t)), "set_tuning")
+
     def write_iop_luts(self, lut_base):
         self._chk(lib().luthe_write_iop_luts(self._h, lut_base), "write_iop_luts")
 

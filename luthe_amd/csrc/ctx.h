@@ -52,6 +52,8 @@ struct luthe_ctx {
   std::map<uint32_t, std::vector<uint32_t>> iop_table;  // key = max_align << 8 | opcode
   int br_grid_override = 0;
   int br_latency_max = -1;  // cohorts up to this size use the latency kernel; -1 = its resident capacity
+  int ntt_polys_per_wg = 2;  // polynomials per NTT workgroup (1, 2, 4)
+  int cohort_max = 0;        // serving cohort size chosen by the autotuner (0 = blind-rotation width)
   luthe_server* server = nullptr;  // attached serving engine (owns the stream while set)
   // DOp-path trace (isc_trace_t events), a ring of trace_cap events
   size_t trace_cap = 0;

@@ -42,8 +42,9 @@ struct BrArgs {
   uint32_t base_log;  // PBS_B_W
 };
 
+// polys_per_wg: polynomials per 256-thread workgroup (1, 2 or 4; tuned).
 hipError_t launch_ntt(int mode, const uint64_t* a, const uint64_t* bn, uint64_t* out, size_t npoly,
-                      const uint64_t* tab, hipStream_t s);
+                      const uint64_t* tab, hipStream_t s, int polys_per_wg = 2);
 hipError_t launch_bsk_prepare(const uint64_t* src, uint64_t* dst, int n_lwe, int rows, int K1, int is_ntt,
                               const uint64_t* tab, hipStream_t s);
 hipError_t launch_keyswitch(const uint64_t* base, const PbsJob* jobs, int nct, const uint32_t* ksk, uint16_t* lwe2n,

@@ -519,10 +519,9 @@ __global__ __launch_bounds__(256) void k_modmul_peak(uint64_t* out, uint64_t see
 // ------------------------------------------------------------------------------------
 // Host launchers
 // ------------------------------------------------------------------------------------
-hipError_t launch_ntt(int mode, const uint64_t* a, const uint64_t* bn, uint64_t* out, size_t npoly,
-                      const uint64_t* tab, hipStream_t s) {
-  constexpr int NP = 2;
-  if (npoly == 0) return hipSuccess;
+template <int NP>
+static void ntt_launch(int mode, const uint64_t* a, const uint64_t* bn, uint64_t* out, size_t npoly,
+                       const uint64_t* tab, hipStream_t s) {
   dim3 grid((unsigned)((npoly + NP - 1) / NP));
   if (mode == 0)
     k_ntt<NP, 0><<<grid, 256, 0, s>>>(a, bn, out, npoly, tab);
@@ -530,6 +529,16 @@ hipError_t launch_ntt(int mode, const uint64_t* a, const uint64_t* bn, uint64_t*
     k_ntt<NP, 1><<<grid, 256, 0, s>>>(a, bn, out, npoly, tab);
   else
     k_ntt<NP, 2><<<grid, 256, 0, s>>>(a, bn, out, npoly, tab);
+}
+
+hipError_t launch_ntt(int mode, const uint64_t* a, const uint64_t* bn, uint64_t* out, size_t npoly,
+                      const uint64_t* tab, hipStream_t s, int polys_per_wg) {
+  if (npoly == 0) return hipSuccess;
+  switch (polys_per_wg) {
+    case 1: ntt_launch<1>(mode, a, bn, out, npoly, tab, s); break;
+    case 4: ntt_launch<4>(mode, a, bn, out, npoly, tab, s); break;
+    default: ntt_launch<2>(mode, a, bn, out, npoly, tab, s); break;
+  }
   return hipGetLastError();
 }
 

@@ -522,17 +522,17 @@ int luthe_graph_destroy(luthe_graph* g) {
 int luthe_ntt_polymul_batch(luthe_ctx* c, const uint64_t* d_a, const uint64_t* d_bn, uint64_t* d_out, size_t npoly,
                             void* stream) {
   if (!c || (npoly && (!d_a || !d_bn || !d_out))) return LUTHE_ERR_ARG;
-  HIPCHK(c, lk::launch_ntt(2, d_a, d_bn, d_out, npoly, c->d_tab, pick_stream(c, stream)));
+  HIPCHK(c, lk::launch_ntt(2, d_a, d_bn, d_out, npoly, c->d_tab, pick_stream(c, stream), c->ntt_polys_per_wg));
   return LUTHE_OK;
 }
 int luthe_ntt_forward_batch(luthe_ctx* c, const uint64_t* d_in, uint64_t* d_out, size_t npoly, void* stream) {
   if (!c || (npoly && (!d_in || !d_out))) return LUTHE_ERR_ARG;
-  HIPCHK(c, lk::launch_ntt(0, d_in, nullptr, d_out, npoly, c->d_tab, pick_stream(c, stream)));
+  HIPCHK(c, lk::launch_ntt(0, d_in, nullptr, d_out, npoly, c->d_tab, pick_stream(c, stream), c->ntt_polys_per_wg));
   return LUTHE_OK;
 }
 int luthe_ntt_inverse_batch(luthe_ctx* c, const uint64_t* d_in, uint64_t* d_out, size_t npoly, void* stream) {
   if (!c || (npoly && (!d_in || !d_out))) return LUTHE_ERR_ARG;
-  HIPCHK(c, lk::launch_ntt(1, d_in, nullptr, d_out, npoly, c->d_tab, pick_stream(c, stream)));
+  HIPCHK(c, lk::launch_ntt(1, d_in, nullptr, d_out, npoly, c->d_tab, pick_stream(c, stream), c->ntt_polys_per_wg));
   return LUTHE_OK;
 }
 
