@@ -35,8 +35,9 @@ GL_HD uint64_t add(uint64_t a, uint64_t b) {
 
 // a - b (mod p) for canonical a, b; canonical result (a - b + p when a < b).
 GL_HD uint64_t sub(uint64_t a, uint64_t b) {
-  uint64_t d = a - b;
-  return (a < b) ? d + P : d;
+  uint64_t d;
+  const bool borrow = __builtin_sub_overflow(a, b, &d);  // the subtraction's own borrow
+  return borrow ? d + P : d;
 }
 
 GL_HD uint64_t neg(uint64_t a) { return sub(0, a); }
@@ -49,6 +50,18 @@ GL_HD uint64_t reduce128(uint64_t lo, uint64_t hi) {
   uint64_t t1 = (hl << 32) - hl;                        // hl * EPS < p
   uint64_t r;
   if (__builtin_add_overflow(t0, t1, &r)) r += EPS;     // carried 2^64 == EPS; cannot wrap again
+  return canonical(r);
+}
+
+// (lo + hi * 2^64 + c * 2^128) mod p for c < 2^32 - 1, canonical: 2^128 == -2^32, so c
+// joins the 2^96 part of hi as the high word of the subtrahend.
+GL_HD uint64_t reduce160(uint64_t lo, uint64_t hi, uint32_t c) {
+  uint64_t hh = (hi >> 32) | ((uint64_t)c << 32), hl = hi & EPS;
+  uint64_t t0;
+  if (__builtin_sub_overflow(lo, hh, &t0)) t0 -= EPS;  // hh < 2^64 - 2^32: cannot wrap again
+  uint64_t t1 = (hl << 32) - hl;
+  uint64_t r;
+  if (__builtin_add_overflow(t0, t1, &r)) r += EPS;
   return canonical(r);
 }
 

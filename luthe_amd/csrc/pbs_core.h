@@ -81,13 +81,26 @@ GL_HD void mac(uint64_t (*z)[8], const uint64_t* bsk_i, int t) {
     uint64_t o[K1];
 #pragma unroll
     for (int j = 0; j < K1; j++) {
-      uint64_t s = 0;
+      // sum of K1*L full 128-bit products, carries past 2^128 counted in c, one reduction
+      uint64_t lo = 0, hi = 0;
+      uint32_t c = 0;
 #pragma unroll
       for (int gl_ = 0; gl_ < K1 * L; gl_++) {
         uint64_t b = bsk_i[((w0 * (K1 * L) + gl_) * K1 + j) * T + t];
-        s = gl::add(s, gl::mul(z[gl_][w0], b));
+        uint64_t plo, phi;
+        gl::mul_full(z[gl_][w0], b, plo, phi);
+        if (gl_ == 0) {
+          lo = plo, hi = phi;
+        } else {
+          uint64_t nlo, t1, nhi;
+          const bool k1 = __builtin_add_overflow(lo, plo, &nlo);
+          const bool o1 = __builtin_add_overflow(hi, phi, &t1);
+          const bool o2 = __builtin_add_overflow(t1, (uint64_t)k1, &nhi);
+          lo = nlo, hi = nhi;
+          c += (uint32_t)o1 + (uint32_t)o2;
+        }
       }
-      o[j] = s;
+      o[j] = gl::reduce160(lo, hi, c);
     }
 #pragma unroll
     for (int j = 0; j < K1; j++) z[j][w0] = o[j];

@@ -89,6 +89,7 @@ uint64_t emu_gl_add(uint64_t a, uint64_t b) { return gl::add(a, b); }
 uint64_t emu_gl_sub(uint64_t a, uint64_t b) { return gl::sub(a, b); }
 uint64_t emu_gl_mul_pow2(uint64_t a, uint32_t s) { return gl::mul_pow2(a, s); }
 uint64_t emu_gl_reduce128(uint64_t lo, uint64_t hi) { return gl::reduce128(lo, hi); }
+uint64_t emu_gl_reduce160(uint64_t lo, uint64_t hi, uint32_t c) { return gl::reduce160(lo, hi, c); }
 uint64_t emu_modswitch(uint64_t e) { return gl::modswitch_p_to_q(e); }
 }
 

@@ -29,6 +29,8 @@ def emu():
         getattr(E, f).restype = C.c_uint64
     E.emu_gl_mul_pow2.argtypes = [C.c_uint64, C.c_uint32]
     E.emu_modswitch.argtypes = [C.c_uint64]
+    E.emu_gl_reduce160.restype = C.c_uint64
+    E.emu_gl_reduce160.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32]
     for f in ("emu_gl_add", "emu_gl_sub", "emu_gl_mul", "emu_gl_reduce128"):
         getattr(E, f).argtypes = [C.c_uint64, C.c_uint64]
     return E
@@ -44,6 +46,9 @@ def test_device_arithmetic(emu):
     for a, b in zip(anyv[:1500], anyv[1500:3000]):
         assert emu.emu_gl_mul(a, b) == (a * b) % P
         assert emu.emu_gl_reduce128(a, b) == (a + (b << 64)) % P
+    for i, (a, b) in enumerate(zip(anyv[:1500], anyv[1500:3000])):
+        c = [0, 1, 2, 3, 7, 2**31, 2**32 - 2][i % 7]
+        assert emu.emu_gl_reduce160(a, b, c) == (a + (b << 64) + (c << 128)) % P
     for x in anyv[:400]:
         for s in (0, 1, 3, 12, 24, 31, 32, 33, 48, 63, 64, 65, 72, 84, 93, 95, 96, 120, 144, 168, 191):
             assert emu.emu_gl_mul_pow2(x, s) == (x << s) % P
