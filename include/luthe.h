@@ -157,10 +157,24 @@ int luthe_register_iop(luthe_ctx* ctx, uint8_t opcode, uint8_t max_align, const 
 int luthe_submit_iop(luthe_ctx* ctx, const uint32_t* words, size_t n, uint64_t* ticket);
 
 /* Host-only utilities (no device needed).  Validate a concrete DOp stream and return
- * each DOp's dependency level (-1 for SYNC) in `levels`; result = number of levels or
- * a negative error.  The schedule is the one luthe_submit_dops executes. */
+ * each DOp's dependency level (-1 for SYNC) in `levels` under register and slot hazards
+ * (read-after-write, write-after-read, write-after-write), the order the HPU's
+ * instruction scheduler enforces; result = number of levels or a negative error. */
 int luthe_dop_schedule(const uint32_t* words, size_t n, uint32_t regfile_size, uint32_t ct_slots,
                        uint32_t lut_slots, int32_t* levels);
+/* The plan luthe_submit_dops executes (host only): register writes renamed to fresh rows,
+ * so levels follow true data dependencies and slot hazards only.  Row space: register
+ * r = r, slot s = regfile_size + s, renamed k = regfile_size + ct_slots + k.
+ * entries[i] = {level (-1 for SYNC), dst row, src0 row, src1 row}; copy_back receives
+ * (register, row) pairs applied after the last level (2 * copy_back_cap words).
+ * Returns the number of levels (copy-back excluded) or a negative error. */
+typedef struct luthe_dop_plan_entry {
+  int32_t level;
+  uint32_t dst, src0, src1;
+} luthe_dop_plan_entry;
+int luthe_dop_plan(const uint32_t* words, size_t n, uint32_t regfile_size, uint32_t ct_slots, uint32_t lut_slots,
+                   luthe_dop_plan_entry* entries, uint32_t* copy_back, size_t copy_back_cap, size_t* n_copy_back,
+                   uint32_t* rename_rows);
 /* Translate one IOp against a DOp template program (parse_iop + patch_dop of
  * ucore_fw.c:135-220, hpu_dop_fmt.h:148-229) without executing: out receives n_tpl
  * concrete DOp words.  Returns 0 or a negative error. */

@@ -180,6 +180,7 @@ def lib():
         "luthe_get_br_latency_max": (i32, [vp]),
         "luthe_microbench_modmul": (i32, [vp, C.POINTER(C.c_double)]),
         "luthe_dop_schedule": (i32, [vp, sz, u32, u32, u32, vp]),
+        "luthe_dop_plan": (i32, [vp, sz, u32, u32, u32, vp, vp, sz, C.POINTER(sz), C.POINTER(u32)]),
         "luthe_alu_batch": (i32, [vp, u32, vp, vp, vp, u32, u32, sz, vp]),
         "luthe_reserve": (i32, [vp, sz]),
         "luthe_capture_begin": (i32, [vp]),
@@ -664,6 +665,22 @@ def dop_schedule(words, regfile_size=64, ct_slots=4096, lut_slots=256):
     if rc < 0:
         raise LutheError(rc, "dop_schedule")
     return rc, lv[:w.size]
+
+
+def dop_plan(words, regfile_size=64, ct_slots=4096, lut_slots=256):
+    """The executor's renamed plan (host only): (nlevels, entries[n, 4] = level, dst, src0,
+    src1 rows, copy_back[k, 2] = (register, row), rename_rows)."""
+    w = np.ascontiguousarray(words, np.uint32)
+    raw = np.zeros((max(w.size, 1), 4), np.uint32)
+    ncb, rows = C.c_size_t(), C.c_uint32()
+    cb = np.zeros(2 * regfile_size, np.uint32)
+    rc = lib().luthe_dop_plan(_vp(w), w.size, regfile_size, ct_slots, lut_slots, _vp(raw), _vp(cb), regfile_size,
+                              C.byref(ncb), C.byref(rows))
+    if rc < 0:
+        raise LutheError(rc, "dop_plan")
+    ent = raw[:w.size].astype(np.int64)
+    ent[:, 0] = raw[:w.size, 0].view(np.int32)
+    return rc, ent, cb[:2 * ncb.value].reshape(-1, 2), rows.value
 
 
 def iop_translate(iop_words, template, ct_slots=4096):

@@ -28,7 +28,8 @@ struct luthe_ctx {
   size_t blen = 0;           // k*N + 1
   size_t lut_len = 0;        // (k+1)*N
   // device memories
-  uint64_t* d_mem = nullptr;  // [regfile_size + ct_slots] BLWE rows
+  uint64_t* d_mem = nullptr;  // [regfile_size + ct_slots + rename_cap] BLWE rows
+  size_t rename_cap = 0;      // rows for renamed registers of DOp streams
   uint64_t* d_luts = nullptr;
   uint64_t* d_bsk = nullptr;
   uint32_t* d_ksk = nullptr;
@@ -69,6 +70,7 @@ int fail(luthe_ctx* c, int code, const char* fmt, ...);
 int dev_alloc(luthe_ctx* c, void** p, size_t bytes);
 int ensure_lwe2n(luthe_ctx* c, size_t n);
 int ensure_jobs(luthe_ctx* c, size_t n);
+int ensure_rename_rows(luthe_ctx* c, size_t n);
 // Key switch + blind rotation + sample extract of n jobs whose descriptors are on the
 // device; rows are relative to in_base / out_base.
 int run_pbs(luthe_ctx* c, const uint64_t* in_base, const lk::PbsJob* d_jobs, int n, uint64_t* out_base,

@@ -1,6 +1,8 @@
 // DOp / IOp front end (host only): see dop.h for the word formats.
 #include "dop.h"
 
+#include <unordered_map>
+
 #include <stdio.h>
 
 #include <algorithm>
@@ -139,6 +141,81 @@ int check_template(const uint32_t* words, size_t n, uint32_t regs, uint32_t luts
     }
   }
   return 0;
+}
+
+int schedule_renamed(const std::vector<Dop>& prog, uint32_t regs, uint32_t slot_base, uint32_t rename_base,
+                     std::vector<Renamed>& out, std::vector<std::pair<uint32_t, uint32_t>>& copy_back,
+                     uint32_t& rename_rows) {
+  std::vector<uint32_t> cur(regs);       // row holding each register's current value
+  std::vector<int> ready(regs, 0);       // level from which that row can be read
+  for (uint32_t r = 0; r < regs; r++) cur[r] = r;
+  std::unordered_map<uint32_t, std::pair<int, int>> slots;  // slot -> (last write, last read)
+  out.assign(prog.size(), Renamed());
+  copy_back.clear();
+  uint32_t next = 0;
+  int nlev = 0;
+  auto fresh = [&](uint32_t count) {
+    const uint32_t row = rename_base + next;
+    next += count;
+    return row;
+  };
+  for (size_t i = 0; i < prog.size(); i++) {
+    const Dop& d = prog[i];
+    Renamed& o = out[i];
+    int lv = 0;
+    switch (d.kind) {
+      case KIND_ARITH: {
+        lv = ready[d.src0];
+        o.src0 = cur[d.src0];
+        if (!(d.opcode & IMM_FLAG)) {
+          lv = std::max(lv, ready[d.src1]);
+          o.src1 = cur[d.src1];
+        } else {
+          o.src1 = o.src0;
+        }
+        o.dst = fresh(1);
+        cur[d.dst] = o.dst;
+        ready[d.dst] = lv + 1;
+        break;
+      }
+      case KIND_PBS: {
+        lv = ready[d.src0];
+        o.src0 = cur[d.src0];
+        const uint32_t nb = 1u << d.log_lut_nb;
+        o.dst = fresh(nb);
+        for (uint32_t j = 0; j < nb; j++) {
+          cur[d.dst + j] = o.dst + j;
+          ready[d.dst + j] = lv + 1;
+        }
+        break;
+      }
+      case KIND_MEM: {
+        auto& st = slots.emplace(d.slot, std::make_pair(-1, -1)).first->second;
+        if (d.opcode == OP_LD) {
+          lv = st.first + 1;
+          o.src0 = slot_base + d.slot;
+          o.dst = fresh(1);
+          cur[d.rid] = o.dst;
+          ready[d.rid] = lv + 1;
+          st.second = std::max(st.second, lv);
+        } else {
+          lv = std::max({ready[d.rid], st.first + 1, st.second + 1});
+          o.src0 = cur[d.rid];
+          o.dst = slot_base + d.slot;
+          st.first = lv;
+        }
+        break;
+      }
+      default:
+        continue;  // SYNC
+    }
+    o.level = lv;
+    nlev = std::max(nlev, lv + 1);
+  }
+  for (uint32_t r = 0; r < regs; r++)
+    if (cur[r] != r) copy_back.emplace_back(r, cur[r]);
+  rename_rows = next;
+  return nlev;
 }
 
 int schedule_levels(const std::vector<Dop>& prog, uint32_t regs, std::vector<int>& level) {

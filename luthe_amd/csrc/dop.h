@@ -43,8 +43,23 @@ int decode_stream(const uint32_t* words, size_t n, uint32_t regs, uint32_t slots
                   std::vector<Dop>& out, std::string& why);
 // Validate a template program (memory modes and IMM_VAR allowed).
 int check_template(const uint32_t* words, size_t n, uint32_t regs, uint32_t luts, std::string& why);
-// Dependency level per DOp (SYNC gets -1).  Returns the number of levels.
+// Dependency level per DOp (SYNC gets -1) on registers and slots (read-after-write,
+// write-after-read, write-after-write).  Returns the number of levels.
 int schedule_levels(const std::vector<Dop>& prog, uint32_t regs, std::vector<int>& level);
+
+// Level schedule with register renaming: every register write goes to a fresh row
+// rename_base + k, so only true data dependencies on registers remain (slots keep
+// read-after-write, write-after-read and write-after-write).  Rows: register r = r,
+// slot s = slot_base + s.  copy_back lists (architectural register, final row) pairs
+// that restore the register file after the last level.  Results equal in-order
+// execution.  Returns the number of levels (copy-back excluded).
+struct Renamed {
+  int level = -1;
+  uint32_t dst = 0, src0 = 0, src1 = 0;  // rows
+};
+int schedule_renamed(const std::vector<Dop>& prog, uint32_t regs, uint32_t slot_base, uint32_t rename_base,
+                     std::vector<Renamed>& out, std::vector<std::pair<uint32_t, uint32_t>>& copy_back,
+                     uint32_t& rename_rows);
 
 struct Operand {
   uint32_t cid_ofst = 0, len = 0;  // len: blocks the operand spans

@@ -87,6 +87,24 @@ int ensure_lwe2n(luthe_ctx* c, size_t n) {
   return LUTHE_OK;
 }
 
+// Rows after the ciphertext memory that hold renamed register values of a DOp stream;
+// growing keeps the register file and ciphertext memory.
+int ensure_rename_rows(luthe_ctx* c, size_t n) {
+  if (n <= c->rename_cap) return LUTHE_OK;
+  const size_t keep = (size_t)c->prm.regfile_size + c->prm.ct_slots;
+  const size_t cap = std::max(n, c->rename_cap * 2);
+  uint64_t* nm = nullptr;
+  int rc = dev_alloc(c, (void**)&nm, (keep + cap) * c->blen * 8);
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(nm, c->d_mem, keep * c->blen * 8, hipMemcpyDeviceToDevice));
+  HIPCHK(c, hipFree(c->d_mem));
+  c->dev_bytes -= (keep + c->rename_cap) * c->blen * 8;
+  c->d_mem = nm;
+  c->rename_cap = cap;
+  return LUTHE_OK;
+}
+
 int ensure_jobs(luthe_ctx* c, size_t n) {
   if (n <= c->jobs_cap) return LUTHE_OK;
   size_t cap = std::max(n, c->jobs_cap * 2);
@@ -541,8 +559,18 @@ static int execute_program(luthe_ctx* c, const std::vector<dop::Dop>& prog, uint
   if (c->server) return fail(c, LUTHE_ERR_BUSY, "context stream is owned by an attached server");
   const luthe_params& p = c->prm;
   const uint32_t R = p.regfile_size;
-  std::vector<int> level;
-  int nlev = dop::schedule_levels(prog, R, level);
+  // Register renaming: each register write lands in a fresh row after the ciphertext
+  // memory, so only true data dependencies order the levels; a final level copies the
+  // registers' last values back (results equal in-order execution).
+  std::vector<dop::Renamed> rn;
+  std::vector<std::pair<uint32_t, uint32_t>> copy_back;
+  uint32_t rename_rows = 0;
+  const int nlev_ops = dop::schedule_renamed(prog, R, R, R + p.ct_slots, rn, copy_back, rename_rows);
+  const int nlev = nlev_ops + (copy_back.empty() ? 0 : 1);
+  std::vector<int> level(prog.size());
+  for (size_t i = 0; i < prog.size(); i++) level[i] = rn[i].level;
+  int rc = ensure_rename_rows(c, rename_rows);
+  if (rc) return rc;
   // descriptors per level, packed into one staging blob
   struct LevelPlan {
     size_t copy_off = 0, alu_off = 0, pbs_off = 0;
@@ -554,25 +582,24 @@ static int execute_program(luthe_ctx* c, const std::vector<dop::Dop>& prog, uint
   std::vector<std::vector<PbsJob>> pbss(nlev);
   for (size_t i = 0; i < prog.size(); i++) {
     const dop::Dop& d = prog[i];
-    int L = level[i];
+    const dop::Renamed& o = rn[i];
+    int L = o.level;
     if (L < 0) continue;  // SYNC
     switch (d.kind) {
       case dop::KIND_MEM:
-        if (d.opcode == dop::OP_LD)
-          copies[L].push_back(CopyOp{d.rid, R + d.slot});
-        else
-          copies[L].push_back(CopyOp{R + d.slot, d.rid});
+        copies[L].push_back(CopyOp{o.dst, o.src0});
         break;
       case dop::KIND_ARITH:
-        alus[L].push_back(AluOp{d.opcode, d.dst, d.src0, d.src1, d.mul_factor, d.msg_cst});
+        alus[L].push_back(AluOp{d.opcode, o.dst, o.src0, o.src1, d.mul_factor, d.msg_cst});
         break;
       case dop::KIND_PBS:
-        pbss[L].push_back(PbsJob{d.src0, d.dst, d.gid, d.log_lut_nb});
+        pbss[L].push_back(PbsJob{o.src0, o.dst, d.gid, d.log_lut_nb});
         break;
       default:
         break;
     }
   }
+  for (const auto& cb : copy_back) copies[nlev - 1].push_back(CopyOp{cb.first, cb.second});
   std::vector<uint8_t> blob;
   auto append = [&](const void* src, size_t bytes) {
     size_t off = (blob.size() + 15) & ~(size_t)15;
@@ -592,7 +619,7 @@ static int execute_program(luthe_ctx* c, const std::vector<dop::Dop>& prog, uint
   }
   if (maxp && (!c->bsk_ok || !c->ksk_ok)) return fail(c, LUTHE_ERR_STATE, "keys not uploaded");
   HIPCHK(c, hipSetDevice(c->device));
-  int rc = stage_upload(c, blob);
+  rc = stage_upload(c, blob);
   if (rc) return rc;
   const int blen = (int)c->blen;
   // trace: REFILL of every DOp now, ISSUE per level as it is launched, RETIRE at completion
@@ -730,6 +757,29 @@ int luthe_dop_schedule(const uint32_t* words, size_t n, uint32_t regs, uint32_t 
   std::vector<int> lv;
   int nlev = dop::schedule_levels(prog, regs, lv);
   for (size_t i = 0; i < n; i++) levels[i] = lv[i];
+  return nlev;
+}
+
+int luthe_dop_plan(const uint32_t* words, size_t n, uint32_t regs, uint32_t slots, uint32_t luts,
+                   luthe_dop_plan_entry* entries, uint32_t* copy_back, size_t copy_back_cap, size_t* n_copy_back,
+                   uint32_t* rename_rows) {
+  if ((!words && n) || (!entries && n) || !n_copy_back || !rename_rows) return LUTHE_ERR_ARG;
+  if (regs < 1 || regs > 128 || slots > 65536 || luts < 1 || luts > 4096) return LUTHE_ERR_ARG;
+  std::vector<dop::Dop> prog;
+  std::string why;
+  int rc = dop::decode_stream(words, n, regs, slots, luts, prog, why);
+  if (rc) return rc;
+  std::vector<dop::Renamed> rn;
+  std::vector<std::pair<uint32_t, uint32_t>> cb;
+  uint32_t rows = 0;
+  int nlev = dop::schedule_renamed(prog, regs, regs, regs + slots, rn, cb, rows);
+  for (size_t i = 0; i < n; i++) entries[i] = luthe_dop_plan_entry{rn[i].level, rn[i].dst, rn[i].src0, rn[i].src1};
+  *n_copy_back = cb.size();
+  *rename_rows = rows;
+  if (copy_back) {
+    if (copy_back_cap < cb.size()) return LUTHE_ERR_ARG;
+    for (size_t i = 0; i < cb.size(); i++) copy_back[2 * i] = cb[i].first, copy_back[2 * i + 1] = cb[i].second;
+  }
   return nlev;
 }
 

@@ -205,3 +205,99 @@ def test_iop_translation_rejects_out_of_range_templates():
         la.iop_translate(words, [dop.ld(0, dop.tslot(0, 9), dop.MEM_SRC)])   # block beyond width
     with pytest.raises(la.LutheError):
         la.iop_translate(words[:-1], [dop.sync()])                           # truncated IOp
+
+
+def _exec_in_order(words, regs, slots_init):
+    """Reference semantics of a concrete DOp stream on plaintext integers (ALU mod 2^16,
+    PBS = x -> (5x + gid) mod 2^16 per output j, + j)."""
+    R = {r: 1000 + r for r in range(regs)}
+    S = dict(slots_init)
+    for w in words:
+        op = w >> 26
+        if op in (0x01, 0x02, 0x05):
+            d, a, b, mf = w & 127, (w >> 7) & 127, (w >> 14) & 127, (w >> 21) & 31
+            R[d] = (R[a] + R[b]) % 65536 if op == 1 else ((R[a] - R[b]) % 65536 if op == 2 else (R[a] * mf + R[b]) % 65536)
+        elif op in (0x09, 0x0A):
+            d, a, cst = w & 127, (w >> 7) & 127, (w >> 15) & 2047
+            R[d] = (R[a] + cst) % 65536 if op == 0x09 else (R[a] - cst) % 65536
+        elif op == 0x20:
+            R[w & 127] = S[(w >> 10) & 0xFFFF]
+        elif op == 0x21:
+            S[(w >> 10) & 0xFFFF] = R[w & 127]
+        elif (op & 0x34) == 0x30:
+            d, s, gid, nb = w & 127, (w >> 7) & 127, (w >> 14) & 4095, 1 << (op & 3)
+            x = R[s]
+            for j in range(nb):
+                R[d + j] = (5 * x + gid + j) % 65536
+    return R, S
+
+
+def _exec_plan(words, regs, nslots, slots_init):
+    """Executes the renamed plan level by level the way the GPU does (copies, then ALU,
+    then PBS of a level; then the copy-back)."""
+    nlev, ent, cb, rows = la.dop_plan(words, regs, nslots, 256)
+    mem = {r: 1000 + r for r in range(regs)}
+    for s, v in slots_init.items():
+        mem[regs + s] = v
+    for L in range(nlev):
+        idx = [i for i in range(len(words)) if ent[i, 0] == L]
+        for kind in ("mem", "alu", "pbs"):
+            upd, reads = {}, set()
+            for i in idx:
+                w, (_, d, a, b) = words[i], ent[i]
+                op = w >> 26
+                if kind == "mem" and op in (0x20, 0x21):
+                    upd[d] = mem[a]
+                    reads.add(int(a))
+                elif kind == "alu" and op in (0x01, 0x02, 0x05, 0x09, 0x0A):
+                    mf, cst = (w >> 21) & 31, (w >> 15) & 2047
+                    upd[d] = {0x01: mem[a] + mem[b], 0x02: mem[a] - mem[b], 0x05: mem[a] * mf + mem[b],
+                              0x09: mem[a] + cst, 0x0A: mem[a] - cst}[op] % 65536
+                    reads.update((int(a), int(b)) if op in (0x01, 0x02, 0x05) else (int(a),))
+                elif kind == "pbs" and (op & 0x34) == 0x30:
+                    for j in range(1 << (op & 3)):
+                        upd[d + j] = (5 * mem[a] + ((w >> 14) & 4095) + j) % 65536
+                    reads.add(int(a))
+            # one launch never reads a row another op of the same launch writes
+            assert not (reads & set(upd)), (L, kind)
+            mem.update(upd)
+    for r, row in cb:
+        mem[int(r)] = mem[int(row)]
+    return nlev, {r: mem[r] for r in range(regs)}, {s: mem[regs + s] for s in range(nslots) if regs + s in mem}
+
+
+def test_renamed_plan_equals_in_order_execution():
+    """Random DOp streams over few registers and slots (dense hazards): the executor's
+    renamed level plan gives the same registers and slots as in-order execution, and its
+    level count never exceeds the hazard-ordered schedule's."""
+    r = np.random.default_rng(31337)
+    for trial in range(300):
+        regs, nslots = 8, 6
+        words = []
+        for _ in range(int(r.integers(5, 60))):
+            k = int(r.integers(0, 6))
+            if k == 0:
+                words.append(dop.arith(int(r.choice([1, 2, 5])), int(r.integers(0, regs)), int(r.integers(0, regs)),
+                                       int(r.integers(0, regs)), int(r.integers(0, 32))))
+            elif k == 1:
+                words.append(dop.arith_msg(int(r.choice([9, 10])), int(r.integers(0, regs)), int(r.integers(0, regs)),
+                                           int(r.integers(0, 2048))))
+            elif k == 2:
+                words.append(dop.ld(int(r.integers(0, regs)), int(r.integers(0, nslots))))
+            elif k == 3:
+                words.append(dop.st(int(r.integers(0, nslots)), int(r.integers(0, regs))))
+            elif k == 4:
+                lognb = int(r.integers(0, 3))
+                nb = 1 << lognb
+                words.append(dop.pbs(int(r.integers(0, regs // nb)) * nb, int(r.integers(0, regs)),
+                                     int(r.integers(0, 256)), lognb, flush=bool(r.integers(0, 2))))
+            else:
+                words.append(dop.sync())
+        words = [int(w) & 0xFFFFFFFF for w in words]
+        init = {s: 7 * s + 3 for s in range(nslots)}
+        R0, S0 = _exec_in_order(words, regs, init)
+        nlev, R1, S1 = _exec_plan(words, regs, nslots, init)
+        assert R1 == R0, trial
+        assert S1 == S0, trial
+        nlev_hazard, _ = la.dop_schedule(words, regfile_size=regs, ct_slots=nslots, lut_slots=256)
+        assert nlev <= nlev_hazard, trial

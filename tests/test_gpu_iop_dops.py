@@ -31,8 +31,9 @@ def test_registered_programs_match_server():
     ctx.write_iop_luts(lut_base)
     rng = np.random.default_rng(19)
     jobs = []
-    for op in (la.IOP_ADD, la.IOP_CMP_GT, la.IOP_MUL, la.IOP_ADD_RIPPLE):
-        for nb in (4, 8):
+    for op, nb in [(o, n) for o in (la.IOP_ADD, la.IOP_CMP_GT, la.IOP_MUL, la.IOP_ADD_RIPPLE) for n in (4, 8)] + [
+            (la.IOP_MUL, 16)]:  # MUL on 16 blocks spills to 192 heap slots
+        if True:
             a, b = (int(x) for x in rng.integers(0, 1 << (2 * nb), 2))
             jobs.append((op, nb, a, b, keys.encrypt_msg(la.to_blocks(a, nb), seed=10 * op + nb),
                          keys.encrypt_msg(la.to_blocks(b, nb), seed=100 + 10 * op + nb)))
