@@ -1,17 +1,19 @@
 // C ABI of the MI355X PBS engine: context, device memories, key upload, batched hot
 // path and the DOp/IOp front end.
 //
-// DOp execution model.  A submitted stream is validated whole, then scheduled by
-// dependency level: each DOp's level is one more than the latest level of any earlier
-// DOp it conflicts with (read-after-write, write-after-read, write-after-write on a
-// register or a ciphertext slot).  DOps of one level are independent, so a level runs
-// as at most three launches: all its LD/ST (one copy kernel), all its ALU DOps (one ALU
-// kernel) and all its PBS (one key-switch + one blind-rotation launch).  Each PBS is a
-// pure function of its source register and LUT and every ciphertext runs its CMUX
-// chain in the fixed order i = 0..n-1, so the results equal in-order execution and do
-// not depend on which other DOps share a batch.  This replaces the HPU instruction
-// scheduler's issue logic (hw/module/instruction_scheduler) and the BPIP batch former
-// (pep_sequencer.sv:584-608) with a host-side level schedule.
+// DOp execution model.  A submitted stream is validated whole, then every register
+// write is renamed to a fresh row after the ciphertext memory and the stream is
+// scheduled by dependency level: a DOp's level is one more than the latest level of the
+// DOps producing its operands (and, for LD/ST, of earlier conflicting accesses to the
+// same slot).  DOps of one level are independent, so a level runs as at most three
+// launches: all its LD/ST (one copy kernel), all its ALU DOps (one ALU kernel) and all
+// its PBS (one key-switch + one blind-rotation launch); a last copy launch writes the
+// registers' final values back.  Each PBS is a pure function of its source row and LUT
+// and every ciphertext runs its CMUX chain in the fixed order i = 0..n-1, so the results
+// equal in-order execution and do not depend on which other DOps share a batch.  This
+// replaces the HPU instruction scheduler's issue logic (hw/module/instruction_scheduler),
+// which orders register reuse with locks, and the BPIP batch former
+// (pep_sequencer.sv:584-608) with a host-side renamed level schedule.
 #include <hip/hip_runtime.h>
 #include <stdarg.h>
 #include <stdio.h>
