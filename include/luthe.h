@@ -56,9 +56,9 @@ typedef struct luthe_params {
   uint32_t poly_size;      /* N, must be 2048 */
   uint32_t pbs_base_log;   /* PBS_B_W */
   uint32_t pbs_level;      /* PBS_L in {1, 2} */
-  uint32_t ks_base_log;    /* KS_B_W in [1, 8] */
+  uint32_t ks_base_log;    /* KS_B_W in [1, 7] (the reference's parameter sets use 2 and 3) */
   uint32_t ks_level;       /* KS_L in [1, 8] */
-  uint32_t ks_mod_log;     /* MOD_KSK_W in [13, 32] */
+  uint32_t ks_mod_log;     /* MOD_KSK_W in [13, 24] (the reference's sets use 21) */
   uint32_t msg_bits;       /* message bits of a block (2) */
   uint32_t carry_bits;     /* carry bits of a block (2) */
   double lwe_noise_std;    /* keygen/encryption only: std relative to the modulus */

@@ -194,7 +194,7 @@ __global__ __launch_bounds__(256) void k_keyswitch(const uint64_t* __restrict__ 
   const int j = blockIdx.x * 256 + t;
   const int n1 = a.n_lwe + 1;
   const size_t blen = (size_t)a.kN + 1;
-  int32_t acc[KS_CT];
+  uint32_t acc[KS_CT];  // sums mod 2^32 (the result is taken mod 2^w, w <= 24)
 #pragma unroll
   for (int c = 0; c < KS_CT; c++) acc[c] = 0;
   int8_t* dig8 = reinterpret_cast<int8_t*>(&dig[0][0][0]);
@@ -225,7 +225,7 @@ __global__ __launch_bounds__(256) void k_keyswitch(const uint64_t* __restrict__ 
 #pragma unroll
             for (int e = 0; e < 4; e++) {
               int32_t dv = (int32_t)(int8_t)(w >> (8 * e));
-              acc[c4 * 4 + e] += __mul24(dv, kv);
+              acc[c4 * 4 + e] += (uint32_t)__mul24(dv, kv);
             }
           }
         }

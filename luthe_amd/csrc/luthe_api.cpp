@@ -65,9 +65,10 @@ int check_params(const luthe_params* p, std::string& why) {
   if (p->pbs_level < 1 || p->pbs_level > 2) return why = "pbs_level must be 1 or 2", -1;
   if (p->pbs_base_log < 1 || p->pbs_base_log * p->pbs_level >= 64) return why = "pbs_base_log*pbs_level must be < 64", -1;
   if (p->ks_level < 1 || p->ks_level > 8) return why = "ks_level must be in [1, 8]", -1;
-  if (p->ks_base_log < 1 || p->ks_base_log > 8) return why = "ks_base_log must be in [1, 8]", -1;
+  // key-switch digits are held as int8 (|d| <= 2^(b-1)) and multiplied with 24-bit MACs
+  if (p->ks_base_log < 1 || p->ks_base_log > 7) return why = "ks_base_log must be in [1, 7]", -1;
   if (p->ks_base_log * p->ks_level >= 64) return why = "ks_base_log*ks_level must be < 64", -1;
-  if (p->ks_mod_log < 13 || p->ks_mod_log > 32) return why = "ks_mod_log must be in [13, 32]", -1;
+  if (p->ks_mod_log < 13 || p->ks_mod_log > 24) return why = "ks_mod_log must be in [13, 24]", -1;
   if (p->msg_bits + p->carry_bits < 1 || p->msg_bits + p->carry_bits > 8) return why = "msg_bits+carry_bits in [1, 8]", -1;
   if (p->regfile_size < 1 || p->regfile_size > 128) return why = "regfile_size in [1, 128]", -1;
   if (p->ct_slots > 65536) return why = "ct_slots <= 65536", -1;

@@ -301,3 +301,12 @@ def test_renamed_plan_equals_in_order_execution():
         assert S1 == S0, trial
         nlev_hazard, _ = la.dop_schedule(words, regfile_size=regs, ct_slots=nslots, lut_slots=256)
         assert nlev <= nlev_hazard, trial
+
+
+def test_keyswitch_parameter_limits():
+    """Digits are int8 and the key-switch MACs are 24-bit: parameters outside that are refused
+    (the reference's sets use KS_B_W 2-3, KS_L 5-8, MOD_KSK_W 21)."""
+    h = C.c_void_p()
+    for over in (dict(ks_base_log=8, ks_level=4), dict(ks_mod_log=25), dict(ks_mod_log=32), dict(ks_base_log=0)):
+        p = la.default_params(**over)
+        assert la.lib().luthe_create(C.byref(p), 0, C.byref(h)) == -2, over
