@@ -34,10 +34,31 @@ GL_HD uint64_t add(uint64_t a, uint64_t b) {
 }
 
 // a - b (mod p) for canonical a, b; canonical result (a - b + p when a < b).
+//
+// On the device: d = a - b with borrow w (carry chain into VCC), then d - w*EPS
+// (== d + w*P mod 2^64) as lo + w, hi - w + carry(lo + w) -- five VALU operations and no
+// select.  Two wait states separate every carry written to an SGPR from its reader.
 GL_HD uint64_t sub(uint64_t a, uint64_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint32_t lo, hi;
+  uint64_t c;
+  asm("v_sub_co_u32 %0, vcc, %3, %5\n\t"
+      "s_nop 1\n\t"
+      "v_subb_co_u32 %1, vcc, %4, %6, vcc\n\t"
+      "s_nop 1\n\t"
+      "v_addc_co_u32 %0, %2, %0, 0, vcc\n\t"
+      "s_nop 1\n\t"
+      "v_addc_co_u32 %1, %2, %1, 0, %2\n\t"
+      "v_subb_co_u32 %1, vcc, %1, 0, vcc"
+      : "=&v"(lo), "=&v"(hi), "=&s"(c)
+      : "v"((uint32_t)a), "v"((uint32_t)(a >> 32)), "v"((uint32_t)b), "v"((uint32_t)(b >> 32))
+      : "vcc");
+  return ((uint64_t)hi << 32) | lo;
+#else
   uint64_t d;
-  const bool borrow = __builtin_sub_overflow(a, b, &d);  // the subtraction's own borrow
+  const bool borrow = __builtin_sub_overflow(a, b, &d);
   return borrow ? d + P : d;
+#endif
 }
 
 GL_HD uint64_t neg(uint64_t a) { return sub(0, a); }
