@@ -33,12 +33,14 @@ GL_HD uint64_t add(uint64_t a, uint64_t b) {
   return (c1 | c2) ? t : s;
 }
 
-// a - b (mod p) for canonical a, b; canonical result (a - b + p when a < b).
+// a - b with the borrow folded back as 2^64 == EPS: (a - b) mod 2^64, minus EPS when the
+// subtraction borrowed.  For canonical a, b this is a - b (mod p), canonical (a - b + p
+// when a < b); the reductions below use it on operands where the fold cannot wrap.
 //
-// On the device: d = a - b with borrow w (carry chain into VCC), then d - w*EPS
-// (== d + w*P mod 2^64) as lo + w, hi - w + carry(lo + w) -- five VALU operations and no
-// select.  Two wait states separate every carry written to an SGPR from its reader.
-GL_HD uint64_t sub(uint64_t a, uint64_t b) {
+// On the device: d = a - b with borrow w (carry chain into VCC), then d - w*EPS as
+// lo + w, hi - w + carry(lo + w) -- five VALU operations and no select.  Two wait states
+// separate every carry written to an SGPR from its reader.
+GL_HD uint64_t sub_fold(uint64_t a, uint64_t b) {
 #if defined(__HIP_DEVICE_COMPILE__)
   uint32_t lo, hi;
   uint64_t c;
@@ -57,18 +59,19 @@ GL_HD uint64_t sub(uint64_t a, uint64_t b) {
 #else
   uint64_t d;
   const bool borrow = __builtin_sub_overflow(a, b, &d);
-  return borrow ? d + P : d;
+  return borrow ? d - EPS : d;
 #endif
 }
+
+GL_HD uint64_t sub(uint64_t a, uint64_t b) { return sub_fold(a, b); }
 
 GL_HD uint64_t neg(uint64_t a) { return sub(0, a); }
 
 // (lo + hi * 2^64) mod p, canonical.  hi * 2^64 = hh * 2^96 + hl * 2^64 == -hh + hl * EPS.
 GL_HD uint64_t reduce128(uint64_t lo, uint64_t hi) {
   uint64_t hh = hi >> 32, hl = hi & EPS;
-  uint64_t t0;
-  if (__builtin_sub_overflow(lo, hh, &t0)) t0 -= EPS;  // borrowed 2^64 == EPS; cannot wrap again
-  uint64_t t1 = (hl << 32) - hl;                        // hl * EPS < p
+  uint64_t t0 = sub_fold(lo, hh);  // borrowed 2^64 == EPS; hh < 2^32: cannot wrap again
+  uint64_t t1 = (hl << 32) - hl;   // hl * EPS < p
   uint64_t r;
   if (__builtin_add_overflow(t0, t1, &r)) r += EPS;     // carried 2^64 == EPS; cannot wrap again
   return canonical(r);
@@ -78,8 +81,7 @@ GL_HD uint64_t reduce128(uint64_t lo, uint64_t hi) {
 // joins the 2^96 part of hi as the high word of the subtrahend.
 GL_HD uint64_t reduce160(uint64_t lo, uint64_t hi, uint32_t c) {
   uint64_t hh = (hi >> 32) | ((uint64_t)c << 32), hl = hi & EPS;
-  uint64_t t0;
-  if (__builtin_sub_overflow(lo, hh, &t0)) t0 -= EPS;  // hh < 2^64 - 2^32: cannot wrap again
+  uint64_t t0 = sub_fold(lo, hh);  // hh < 2^64 - 2^32: cannot wrap again
   uint64_t t1 = (hl << 32) - hl;
   uint64_t r;
   if (__builtin_add_overflow(t0, t1, &r)) r += EPS;
