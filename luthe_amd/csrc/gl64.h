@@ -77,6 +77,15 @@ GL_HD uint64_t reduce128(uint64_t lo, uint64_t hi) {
   return canonical(r);
 }
 
+// (lo + hl * 2^64) mod p for hl < 2^32, canonical: hl * 2^64 == hl * EPS (the 2^96 part is
+// zero, so no subtraction).  lo + hl * EPS - 2^64 + EPS < 2^64 - 2^32: one fold suffices.
+GL_HD uint64_t reduce96(uint64_t lo, uint64_t hl) {
+  uint64_t t1 = (hl << 32) - hl;
+  uint64_t r;
+  if (__builtin_add_overflow(lo, t1, &r)) r += EPS;
+  return canonical(r);
+}
+
 // (lo + hi * 2^64 + c * 2^128) mod p for c < 2^32 - 1, canonical: 2^128 == -2^32, so c
 // joins the 2^96 part of hi as the high word of the subtrahend.
 GL_HD uint64_t reduce160(uint64_t lo, uint64_t hi, uint32_t c) {
@@ -114,11 +123,13 @@ GL_HD uint64_t mul(uint64_t a, uint64_t b) {
 //  s < 64 : x << s is the 128-bit (x << s, x >> (64 - s)), reduced as above.
 //  s >= 64: x << s = v1 * 2^64 + v2 * 2^96 with v1 = (x << (s - 64)) mod 2^32 and
 //           v2 = x >> (96 - s), == v1 * EPS - v2 (mod p); v1 * EPS < p.
+//  s <= 32 needs no 2^96 part (x >> (64 - s) < 2^32).  x >> (96 - s) < 2^63 < p.
 GL_HD uint64_t shl96(uint64_t x, uint32_t s) {
   if (s == 0) return canonical(x);
+  if (s <= 32) return reduce96(x << s, x >> (64 - s));
   if (s < 64) return reduce128(x << s, x >> (64 - s));
   uint64_t v1 = (x << (s - 64)) & EPS;
-  uint64_t v2 = canonical(x >> (96 - s));
+  uint64_t v2 = x >> (96 - s);
   return sub((v1 << 32) - v1, v2);
 }
 
@@ -128,16 +139,24 @@ GL_HD uint64_t mul_pow2(uint64_t x, uint32_t s) {
   return shl96(x, s);
 }
 
-// Compile-time shift amount (the DFT-8/DFT-4 twiddles).
+// Compile-time shift amount (the DFT and four-step twiddles), x canonical.  S is taken
+// mod 192; S mod 192 >= 96 is the negated shift by S - 96.  For shifts in [64, 96) the
+// result is v1 * EPS - v2, so the negated form is the swapped subtraction at no cost.
 template <int S>
 GL_HD uint64_t mul_pow2c(uint64_t x) {
   constexpr int s = ((S % 192) + 192) % 192;
-  if constexpr (s == 0) {
-    return x;
-  } else if constexpr (s >= 96) {
-    return neg(shl96(x, (uint32_t)(s - 96)));
+  constexpr bool negate = s >= 96;
+  constexpr int e = negate ? s - 96 : s;
+  if constexpr (e == 0) {
+    return negate ? neg(x) : x;
+  } else if constexpr (e >= 64) {
+    const uint64_t v1 = (x << (e - 64)) & EPS;
+    const uint64_t v2 = x >> (96 - e);
+    const uint64_t v1e = (v1 << 32) - v1;
+    return negate ? sub(v2, v1e) : sub(v1e, v2);
   } else {
-    return shl96(x, (uint32_t)s);
+    const uint64_t r = (e <= 32) ? reduce96(x << e, x >> (64 - e)) : reduce128(x << e, x >> (64 - e));
+    return negate ? neg(r) : r;
   }
 }
 

@@ -94,7 +94,7 @@ __device__ __forceinline__ void inv_passes(uint64_t (*v)[8], uint64_t* lds, cons
 #pragma unroll
   for (int p = 0; p < NP; p++) {
     inv_a1_load(v[p], lds + p * N, r, m0);
-    inv_a1_compute(v[p], m0u);
+    inv_a1_compute(v[p]);
   }
 }
 

@@ -15,9 +15,11 @@
 // [5,6] (sw/ntt/ntt_gf64.sage:367-445: negacyclic radix-32 column, network, ngc phi,
 // cyclic radix-64 column), i.e. the HPU's n5c6 cut, so NTT-domain operands given in
 // the reference order need no permutation.  All twiddles except B are powers of two
-// (omega_64 = 8, sw/ntt/ntt_gf64.sage:35-38) applied as shifts; B is the only general
-// modular multiplication.  The inverse mirrors the passes with inverse roots and
-// folds 1/N into the B twist table.
+// (omega_64 = 8, sw/ntt/ntt_gf64.sage:35-38): those of the A passes are shifts by
+// compile-time amounts (specialised per wave, whose m0 or k1 they depend on); C's
+// 8^(w1 r0), whose r0 varies per lane, comes from a 64-entry table.  The inverse runs
+// the forward DFT networks with reversed outputs (inverse roots), the inverse twiddles,
+// and folds 1/N into the B twist table.
 //
 // Thread mappings (t = thread id in 0..255):
 //   A map   : r = t & 63, m0 = t >> 6   owns coefficients n = 64 (m0 + 4 m1) + r, m1 < 8
@@ -46,7 +48,7 @@ GL_HD int idx_y1(int k1, int m0, int r) { return (k1 * 4 + m0) * 64 + r; }
 GL_HD int idx_t(int k, int r) { return k * 64 + (r ^ ((k & 7) << 3)); }
 GL_HD int idx_u(int k, int w1, int r0) { return k * 64 + 8 * (w1 ^ (k & 7)) + (r0 ^ w1); }
 
-// DFT-8 in natural order with root 2^S1 (S1 = 24 forward, 168 = -24 inverse).
+// DFT-8 in natural order with root 2^S1 (S1 = 24).
 template <int S1>
 GL_HD void dft8(uint64_t* x) {
   uint64_t a, b;
@@ -70,7 +72,7 @@ GL_HD void dft8(uint64_t* x) {
   a = x[3]; x[3] = x[6]; x[6] = a;
 }
 
-// DFT-4 in natural order with root 2^S2 (S2 = 48 forward, 144 = -48 inverse).
+// DFT-4 in natural order with root 2^S2 (S2 = 48).
 template <int S2>
 GL_HD void dft4(uint64_t* x) {
   uint64_t a0 = gl::add(x[0], x[2]), a1 = gl::sub(x[0], x[2]);
@@ -79,6 +81,49 @@ GL_HD void dft4(uint64_t* x) {
   x[2] = gl::sub(a0, b0);
   x[1] = gl::add(a1, b1);
   x[3] = gl::sub(a1, b1);
+}
+
+// Inverse DFT-8 / DFT-4 (roots 2^-24, 2^-48): sum_j x[j] w^(-jk) is output (n - k) mod n
+// of the forward transform, so the inverse is the forward network (no negated twiddles)
+// with outputs k and n - k exchanged -- a register renaming.
+GL_HD void swap_(uint64_t& a, uint64_t& b) {
+  uint64_t t = a;
+  a = b;
+  b = t;
+}
+GL_HD void idft8(uint64_t* x) {
+  dft8<24>(x);
+  swap_(x[1], x[7]);
+  swap_(x[2], x[6]);
+  swap_(x[3], x[5]);
+}
+GL_HD void idft4(uint64_t* x) {
+  dft4<48>(x);
+  swap_(x[1], x[3]);
+}
+
+// Twists by compile-time powers of two over registers K..7 / K..3 (template recursion keeps
+// every shift amount a constant, so each wave-uniform case compiles to straight-line code).
+template <int M0, int K = 0>
+GL_HD void a1_post_twist(uint64_t* v) {  // v[k1] *= 2^(3 M0 (1 + 2 k1))
+  if constexpr (K < 8) {
+    v[K] = gl::mul_pow2c<3 * M0 * (1 + 2 * K)>(v[K]);
+    a1_post_twist<M0, K + 1>(v);
+  }
+}
+template <int K1, int M0 = 0>
+GL_HD void ia2_twist(uint64_t* v) {  // v[m0] *= -2^(-3 m0 (2 K1 + 1)) = 2^(96 - 3 m0 (2 K1 + 1))
+  if constexpr (M0 < 4) {
+    v[M0] = gl::mul_pow2c<96 - 3 * M0 * (2 * K1 + 1)>(v[M0]);
+    ia2_twist<K1, M0 + 1>(v);
+  }
+}
+template <int M1 = 0>
+GL_HD void ia1_twist(uint64_t* v) {  // v[m1] *= -2^(-12 m1) = 2^(96 - 12 m1)
+  if constexpr (M1 < 8) {
+    v[M1] = gl::mul_pow2c<96 - 12 * M1>(v[M1]);
+    ia1_twist<M1 + 1>(v);
+  }
 }
 
 // Wave-uniform copy of a value that is uniform within each 64-lane wave (the A-pass
@@ -104,9 +149,11 @@ GL_HD void fwd_a1_compute(uint64_t* v, int m0u) {
   v[6] = gl::mul_pow2c<72>(v[6]);
   v[7] = gl::mul_pow2c<84>(v[7]);
   dft8<24>(v);
-  if (m0u != 0) {
-#pragma unroll
-    for (int k1 = 0; k1 < 8; k1++) v[k1] = gl::mul_pow2(v[k1], (uint32_t)(3 * m0u * (1 + 2 * k1)));
+  switch (m0u) {
+    case 1: a1_post_twist<1>(v); break;
+    case 2: a1_post_twist<2>(v); break;
+    case 3: a1_post_twist<3>(v); break;
+    default: break;
   }
 }
 GL_HD void fwd_a1_store(const uint64_t* v, uint64_t* lds, int r, int m0) {
@@ -161,7 +208,7 @@ GL_HD void fwd_c2_compute(uint64_t* v) { dft8<24>(v); }  // v[w0] = Z[64k + w1 +
 // ---------------------------------------------------------------- inverse passes
 // IC2: omega^(-w1 r0) with w1 per lane, from ic64[8 w1 + r0].
 GL_HD void inv_c2_compute(uint64_t* v, int w1, const uint64_t* ic64) {
-  dft8<168>(v);
+  idft8(v);
 #pragma unroll
   for (int r0 = 1; r0 < 8; r0++) v[r0] = gl::mul(v[r0], ic64[8 * w1 + r0]);
 }
@@ -174,7 +221,7 @@ GL_HD void inv_c1_load(uint64_t* v, const uint64_t* lds, int k, int r0) {
   for (int w1 = 0; w1 < 8; w1++) v[w1] = lds[idx_u(k, w1, r0)];
 }
 GL_HD void inv_c1_compute(uint64_t* v, int k, int r0, const uint64_t* itw) {
-  dft8<168>(v);
+  idft8(v);
 #pragma unroll
   for (int r1 = 0; r1 < 8; r1++) v[r1] = gl::mul(v[r1], itw[k * 64 + r0 + 8 * r1]);
 }
@@ -188,16 +235,17 @@ GL_HD void inv_a2_load(uint64_t* v, const uint64_t* lds, int r, int q) {
 #pragma unroll
     for (int k0 = 0; k0 < 4; k0++) v[h * 4 + k0] = lds[idx_t(2 * q + h + 8 * k0, r)];
 }
-// IA2: Omega^(-k1 m0) = 2^(-6 k1 m0), uniform per wave (k1 = 2q + h).
+// IA2: Omega^(-k1 m0) psi^(-m0) = 2^(-3 m0 (2 k1 + 1)), uniform per wave (k1 = 2q + h).
+// The psi^(-m0) part of the final untwist psi^-(m0 + 4 m1) is constant over IA1's
+// DFT-8 and moves here; both stages carry a factor -1 (the cheaper shift forms).
 GL_HD void inv_a2_compute(uint64_t* v, int qu) {
-#pragma unroll
-  for (int h = 0; h < 2; h++) {
-    dft4<144>(v + h * 4);
-    int k1 = 2 * qu + h;
-    if (k1 != 0) {
-#pragma unroll
-      for (int m0 = 1; m0 < 4; m0++) v[h * 4 + m0] = gl::mul_pow2(v[h * 4 + m0], (uint32_t)(192 - 6 * k1 * m0));
-    }
+  idft4(v);
+  idft4(v + 4);
+  switch (qu) {
+    case 0: ia2_twist<0>(v); ia2_twist<1>(v + 4); break;
+    case 1: ia2_twist<2>(v); ia2_twist<3>(v + 4); break;
+    case 2: ia2_twist<4>(v); ia2_twist<5>(v + 4); break;
+    default: ia2_twist<6>(v); ia2_twist<7>(v + 4); break;
   }
 }
 GL_HD void inv_a2_store(const uint64_t* v, uint64_t* lds, int r, int q) {
@@ -210,14 +258,11 @@ GL_HD void inv_a1_load(uint64_t* v, const uint64_t* lds, int r, int m0) {
 #pragma unroll
   for (int k1 = 0; k1 < 8; k1++) v[k1] = lds[idx_y1(k1, m0, r)];
 }
-// Output v[m1] = p[64 (m0 + 4 m1) + r] (canonical); psi^-(m0 + 4 m1) uniform per wave.
-GL_HD void inv_a1_compute(uint64_t* v, int m0u) {
-  dft8<168>(v);
-#pragma unroll
-  for (int m1 = 0; m1 < 8; m1++) {
-    uint32_t s = (uint32_t)((192 - 3 * (m0u + 4 * m1)) % 192);
-    v[m1] = gl::mul_pow2(v[m1], s);
-  }
+// Output v[m1] = p[64 (m0 + 4 m1) + r] (canonical): the remaining untwist psi^(-4 m1)
+// times the -1 that cancels IA2's.
+GL_HD void inv_a1_compute(uint64_t* v) {
+  idft8(v);
+  ia1_twist(v);
 }
 
 // Twiddle tables (host side), one buffer of TABLE_WORDS u64:

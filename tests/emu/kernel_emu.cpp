@@ -79,7 +79,7 @@ void emu_ntt_inv(const uint64_t* in, uint64_t* out) {
   for (int t = 0; t < T; t++) {
     int r = t & 63, m0 = t >> 6;
     inv_a1_load(v[t], lds.data(), r, m0);
-    inv_a1_compute(v[t], m0);
+    inv_a1_compute(v[t]);
     for (int m1 = 0; m1 < 8; m1++) out[64 * (m0 + 4 * m1) + r] = v[t][m1];
   }
 }
@@ -88,6 +88,8 @@ uint64_t emu_gl_mul(uint64_t a, uint64_t b) { return gl::mul(a, b); }
 uint64_t emu_gl_add(uint64_t a, uint64_t b) { return gl::add(a, b); }
 uint64_t emu_gl_sub(uint64_t a, uint64_t b) { return gl::sub(a, b); }
 uint64_t emu_gl_mul_pow2(uint64_t a, uint32_t s) { return gl::mul_pow2(a, s); }
+// every compile-time shift: out[i] = x * 2^(i - 192) for i < 384 (negative amounts included)
+void emu_gl_mul_pow2c_all(uint64_t x, uint64_t* out);
 uint64_t emu_gl_reduce128(uint64_t lo, uint64_t hi) { return gl::reduce128(lo, hi); }
 uint64_t emu_gl_reduce160(uint64_t lo, uint64_t hi, uint32_t c) { return gl::reduce160(lo, hi, c); }
 uint64_t emu_modswitch(uint64_t e) { return gl::modswitch_p_to_q(e); }
@@ -176,7 +178,7 @@ void blind_rotate(int n, const uint64_t* bsk_int, const uint64_t* lut, const uin
     for (int t = 0; t < T; t++)
       for (int j = 0; j < K1; j++) {
         inv_a1_load(v[t][j], lds.data() + j * N, t & 63, t >> 6);
-        inv_a1_compute(v[t][j], t >> 6);
+        inv_a1_compute(v[t][j]);
         for (int m1 = 0; m1 < 8; m1++) acc[t][j][m1] += gl::modswitch_p_to_q(v[t][j][m1]);
       }
   }
@@ -214,3 +216,15 @@ uint64_t emu_extract_coef(const uint64_t* acc, int g, int m, uint32_t b, uint32_
   return pbs::extract_coef(acc, g, m, b, offs);
 }
 }
+
+namespace {
+template <int S>
+void pow2c_all(uint64_t x, uint64_t* out) {
+  if constexpr (S < 192) {
+    out[S + 192] = gl::mul_pow2c<S>(x);
+    pow2c_all<S + 1>(x, out);
+  }
+}
+}  // namespace
+
+extern "C" void emu_gl_mul_pow2c_all(uint64_t x, uint64_t* out) { pow2c_all<-192>(x, out); }

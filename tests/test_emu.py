@@ -54,6 +54,14 @@ def test_device_arithmetic(emu):
             assert emu.emu_gl_mul_pow2(x, s) == (x << s) % P
     for e in canon:
         assert emu.emu_modswitch(e) == ref.modsw(e)
+    # compile-time shifts, every amount in [-192, 192): negated forms, the 2^96-free
+    # reduction (amounts <= 32) and the swapped subtraction (amounts in [64, 96) + 96)
+    out = np.zeros(384, np.uint64)
+    emu.emu_gl_mul_pow2c_all.argtypes = [C.c_uint64, C.c_void_p]
+    for x in canon[:300]:
+        emu.emu_gl_mul_pow2c_all(x, out.ctypes.data_as(C.c_void_p))
+        exp = [(x * pow(2, s % 192, P)) % P for s in range(-192, 192)]
+        assert [int(v) for v in out] == exp, x
 
 
 def test_kernel_ntt_matches_oracle(emu):
