@@ -174,6 +174,10 @@ GL_HD uint64_t inv(uint64_t a) { return pow(a, P - 2); }
 
 // Digit lift: signed integer -> field element.
 GL_HD uint64_t from_signed(int64_t d) { return d < 0 ? P - (uint64_t)(-d) : (uint64_t)d; }
+GL_HD uint64_t from_signed32(int32_t d) {
+  const uint64_t s = (uint64_t)(int64_t)d;  // d + 2^64 when negative; + p - 2^64 = - EPS
+  return d < 0 ? s - EPS : s;
+}
 
 // p -> 2^64 modulus switch of the reference (mod_switch_to_2powerN.sv:60-64,132-135,
 // PRECISION_W = 96 per pe_pbs.sv:38): z = bits[159:96] of e*C + bit 95, with

@@ -250,7 +250,8 @@ __global__ __launch_bounds__(256) void k_keyswitch(const uint64_t* __restrict__ 
 // ------------------------------------------------------------------------------------
 // Resident workgroups per CU requested from the register allocator: 32 KB of LDS per
 // workgroup at L = 1 allows 4 (4 waves per SIMD at <= 128 VGPRs); L = 2 needs 64 KB.
-template <int K1, int L>
+// HI: L * base_log <= 31, the decomposition reads only the high word of each coefficient.
+template <int K1, int L, bool HI>
 __global__ __launch_bounds__(256, (L == 1 ? 4 : 2)) void k_blind_rotate(const uint64_t* __restrict__ bsk,
                                                       const uint64_t* __restrict__ luts,
                                                       const uint16_t* __restrict__ lwe2n,
@@ -289,7 +290,7 @@ __global__ __launch_bounds__(256, (L == 1 ? 4 : 2)) void k_blind_rotate(const ui
         for (int m1 = 0; m1 < 8; m1++) lds[g * N + 64 * (mm0 + 4 * m1) + rr] = acc[g][m1];
       __syncthreads();
 #pragma unroll
-      for (int g = 0; g < K1; g++) pbs::rotate_decompose<L>(lds + g * N, acc[g], tt, rot, a.base_log, &v[g * L]);
+      for (int g = 0; g < K1; g++) pbs::rotate_decompose<L, HI>(lds + g * N, acc[g], tt, rot, a.base_log, &v[g * L]);
       __syncthreads();
       fwd_passes<NPF>(v, lds, tab, tt);
       pbs::mac<K1, L>(v, bsk_i, tt);
@@ -331,7 +332,7 @@ __global__ __launch_bounds__(256, (L == 1 ? 4 : 2)) void k_blind_rotate(const ui
 // halves exchange NTT-domain digits through LDS before the product.  Arithmetic is the
 // same canonical Goldilocks arithmetic, so results equal k_blind_rotate's bit for bit.
 // ------------------------------------------------------------------------------------
-template <int K1, int L>
+template <int K1, int L, bool HI>
 __global__ __launch_bounds__(512, 1) void k_blind_rotate_lat(const uint64_t* __restrict__ bsk,
                                                              const uint64_t* __restrict__ luts,
                                                              const uint16_t* __restrict__ lwe2n,
@@ -372,7 +373,7 @@ __global__ __launch_bounds__(512, 1) void k_blind_rotate_lat(const uint64_t* __r
       for (int m1 = 0; m1 < 8; m1++) lds[h * N + 64 * (mm0 + 4 * m1) + rr] = acc[m1];
       __syncthreads();
       uint64_t v[L][8];
-      pbs::rotate_decompose<L>(lds + h * N, acc, tt, rot, a.base_log, v);
+      pbs::rotate_decompose<L, HI>(lds + h * N, acc, tt, rot, a.base_log, v);
       __syncthreads();
       fwd_passes<L>(v, lds + h * L * N, tab, tt);
       // exchange the digit NTTs: row gl = g * L + l at [(gl * 8 + w0) * T + t]
@@ -576,7 +577,7 @@ static int br_max_grid() {
     int dev = 0, cus = 0, per = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_blind_rotate<K1, L>, 256, 0);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_blind_rotate<K1, L, true>, 256, 0);
     cached = (cus > 0 ? cus : 256) * (per > 0 ? per : 1);
   }
   return cached;
@@ -589,38 +590,44 @@ static int br_lat_max_grid() {
     int dev = 0, cus = 0, per = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_blind_rotate_lat<K1, L>, 512, 0);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_blind_rotate_lat<K1, L, true>, 512, 0);
     cached = (cus > 0 ? cus : 256) * (per > 0 ? per : 1);
   }
   return cached;
+}
+
+template <int L, bool HI>
+static void br_launch(const uint64_t* bsk, const uint64_t* luts, const uint16_t* lwe2n, const PbsJob* jobs, int nct,
+                      uint64_t* out_base, const uint64_t* tab, const BrArgs& a, int grid_override, hipStream_t s,
+                      bool latency) {
+  if (latency) {
+    int g = br_lat_max_grid<2, L>();
+    g = g < nct ? g : nct;
+    k_blind_rotate_lat<2, L, HI><<<dim3(g), 512, 0, s>>>(bsk, luts, lwe2n, jobs, nct, out_base, tab, a);
+  } else {
+    int g = grid_override > 0 ? grid_override : br_max_grid<2, L>();
+    g = g < nct ? g : nct;
+    k_blind_rotate<2, L, HI><<<dim3(g), 256, 0, s>>>(bsk, luts, lwe2n, jobs, nct, out_base, tab, a);
+  }
 }
 
 hipError_t launch_blind_rotate(const uint64_t* bsk, const uint64_t* luts, const uint16_t* lwe2n, const PbsJob* jobs,
                                int nct, uint64_t* out_base, const uint64_t* tab, const BrArgs& a,
                                int pbs_level, int grid_override, hipStream_t s, int latency_max) {
   if (nct == 0) return hipSuccess;
-  if (nct <= latency_max && (pbs_level == 1 || pbs_level == 2)) {
-    if (pbs_level == 1) {
-      int g = br_lat_max_grid<2, 1>();
-      g = g < nct ? g : nct;
-      k_blind_rotate_lat<2, 1><<<dim3(g), 512, 0, s>>>(bsk, luts, lwe2n, jobs, nct, out_base, tab, a);
-    } else {
-      int g = br_lat_max_grid<2, 2>();
-      g = g < nct ? g : nct;
-      k_blind_rotate_lat<2, 2><<<dim3(g), 512, 0, s>>>(bsk, luts, lwe2n, jobs, nct, out_base, tab, a);
-    }
-    return hipGetLastError();
-  }
+  if (pbs_level != 1 && pbs_level != 2) return hipErrorInvalidValue;
+  const bool latency = nct <= latency_max;
+  const bool hi = (uint32_t)pbs_level * a.base_log <= 31;
   if (pbs_level == 1) {
-    int g = grid_override > 0 ? grid_override : br_max_grid<2, 1>();
-    g = g < nct ? g : nct;
-    k_blind_rotate<2, 1><<<dim3(g), 256, 0, s>>>(bsk, luts, lwe2n, jobs, nct, out_base, tab, a);
-  } else if (pbs_level == 2) {
-    int g = grid_override > 0 ? grid_override : br_max_grid<2, 2>();
-    g = g < nct ? g : nct;
-    k_blind_rotate<2, 2><<<dim3(g), 256, 0, s>>>(bsk, luts, lwe2n, jobs, nct, out_base, tab, a);
+    if (hi)
+      br_launch<1, true>(bsk, luts, lwe2n, jobs, nct, out_base, tab, a, grid_override, s, latency);
+    else
+      br_launch<1, false>(bsk, luts, lwe2n, jobs, nct, out_base, tab, a, grid_override, s, latency);
   } else {
-    return hipErrorInvalidValue;
+    if (hi)
+      br_launch<2, true>(bsk, luts, lwe2n, jobs, nct, out_base, tab, a, grid_override, s, latency);
+    else
+      br_launch<2, false>(bsk, luts, lwe2n, jobs, nct, out_base, tab, a, grid_override, s, latency);
   }
   return hipGetLastError();
 }

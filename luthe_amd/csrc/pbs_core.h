@@ -19,13 +19,16 @@ GL_HD int coef_index(int t, int m1) { return 64 * ((t >> 6) + 4 * m1) + (t & 63)
 // first: closest representable value (top L*b bits + round bit), sign of the closest
 // representable, then the carry chain of decomp_parallel.sv:56-150 (same algorithm as
 // decomp_balseq_core.sv:10-173 and balanced_decomp_unfix_sign_round_bit.py:42-105).
+// With v = u + c (raw digit plus incoming carry) and H = 2^(b-1), the chain's
+// propagate = ov | gt | (ge & msb(next)) with gt = [v > H], ge = [v >= H], ov = [v = 2^b]
+// is [v + msb(next) > H]; the top level's "next" msb is the sign.
 // Requires L*b < 64.  Digits are returned as signed ints.
 template <int L>
 GL_HD void decompose(uint64_t x, uint32_t b, int64_t* d) {
   const uint32_t R = (uint32_t)L * b;
   const uint32_t ofs = 64 - R;
   const uint64_t bmask = (1ull << b) - 1;
-  const uint64_t hlow = (b >= 2) ? ((1ull << (b - 1)) - 1) : 0;
+  const uint64_t H = 1ull << (b - 1);
   uint64_t rb = (x >> (ofs - 1)) & 1;
   uint64_t top = x >> ofs;                                 // R bits
   uint64_t closest = (top + rb) & ((1ull << R) - 1);
@@ -36,17 +39,42 @@ GL_HD void decompose(uint64_t x, uint32_t b, int64_t* d) {
   uint64_t c = 0;
 #pragma unroll
   for (int i = 0; i < L; i++) {
-    uint64_t u = (closest >> (i * b)) & bmask;
-    uint64_t un = (i + 1 < L) ? ((closest >> ((i + 1) * b)) & bmask) : (sign ? bmask : 0);
-    uint64_t half = (u >> (b - 1)) & 1;
-    uint64_t low = u & hlow;
-    uint64_t gt = half & (uint64_t)((low != 0) | c);
-    uint64_t ge = half | ((uint64_t)(low == hlow) & c);
-    uint64_t ov = (uint64_t)(u == bmask) & c;
-    uint64_t prop = ov | gt | (ge & ((un >> (b - 1)) & 1));
-    int64_t t = (int64_t)(u + c);
-    d[i] = prop ? t - (int64_t)(1ull << b) : t;
+    uint64_t v = ((closest >> (i * b)) & bmask) + c;
+    uint64_t nmsb = (i + 1 < L) ? ((closest >> ((i + 1) * b + b - 1)) & 1) : sign;
+    uint64_t prop = (v + nmsb) > H;
+    d[i] = (int64_t)v - (int64_t)(prop << b);
     c = prop;
+  }
+}
+
+// The same digits when L*b <= 31, from the high word xh = x >> 32 alone: the round bit
+// (bit 63 - L*b) and every digit bit lie in it, and adding the round bit at bit 31 - L*b
+// of xh gives closest in its top L*b bits.  For L = 1 the digit is closest read as a
+// signed L*b-bit number, except closest = 2^(b-1) reached without rounding (top = 2^(b-1),
+// rb = 0), whose sign is 0 and whose digit stays +2^(b-1).
+template <int L>
+GL_HD void decompose_hi(uint32_t xh, uint32_t b, int32_t* d) {
+  const uint32_t R = (uint32_t)L * b;
+  const uint32_t rounded = xh + (1u << (31 - R));
+  if constexpr (L == 1) {
+    const int32_t sx = (int32_t)rounded >> (32 - R);
+    const bool tie = (xh >> (31 - R)) == (1u << R);
+    d[0] = tie ? (int32_t)(1u << (R - 1)) : sx;
+  } else {
+    const uint32_t closest = rounded >> (32 - R);
+    const uint32_t top = xh >> (32 - R), rb = (xh >> (31 - R)) & 1;
+    const uint32_t below_mask = (1u << (R - 1)) - 1, below = top & below_mask;
+    const uint32_t sign = (xh >> 31) ? (uint32_t)((below != 0) | rb) : (uint32_t)((below == below_mask) & rb);
+    const uint32_t bmask = (1u << b) - 1, H = 1u << (b - 1);
+    uint32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < L; i++) {
+      uint32_t v = ((closest >> (i * b)) & bmask) + c;
+      uint32_t nmsb = (i + 1 < L) ? ((closest >> ((i + 1) * b + b - 1)) & 1) : sign;
+      uint32_t prop = (v + nmsb) > H;
+      d[i] = (int32_t)v - (int32_t)(prop << b);
+      c = prop;
+    }
   }
 }
 
@@ -54,7 +82,8 @@ GL_HD void decompose(uint64_t x, uint32_t b, int64_t* d) {
 // and lifted into Z_p: out[l][m1].  acc_lds is the polynomial in plain order.
 // Rotation semantics of pep_mmacc_splitc_feed_read.sv:484-488 and feed_final.sv:203-330:
 // coefficient n reads ACC[(n - a) mod 2N], negated when that index lands in [N, 2N).
-template <int L>
+// HI: L * base_log <= 31 (decompose_hi on the high word).
+template <int L, bool HI>
 GL_HD void rotate_decompose(const uint64_t* acc_lds, const uint64_t* acc_reg, int t, uint32_t a,
                             uint32_t base_log, uint64_t (*out)[8]) {
 #pragma unroll
@@ -64,10 +93,17 @@ GL_HD void rotate_decompose(const uint64_t* acc_lds, const uint64_t* acc_reg, in
     uint64_t val = acc_lds[idx & (N - 1)];
     val = (idx >= (uint32_t)N) ? (0 - val) : val;
     uint64_t ct10 = val - acc_reg[m1];
-    int64_t d[L];
-    decompose<L>(ct10, base_log, d);
+    if constexpr (HI) {
+      int32_t d[L];
+      decompose_hi<L>((uint32_t)(ct10 >> 32), base_log, d);
 #pragma unroll
-    for (int l = 0; l < L; l++) out[l][m1] = gl::from_signed(d[l]);
+      for (int l = 0; l < L; l++) out[l][m1] = gl::from_signed32(d[l]);
+    } else {
+      int64_t d[L];
+      decompose<L>(ct10, base_log, d);
+#pragma unroll
+      for (int l = 0; l < L; l++) out[l][m1] = gl::from_signed(d[l]);
+    }
   }
 }
 

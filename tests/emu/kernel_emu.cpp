@@ -131,7 +131,12 @@ void blind_rotate(int n, const uint64_t* bsk_int, const uint64_t* lut, const uin
       for (int g = 0; g < K1; g++)
         for (int m1 = 0; m1 < 8; m1++) lds[g * N + pbs::coef_index(t, m1)] = acc[t][g][m1];
     for (int t = 0; t < T; t++)
-      for (int g = 0; g < K1; g++) pbs::rotate_decompose<L>(lds.data() + g * N, acc[t][g], t, a, base_log, &v[t][g * L]);
+      for (int g = 0; g < K1; g++) {
+        if (L * base_log <= 31)
+          pbs::rotate_decompose<L, true>(lds.data() + g * N, acc[t][g], t, a, base_log, &v[t][g * L]);
+        else
+          pbs::rotate_decompose<L, false>(lds.data() + g * N, acc[t][g], t, a, base_log, &v[t][g * L]);
+      }
     for (int t = 0; t < T; t++)
       for (int p = 0; p < NPF; p++) fwd_a1_compute(v[t][p], t >> 6);
     for (int t = 0; t < T; t++)
@@ -212,6 +217,21 @@ void emu_decompose(uint64_t x, uint32_t b, int L, int64_t* d) {
     default: break;
   }
 }
+// decompose_hi for L * b <= 31 (digits as int32); returns -1 when not applicable
+int emu_decompose_hi(uint64_t x, uint32_t b, int L, int32_t* d) {
+  if ((uint32_t)L * b > 31) return -1;
+  const uint32_t xh = (uint32_t)(x >> 32);
+  switch (L) {
+    case 1: pbs::decompose_hi<1>(xh, b, d); break;
+    case 2: pbs::decompose_hi<2>(xh, b, d); break;
+    case 3: pbs::decompose_hi<3>(xh, b, d); break;
+    case 4: pbs::decompose_hi<4>(xh, b, d); break;
+    case 5: pbs::decompose_hi<5>(xh, b, d); break;
+    default: return -1;
+  }
+  return 0;
+}
+uint64_t emu_from_signed32(int32_t d) { return gl::from_signed32(d); }
 uint64_t emu_extract_coef(const uint64_t* acc, int g, int m, uint32_t b, uint32_t offs) {
   return pbs::extract_coef(acc, g, m, b, offs);
 }

@@ -113,6 +113,42 @@ def test_kernel_decomposition_and_extract(emu):
         for x in xs:
             emu.emu_decompose(x, b, Lv, d.ctypes.data_as(vp))
             assert list(d) == ref.decompose(x, 64, b, Lv)
+    # the high-word path (L*b <= 31) against the generic one: every top/round-bit pattern
+    # for small bases, and the tie neighbourhoods (closest = 2^(b-1) with and without
+    # rounding, all-ones top with the round bit) at the benchmark bases
+    emu.emu_decompose_hi.argtypes = [C.c_uint64, C.c_uint32, C.c_int, vp]
+    emu.emu_from_signed32.restype = C.c_uint64
+    emu.emu_from_signed32.argtypes = [C.c_int32]
+    for d in (0, 1, -1, 2**31 - 1, -2**31, 12345, -12345):
+        assert emu.emu_from_signed32(d) == d % P
+    def hi_cases(Lv, b):
+        R = Lv * b
+        ofs = 64 - R
+        if R <= 9:
+            tops = range(1 << R)
+        else:
+            t = [0, 1, (1 << R) - 1, (1 << (R - 1)), (1 << (R - 1)) - 1, (1 << (R - 1)) + 1]
+            t += [(1 << (b - 1)) | (k << b) for k in range(4)] + [((1 << (b - 1)) - 1) | (k << b) for k in range(4)]
+            tops = t + [int(v) for v in rng.integers(0, 1 << R, 200)]
+        for top in tops:
+            for rb in (0, 1):
+                for low in (0, 1, (1 << (ofs - 1)) - 1):
+                    yield (top << ofs) | (rb << (ofs - 1)) | low
+    for Lv, bs in [(1, range(1, 32)), (2, range(1, 16)), (3, range(1, 11)), (4, range(1, 8)), (5, range(1, 7))]:
+        for b in bs:
+            d64 = np.zeros(Lv, np.int64)
+            d32 = np.zeros(Lv, np.int32)
+            for x in hi_cases(Lv, b):
+                emu.emu_decompose(x, b, Lv, d64.ctypes.data_as(vp))
+                assert emu.emu_decompose_hi(x, b, Lv, d32.ctypes.data_as(vp)) == 0
+                assert list(d32) == list(d64), (Lv, b, hex(x))
+    # the same structured cases through the generic path against the oracle, at the
+    # PBS and key-switch parameter sets
+    for Lv, b in [(1, 23), (2, 15), (5, 3), (1, 5), (2, 4)]:
+        d64 = np.zeros(Lv, np.int64)
+        for x in hi_cases(Lv, b):
+            emu.emu_decompose(x, b, Lv, d64.ctypes.data_as(vp))
+            assert list(d64) == ref.decompose(x, 64, b, Lv), (Lv, b, hex(x))
     emu.emu_extract_coef.restype = C.c_uint64
     emu.emu_extract_coef.argtypes = [vp, C.c_int, C.c_int, C.c_uint32, C.c_uint32]
     acc = rng.integers(0, 2**64, 2 * 2048, dtype=np.uint64)
