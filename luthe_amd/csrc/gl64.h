@@ -180,21 +180,15 @@ GL_HD uint64_t from_signed32(int32_t d) {
 }
 
 // p -> 2^64 modulus switch of the reference (mod_switch_to_2powerN.sv:60-64,132-135,
-// PRECISION_W = 96 per pe_pbs.sv:38): z = bits[159:96] of e*C + bit 95, with
-// C = floor(2^160 / p) = 2^96 + 2^64 - 2, so e*C = (e*2^32 + e)*2^64 - 2e needs no
-// multiplier.  e must be canonical (< p).
+// PRECISION_W = 96 per pe_pbs.sv:38): z = bits[159:96] of e*C + bit 95 (mod 2^64), with
+// C = floor(2^160 / p) = 2^96 + 2^64 - 2.  For e = eh*2^32 + el, e*C + 2^95 over 2^96 is
+// e + eh + (el*2^64 + 2^95 - 2e) / 2^96 with the last term in [0, 2), so
+//   z = e + eh + [el*2^64 - 2e >= 2^95] = e + eh + [el > 2^31 + [e > 2^63]].
+// e must be canonical (< p); then eh + bit < 2^32 (eh = 2^32 - 1 only for e = p - 1).
 GL_HD uint64_t modswitch_p_to_q(uint64_t e) {
-  // W = e*2^32 + e as (w_hi:w_lo), 97 bits
-  uint64_t w_lo = (e << 32) + e;
-  uint64_t w_hi = (e >> 32) + (uint64_t)(w_lo < e);
-  // U = W - ceil(2e / 2^64) ... exact: V = W*2^64 - 2e ; U = V >> 64
-  uint64_t two_e_hi = e >> 63, two_e_lo = e << 1;
-  uint64_t t = two_e_hi + (uint64_t)(two_e_lo != 0);   // 0..2
-  uint64_t u_lo = w_lo - t;
-  uint64_t u_hi = w_hi - (uint64_t)(w_lo < t);
-  // z = U[95:32] + U[31]
-  uint64_t z = (u_lo >> 32) | (u_hi << 32);
-  return z + ((u_lo >> 31) & 1);
+  const uint32_t el = (uint32_t)e, eh = (uint32_t)(e >> 32);
+  const uint32_t thr = 0x80000000u + (uint32_t)(e > 0x8000000000000000ull);
+  return e + (uint64_t)(eh + (uint32_t)(el > thr));
 }
 
 }  // namespace gl

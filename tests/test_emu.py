@@ -52,7 +52,9 @@ def test_device_arithmetic(emu):
     for x in anyv[:400]:
         for s in (0, 1, 3, 12, 24, 31, 32, 33, 48, 63, 64, 65, 72, 84, 93, 95, 96, 120, 144, 168, 191):
             assert emu.emu_gl_mul_pow2(x, s) == (x << s) % P
-    for e in canon:
+    limbs = [0, 1, 2**31 - 1, 2**31, 2**31 + 1, 2**31 + 2, 2**32 - 1]
+    edges = [h * 2**32 + l for h in limbs + [2**31 + 2, 2**32 - 2, 2**32 - 1] for l in limbs]
+    for e in canon + [e for e in edges if e < P]:
         assert emu.emu_modswitch(e) == ref.modsw(e)
     # compile-time shifts, every amount in [-192, 192): negated forms, the 2^96-free
     # reduction (amounts <= 32) and the swapped subtraction (amounts in [64, 96) + 96)
