@@ -1,0 +1,63 @@
+// Test library: runs the device builds of the Goldilocks primitives (gl64.h) and the
+// high-word decomposition on caller-given operands, so that the device code paths
+// (inline-asm carry chains included) are checked against exact integer arithmetic at edge
+// values.  Test infrastructure only.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../luthe_amd/csrc/pbs_core.h"
+
+namespace {
+constexpr int NOUT = 16;  // results per operand triple, in the order listed in prims_run
+
+__global__ void k_prims(const uint64_t* a, const uint64_t* b, const uint32_t* c, int n, uint64_t* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t x = a[i], y = b[i];
+  const uint32_t z = c[i];
+  uint64_t* o = out + (size_t)i * NOUT;
+  const uint64_t xc = gl::canonical(x), yc = gl::canonical(y);
+  o[0] = gl::canonical(x);
+  o[1] = gl::add(xc, yc);
+  o[2] = gl::sub(xc, yc);
+  o[3] = gl::mul(x, y);
+  o[4] = gl::reduce128(x, y);
+  o[5] = gl::reduce96(x, y & gl::EPS);
+  o[6] = gl::reduce160(x, y, z == 0xFFFFFFFFu ? 0 : z);
+  o[7] = gl::sub_fold(x, yc);
+  o[8] = gl::mul_pow2c<24>(xc);
+  o[9] = gl::mul_pow2c<48>(xc);
+  o[10] = gl::mul_pow2c<72>(xc);
+  o[11] = gl::mul_pow2c<120>(xc);
+  o[12] = gl::mul_pow2c<168>(xc);
+  o[13] = gl::modswitch_p_to_q(xc);
+  o[14] = gl::from_signed32((int32_t)z);
+  int32_t d[1];
+  pbs::decompose_hi<1>((uint32_t)(x >> 32), 23, d);
+  o[15] = (uint64_t)(int64_t)d[0];
+}
+}  // namespace
+
+// a, b, c: host arrays of n operands; out: host array of n * 16 results.  Returns 0 or the
+// HIP error code.
+extern "C" int prims_run(const uint64_t* a, const uint64_t* b, const uint32_t* c, int n, uint64_t* out) {
+  uint64_t *da = nullptr, *db = nullptr, *dout = nullptr;
+  uint32_t* dc = nullptr;
+  hipError_t e = hipSuccess;
+  const size_t n8 = (size_t)n * 8;
+  if ((e = hipMalloc(&da, n8)) != hipSuccess || (e = hipMalloc(&db, n8)) != hipSuccess ||
+      (e = hipMalloc(&dc, (size_t)n * 4)) != hipSuccess || (e = hipMalloc(&dout, n8 * NOUT)) != hipSuccess)
+    goto done;
+  if ((e = hipMemcpy(da, a, n8, hipMemcpyHostToDevice)) != hipSuccess) goto done;
+  if ((e = hipMemcpy(db, b, n8, hipMemcpyHostToDevice)) != hipSuccess) goto done;
+  if ((e = hipMemcpy(dc, c, (size_t)n * 4, hipMemcpyHostToDevice)) != hipSuccess) goto done;
+  k_prims<<<dim3((unsigned)((n + 255) / 256)), 256>>>(da, db, dc, n, dout);
+  if ((e = hipGetLastError()) != hipSuccess) goto done;
+  e = hipMemcpy(out, dout, n8 * NOUT, hipMemcpyDeviceToHost);
+done:
+  (void)hipFree(da);
+  (void)hipFree(db);
+  (void)hipFree(dc);
+  (void)hipFree(dout);
+  return (int)e;
+}
