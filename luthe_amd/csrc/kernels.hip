@@ -43,14 +43,10 @@ __device__ __forceinline__ void fwd_passes(uint64_t (*v)[8], uint64_t* lds, cons
 #pragma unroll
   for (int p = 0; p < NP; p++) fwd_a2_store(v[p], lds + p * N, r, m0);
   __syncthreads();
-#pragma unroll
-  for (int p = 0; p < NP; p++) {
-    fwd_c1_load(v[p], lds + p * N, k, j);
-    fwd_c1_compute(v[p], j, c64);
-  }
+  fwd_c1_load<NP>(v, lds, t);
+  fwd_c1_compute<NP>(v, t, c64);
   __syncthreads();
-#pragma unroll
-  for (int p = 0; p < NP; p++) fwd_c1_store(v[p], lds + p * N, k, j);
+  fwd_c1_store<NP>(v, lds, t);
   __syncthreads();
 #pragma unroll
   for (int p = 0; p < NP; p++) {
@@ -63,24 +59,23 @@ __device__ __forceinline__ void fwd_passes(uint64_t (*v)[8], uint64_t* lds, cons
 // barrier that orders the previous readers.  Exit: v[p][m1] canonical coefficients.
 template <int NP>
 __device__ __forceinline__ void inv_passes(uint64_t (*v)[8], uint64_t* lds, const uint64_t* tab, int t) {
+  // Opaque copy of the thread id: the index values below are derived here rather than
+  // kept live across the caller's code (which would spill at 128 VGPRs).
+  asm volatile("" : "+v"(t));
   const int r = t & 63, m0 = t >> 6, k = t >> 3, j = t & 7;
   const int m0u = wave_uniform(m0);
   const uint64_t* itw = tab + ITW_OFF;
   const uint64_t* ic64 = tab + IC64_OFF;
 #pragma unroll
-  for (int p = 0; p < NP; p++) inv_c2_compute(v[p], j, ic64);
+  for (int p = 0; p < NP; p++) inv_c2_compute(v[p]);
   __syncthreads();
 #pragma unroll
   for (int p = 0; p < NP; p++) inv_c2_store(v[p], lds + p * N, k, j);
   __syncthreads();
-#pragma unroll
-  for (int p = 0; p < NP; p++) {
-    inv_c1_load(v[p], lds + p * N, k, j);
-    inv_c1_compute(v[p], k, j, itw);
-  }
+  inv_c1_load<NP>(v, lds, t);
+  inv_c1_compute<NP>(v, t, itw, ic64);
   __syncthreads();
-#pragma unroll
-  for (int p = 0; p < NP; p++) inv_c1_store(v[p], lds + p * N, k, j);
+  inv_c1_store<NP>(v, lds, t);
   __syncthreads();
 #pragma unroll
   for (int p = 0; p < NP; p++) {

@@ -21,17 +21,22 @@
 // the forward DFT networks with reversed outputs (inverse roots), the inverse twiddles,
 // and folds 1/N into the B twist table.
 //
-// Thread mappings (t = thread id in 0..255):
+// Thread mappings (t = thread id in 0..255, wave w = t >> 6):
 //   A map   : r = t & 63, m0 = t >> 6   owns coefficients n = 64 (m0 + 4 m1) + r, m1 < 8
 //   A2 map  : r = t & 63, q = t >> 6    owns k1 in {2q, 2q+1}, all k0 < 4
-//   C map   : k = t >> 3, j = t & 7     (j = r0 in C1, w1 in C2)
+//   C1 map  : k = t & 31, h = (t >> 5) & 1; r0 = 2w + h (one polynomial) or, with 2 or 4
+//             polynomials per workgroup, tasks (polynomial h + 2i, r0 = 2w + j), j < 2:
+//             r0 is then uniform per wave and task, so C1's twiddles 8^(w1 r0) (and the
+//             inverse's 8^(-w1 r0), applied at the start of inverse C1) are compile-time
+//             shifts; with one polynomial they come from a 64-entry table
+//   C2 map  : k = t >> 3, w1 = t & 7
 //   NTT domain: thread (k, w1) owns indices 64 k + w1 + 8 w0, w0 < 8.
 //
 // LDS layouts per polynomial (2048 u64), XOR-swizzled to be bank-conflict free for
-// 64-bit accesses with 32 or 64 banks:
+// 64-bit accesses with 32 or 64 banks under every map above:
 //   Y1(k1, m0, r) = (4 k1 + m0) * 64 + r
-//   T (k, r)      = 64 k + (r ^ 8 (k & 7))
-//   U (k, w1, r0) = 64 k + 8 (w1 ^ (k & 7)) + (r0 ^ w1)
+//   T (k, r)      = 64 k + (r ^ k)
+//   U (k, w1, r0) = 64 k + ((8 w1 + r0) ^ k ^ 4 (w1 >> 2))
 //
 // Field values stay canonical (gl64.h).  Each pass is split into load / compute / store so that a kernel can put a barrier
 // between all loads and all stores of the same buffer (in-place LDS reuse), and a
@@ -45,8 +50,8 @@ constexpr int N = 2048;
 constexpr int T = 256;
 
 GL_HD int idx_y1(int k1, int m0, int r) { return (k1 * 4 + m0) * 64 + r; }
-GL_HD int idx_t(int k, int r) { return k * 64 + (r ^ ((k & 7) << 3)); }
-GL_HD int idx_u(int k, int w1, int r0) { return k * 64 + 8 * (w1 ^ (k & 7)) + (r0 ^ w1); }
+GL_HD int idx_t(int k, int r) { return k * 64 + (r ^ k); }
+GL_HD int idx_u(int k, int w1, int r0) { return k * 64 + ((8 * w1 + r0) ^ k ^ (4 * (w1 >> 2))); }
 
 // DFT-8 in natural order with root 2^S1 (S1 = 24).
 template <int S1>
@@ -184,19 +189,75 @@ GL_HD void fwd_a2_store(const uint64_t* v, uint64_t* lds, int r, int q) {
 #pragma unroll
     for (int k0 = 0; k0 < 4; k0++) lds[idx_t(2 * q + h + 8 * k0, r)] = v[h * 4 + k0];
 }
-// C1: omega^(w1 r0) with r0 per lane, from the 64-entry table c64[8 w1 + r0].
-GL_HD void fwd_c1_load(uint64_t* v, const uint64_t* lds, int k, int r0) {
-#pragma unroll
-  for (int r1 = 0; r1 < 8; r1++) v[r1] = lds[idx_t(k, r0 + 8 * r1)];
+// C1 tasks (C1 map above): polynomial and r0 of task j.
+template <int NP>
+GL_HD int c1_poly(int j, int h) { return NP == 1 ? 0 : h + 2 * (j >> 1); }
+template <int NP>
+GL_HD int c1_r0(int j, int w, int h) { return NP == 1 ? 2 * w + h : 2 * w + (j & 1); }
+
+// v[w1] *= 8^(S w1 R0) = 2^(3 S w1 R0), w1 = W1..7 (S = 1 forward, -1 inverse).
+template <int S, int R0, int W1 = 1>
+GL_HD void c_twist(uint64_t* v) {
+  if constexpr (W1 < 8) {
+    v[W1] = gl::mul_pow2c<3 * S * W1 * R0>(v[W1]);
+    c_twist<S, R0, W1 + 1>(v);
+  }
 }
-GL_HD void fwd_c1_compute(uint64_t* v, int r0, const uint64_t* c64) {
-  dft8<24>(v);
+template <int S, int NP, int W>
+GL_HD void c_twist_wave(uint64_t (*v)[8]) {
 #pragma unroll
-  for (int w1 = 1; w1 < 8; w1++) v[w1] = gl::mul(v[w1], c64[8 * w1 + r0]);
+  for (int j = 0; j < NP; j++) {
+    if (j & 1)
+      c_twist<S, 2 * W + 1>(v[j]);
+    else
+      c_twist<S, 2 * W>(v[j]);
+  }
 }
-GL_HD void fwd_c1_store(const uint64_t* v, uint64_t* lds, int k, int r0) {
+// Wave-specialised twist of every task (NP >= 2), or the per-lane table (NP = 1).
+template <int S, int NP>
+GL_HD void c_twist_all(uint64_t (*v)[8], int t, const uint64_t* tab64) {
+  if constexpr (NP == 1) {
+    const int r0 = c1_r0<1>(0, t >> 6, (t >> 5) & 1);
 #pragma unroll
-  for (int w1 = 0; w1 < 8; w1++) lds[idx_u(k, w1, r0)] = v[w1];
+    for (int w1 = 1; w1 < 8; w1++) v[0][w1] = gl::mul(v[0][w1], tab64[8 * w1 + r0]);
+  } else {
+    switch (wave_uniform(t >> 6)) {
+      case 0: c_twist_wave<S, NP, 0>(v); break;
+      case 1: c_twist_wave<S, NP, 1>(v); break;
+      case 2: c_twist_wave<S, NP, 2>(v); break;
+      default: c_twist_wave<S, NP, 3>(v); break;
+    }
+  }
+}
+
+// C1: DFT-8 over r1, then omega^(w1 r0).  lds: the NP polynomials, 2048 words apart.
+template <int NP>
+GL_HD void fwd_c1_load(uint64_t (*v)[8], const uint64_t* lds, int t) {
+  const int k = t & 31, h = (t >> 5) & 1, w = t >> 6;
+#pragma unroll
+  for (int j = 0; j < NP; j++) {
+    const uint64_t* pl = lds + c1_poly<NP>(j, h) * N;
+    const int r0 = c1_r0<NP>(j, w, h);
+#pragma unroll
+    for (int r1 = 0; r1 < 8; r1++) v[j][r1] = pl[idx_t(k, r0 + 8 * r1)];
+  }
+}
+template <int NP>
+GL_HD void fwd_c1_compute(uint64_t (*v)[8], int t, const uint64_t* c64) {
+#pragma unroll
+  for (int j = 0; j < NP; j++) dft8<24>(v[j]);
+  c_twist_all<1, NP>(v, t, c64);
+}
+template <int NP>
+GL_HD void fwd_c1_store(const uint64_t (*v)[8], uint64_t* lds, int t) {
+  const int k = t & 31, h = (t >> 5) & 1, w = t >> 6;
+#pragma unroll
+  for (int j = 0; j < NP; j++) {
+    uint64_t* pl = lds + c1_poly<NP>(j, h) * N;
+    const int r0 = c1_r0<NP>(j, w, h);
+#pragma unroll
+    for (int w1 = 0; w1 < 8; w1++) pl[idx_u(k, w1, r0)] = v[j][w1];
+  }
 }
 // C2
 GL_HD void fwd_c2_load(uint64_t* v, const uint64_t* lds, int k, int w1) {
@@ -206,28 +267,47 @@ GL_HD void fwd_c2_load(uint64_t* v, const uint64_t* lds, int k, int w1) {
 GL_HD void fwd_c2_compute(uint64_t* v) { dft8<24>(v); }  // v[w0] = Z[64k + w1 + 8 w0]
 
 // ---------------------------------------------------------------- inverse passes
-// IC2: omega^(-w1 r0) with w1 per lane, from ic64[8 w1 + r0].
-GL_HD void inv_c2_compute(uint64_t* v, int w1, const uint64_t* ic64) {
-  idft8(v);
-#pragma unroll
-  for (int r0 = 1; r0 < 8; r0++) v[r0] = gl::mul(v[r0], ic64[8 * w1 + r0]);
-}
+// IC2: inverse DFT-8 over w0 (C2 map); the twiddle omega^(-w1 r0) is applied by IC1.
+GL_HD void inv_c2_compute(uint64_t* v) { idft8(v); }
 GL_HD void inv_c2_store(const uint64_t* v, uint64_t* lds, int k, int w1) {
 #pragma unroll
   for (int r0 = 0; r0 < 8; r0++) lds[idx_u(k, w1, r0)] = v[r0];
 }
-GL_HD void inv_c1_load(uint64_t* v, const uint64_t* lds, int k, int r0) {
+// IC1 (C1 map): omega^(-w1 r0), inverse DFT-8 over w1, then the B untwist
+// itw[(r0 + 8 r1) * 32 + k] (1/N folded in; r-major so that the lanes' k are contiguous).
+template <int NP>
+GL_HD void inv_c1_load(uint64_t (*v)[8], const uint64_t* lds, int t) {
+  const int k = t & 31, h = (t >> 5) & 1, w = t >> 6;
 #pragma unroll
-  for (int w1 = 0; w1 < 8; w1++) v[w1] = lds[idx_u(k, w1, r0)];
+  for (int j = 0; j < NP; j++) {
+    const uint64_t* pl = lds + c1_poly<NP>(j, h) * N;
+    const int r0 = c1_r0<NP>(j, w, h);
+#pragma unroll
+    for (int w1 = 0; w1 < 8; w1++) v[j][w1] = pl[idx_u(k, w1, r0)];
+  }
 }
-GL_HD void inv_c1_compute(uint64_t* v, int k, int r0, const uint64_t* itw) {
-  idft8(v);
+template <int NP>
+GL_HD void inv_c1_compute(uint64_t (*v)[8], int t, const uint64_t* itw, const uint64_t* ic64) {
+  c_twist_all<-1, NP>(v, t, ic64);
+  const int k = t & 31, h = (t >> 5) & 1, w = t >> 6;
 #pragma unroll
-  for (int r1 = 0; r1 < 8; r1++) v[r1] = gl::mul(v[r1], itw[k * 64 + r0 + 8 * r1]);
+  for (int j = 0; j < NP; j++) {
+    idft8(v[j]);
+    const int r0 = c1_r0<NP>(j, w, h);
+#pragma unroll
+    for (int r1 = 0; r1 < 8; r1++) v[j][r1] = gl::mul(v[j][r1], itw[(r0 + 8 * r1) * 32 + k]);
+  }
 }
-GL_HD void inv_c1_store(const uint64_t* v, uint64_t* lds, int k, int r0) {
+template <int NP>
+GL_HD void inv_c1_store(const uint64_t (*v)[8], uint64_t* lds, int t) {
+  const int k = t & 31, h = (t >> 5) & 1, w = t >> 6;
 #pragma unroll
-  for (int r1 = 0; r1 < 8; r1++) lds[idx_t(k, r0 + 8 * r1)] = v[r1];
+  for (int j = 0; j < NP; j++) {
+    uint64_t* pl = lds + c1_poly<NP>(j, h) * N;
+    const int r0 = c1_r0<NP>(j, w, h);
+#pragma unroll
+    for (int r1 = 0; r1 < 8; r1++) pl[idx_t(k, r0 + 8 * r1)] = v[j][r1];
+  }
 }
 GL_HD void inv_a2_load(uint64_t* v, const uint64_t* lds, int r, int q) {
 #pragma unroll
@@ -267,7 +347,7 @@ GL_HD void inv_a1_compute(uint64_t* v) {
 
 // Twiddle tables (host side), one buffer of TABLE_WORDS u64:
 //   [0, 2048)    tw [k*64 + r] = zeta^((2k+1) r)
-//   [2048, 4096) itw[k*64 + r] = zeta^(-(2k+1) r) / 2048
+//   [2048, 4096) itw[r*32 + k] = zeta^(-(2k+1) r) / 2048
 //   [4096, 4160) c64 [8 w1 + r0] = 8^(w1 r0)
 //   [4160, 4224) ic64[8 w1 + r0] = 8^(-w1 r0)
 // zeta = OMG_2_32^(2^20) (sw/ntt/ntt_gf64.sage:44,69-75 power_of_rou(4096, 1)).
@@ -282,7 +362,7 @@ inline void make_twist_tables(uint64_t* tab) {
     for (int r = 0; r < 64; r++) {
       uint64_t e = (uint64_t)(2 * k + 1) * (uint64_t)r;
       tab[TW_OFF + k * 64 + r] = gl::pow(zeta, e);
-      tab[ITW_OFF + k * 64 + r] = gl::mul(gl::pow(izeta, e), inv_n);
+      tab[ITW_OFF + r * 32 + k] = gl::mul(gl::pow(izeta, e), inv_n);
     }
   uint64_t i8 = gl::inv(8);
   for (int w1 = 0; w1 < 8; w1++)

@@ -43,10 +43,10 @@ void emu_ntt_fwd(const uint64_t* in, uint64_t* out) {
   }
   for (int t = 0; t < T; t++) fwd_a2_store(v[t], lds.data(), t & 63, t >> 6);
   for (int t = 0; t < T; t++) {
-    fwd_c1_load(v[t], lds.data(), t >> 3, t & 7);
-    fwd_c1_compute(v[t], t & 7, tb.c64);
+    fwd_c1_load<1>(reinterpret_cast<uint64_t(*)[8]>(v[t]), lds.data(), t);
+    fwd_c1_compute<1>(reinterpret_cast<uint64_t(*)[8]>(v[t]), t, tb.c64);
   }
-  for (int t = 0; t < T; t++) fwd_c1_store(v[t], lds.data(), t >> 3, t & 7);
+  for (int t = 0; t < T; t++) fwd_c1_store<1>(reinterpret_cast<uint64_t(*)[8]>(v[t]), lds.data(), t);
   for (int t = 0; t < T; t++) {
     fwd_c2_load(v[t], lds.data(), t >> 3, t & 7);
     fwd_c2_compute(v[t]);
@@ -63,14 +63,14 @@ void emu_ntt_inv(const uint64_t* in, uint64_t* out) {
   for (int t = 0; t < T; t++) {
     int k = t >> 3, w1 = t & 7;
     for (int w0 = 0; w0 < 8; w0++) v[t][w0] = in[64 * k + w1 + 8 * w0];
-    inv_c2_compute(v[t], w1, tb.ic64);
+    inv_c2_compute(v[t]);
   }
   for (int t = 0; t < T; t++) inv_c2_store(v[t], lds.data(), t >> 3, t & 7);
   for (int t = 0; t < T; t++) {
-    inv_c1_load(v[t], lds.data(), t >> 3, t & 7);
-    inv_c1_compute(v[t], t >> 3, t & 7, tb.itw);
+    inv_c1_load<1>(reinterpret_cast<uint64_t(*)[8]>(v[t]), lds.data(), t);
+    inv_c1_compute<1>(reinterpret_cast<uint64_t(*)[8]>(v[t]), t, tb.itw, tb.ic64);
   }
-  for (int t = 0; t < T; t++) inv_c1_store(v[t], lds.data(), t >> 3, t & 7);
+  for (int t = 0; t < T; t++) inv_c1_store<1>(reinterpret_cast<uint64_t(*)[8]>(v[t]), lds.data(), t);
   for (int t = 0; t < T; t++) {
     inv_a2_load(v[t], lds.data(), t & 63, t >> 6);
     inv_a2_compute(v[t], t >> 6);
@@ -148,13 +148,11 @@ void blind_rotate(int n, const uint64_t* bsk_int, const uint64_t* lut, const uin
       }
     for (int t = 0; t < T; t++)
       for (int p = 0; p < NPF; p++) fwd_a2_store(v[t][p], lds.data() + p * N, t & 63, t >> 6);
-    for (int t = 0; t < T; t++)
-      for (int p = 0; p < NPF; p++) {
-        fwd_c1_load(v[t][p], lds.data() + p * N, t >> 3, t & 7);
-        fwd_c1_compute(v[t][p], t & 7, tb.c64);
-      }
-    for (int t = 0; t < T; t++)
-      for (int p = 0; p < NPF; p++) fwd_c1_store(v[t][p], lds.data() + p * N, t >> 3, t & 7);
+    for (int t = 0; t < T; t++) {
+      fwd_c1_load<NPF>(v[t], lds.data(), t);
+      fwd_c1_compute<NPF>(v[t], t, tb.c64);
+    }
+    for (int t = 0; t < T; t++) fwd_c1_store<NPF>(v[t], lds.data(), t);
     const uint64_t* bsk_i = bsk_int + (size_t)i * 8 * NPF * K1 * T;
     for (int t = 0; t < T; t++) {
       for (int p = 0; p < NPF; p++) {
@@ -162,17 +160,15 @@ void blind_rotate(int n, const uint64_t* bsk_int, const uint64_t* lut, const uin
         fwd_c2_compute(v[t][p]);
       }
       pbs::mac<K1, L>(v[t], bsk_i, t);
-      for (int j = 0; j < K1; j++) inv_c2_compute(v[t][j], t & 7, tb.ic64);
+      for (int j = 0; j < K1; j++) inv_c2_compute(v[t][j]);
     }
     for (int t = 0; t < T; t++)
       for (int j = 0; j < K1; j++) inv_c2_store(v[t][j], lds.data() + j * N, t >> 3, t & 7);
-    for (int t = 0; t < T; t++)
-      for (int j = 0; j < K1; j++) {
-        inv_c1_load(v[t][j], lds.data() + j * N, t >> 3, t & 7);
-        inv_c1_compute(v[t][j], t >> 3, t & 7, tb.itw);
-      }
-    for (int t = 0; t < T; t++)
-      for (int j = 0; j < K1; j++) inv_c1_store(v[t][j], lds.data() + j * N, t >> 3, t & 7);
+    for (int t = 0; t < T; t++) {
+      inv_c1_load<K1>(v[t], lds.data(), t);
+      inv_c1_compute<K1>(v[t], t, tb.itw, tb.ic64);
+    }
+    for (int t = 0; t < T; t++) inv_c1_store<K1>(v[t], lds.data(), t);
     for (int t = 0; t < T; t++)
       for (int j = 0; j < K1; j++) {
         inv_a2_load(v[t][j], lds.data() + j * N, t & 63, t >> 6);
