@@ -189,11 +189,15 @@ GL_HD void fwd_a2_store(const uint64_t* v, uint64_t* lds, int r, int q) {
 #pragma unroll
     for (int k0 = 0; k0 < 4; k0++) lds[idx_t(2 * q + h + 8 * k0, r)] = v[h * 4 + k0];
 }
-// C1 tasks (C1 map above): polynomial and r0 of task j.
+// C1 tasks (C1 map above): polynomial and r0 of task j.  Wave w takes the r0 pair
+// R0_PAIR[w] rather than {2w, 2w+1}: the twiddle cost grows with r0 (r0 = 0 is free), and
+// the pairing evens the per-wave VALU load of the C and A twists (wave 0 has no A1 twist).
+constexpr int R0_PAIR[4][2] = {{3, 4}, {1, 6}, {2, 5}, {0, 7}};
+GL_HD int r0_of(int w, int j) { return (int)((0x70526143u >> (4 * (2 * w + j))) & 15); }  // R0_PAIR[w][j]
 template <int NP>
 GL_HD int c1_poly(int j, int h) { return NP == 1 ? 0 : h + 2 * (j >> 1); }
 template <int NP>
-GL_HD int c1_r0(int j, int w, int h) { return NP == 1 ? 2 * w + h : 2 * w + (j & 1); }
+GL_HD int c1_r0(int j, int w, int h) { return NP == 1 ? r0_of(w, h) : r0_of(w, j & 1); }
 
 // v[w1] *= 8^(S w1 R0) = 2^(3 S w1 R0), w1 = W1..7 (S = 1 forward, -1 inverse).
 template <int S, int R0, int W1 = 1>
@@ -208,9 +212,9 @@ GL_HD void c_twist_wave(uint64_t (*v)[8]) {
 #pragma unroll
   for (int j = 0; j < NP; j++) {
     if (j & 1)
-      c_twist<S, 2 * W + 1>(v[j]);
+      c_twist<S, R0_PAIR[W][1]>(v[j]);
     else
-      c_twist<S, 2 * W>(v[j]);
+      c_twist<S, R0_PAIR[W][0]>(v[j]);
   }
 }
 // Wave-specialised twist of every task (NP >= 2), or the per-lane table (NP = 1).
