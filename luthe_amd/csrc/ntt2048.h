@@ -36,7 +36,10 @@
 // 64-bit accesses with 32 or 64 banks under every map above:
 //   Y1(k1, m0, r) = (4 k1 + m0) * 64 + r
 //   T (k, r)      = 64 k + (r ^ k)
-//   U (k, w1, r0) = 64 k + ((8 w1 + r0) ^ k ^ 4 (w1 >> 2))
+//   U (k, w1, r0) = 64 k + (x ^ 2 (x >> 4) ^ k), x = 8 w1 + r0
+// (checked for lane groups of 16 over 32 banks and of 32 over 64 banks).  k < 32 leaves
+// bit 5 of r alone, so T(k, r + 32) = T(k, r) + 32: the C1 accesses of r1 and r1 + 4 are
+// one base and an offset.
 //
 // Field values stay canonical (gl64.h).  Each pass is split into load / compute / store so that a kernel can put a barrier
 // between all loads and all stores of the same buffer (in-place LDS reuse), and a
@@ -51,7 +54,10 @@ constexpr int T = 256;
 
 GL_HD int idx_y1(int k1, int m0, int r) { return (k1 * 4 + m0) * 64 + r; }
 GL_HD int idx_t(int k, int r) { return k * 64 + (r ^ k); }
-GL_HD int idx_u(int k, int w1, int r0) { return k * 64 + ((8 * w1 + r0) ^ k ^ (4 * (w1 >> 2))); }
+GL_HD int idx_u(int k, int w1, int r0) {
+  const int x = 8 * w1 + r0;
+  return k * 64 + (x ^ (2 * (x >> 4)) ^ k);
+}
 
 // DFT-8 in natural order with root 2^S1 (S1 = 24).
 template <int S1>
@@ -243,7 +249,11 @@ GL_HD void fwd_c1_load(uint64_t (*v)[8], const uint64_t* lds, int t) {
     const uint64_t* pl = lds + c1_poly<NP>(j, h) * N;
     const int r0 = c1_r0<NP>(j, w, h);
 #pragma unroll
-    for (int r1 = 0; r1 < 8; r1++) v[j][r1] = pl[idx_t(k, r0 + 8 * r1)];
+    for (int r1 = 0; r1 < 4; r1++) {
+      const uint64_t* q = pl + idx_t(k, r0 + 8 * r1);
+      v[j][r1] = q[0];
+      v[j][r1 + 4] = q[32];
+    }
   }
 }
 template <int NP>
@@ -310,7 +320,11 @@ GL_HD void inv_c1_store(const uint64_t (*v)[8], uint64_t* lds, int t) {
     uint64_t* pl = lds + c1_poly<NP>(j, h) * N;
     const int r0 = c1_r0<NP>(j, w, h);
 #pragma unroll
-    for (int r1 = 0; r1 < 8; r1++) pl[idx_t(k, r0 + 8 * r1)] = v[j][r1];
+    for (int r1 = 0; r1 < 4; r1++) {
+      uint64_t* q = pl + idx_t(k, r0 + 8 * r1);
+      q[0] = v[j][r1];
+      q[32] = v[j][r1 + 4];
+    }
   }
 }
 GL_HD void inv_a2_load(uint64_t* v, const uint64_t* lds, int r, int q) {
