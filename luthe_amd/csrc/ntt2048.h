@@ -16,10 +16,11 @@
 // cyclic radix-64 column), i.e. the HPU's n5c6 cut, so NTT-domain operands given in
 // the reference order need no permutation.  All twiddles except B are powers of two
 // (omega_64 = 8, sw/ntt/ntt_gf64.sage:35-38): those of the A passes are shifts by
-// compile-time amounts (specialised per wave, whose m0 or k1 they depend on); C's
-// 8^(w1 r0), whose r0 varies per lane, comes from a 64-entry table.  The inverse runs
-// the forward DFT networks with reversed outputs (inverse roots), the inverse twiddles,
-// and folds 1/N into the B twist table.
+// compile-time amounts (specialised per wave, whose m0 or k1 they depend on), and so are
+// C's 8^(w1 r0) whenever a workgroup holds two or more polynomials (C1 map below; with one
+// polynomial they come from a 64-entry table).  The inverse runs the forward DFT networks
+// with reversed outputs (inverse roots), the inverse twiddles, and folds 1/N into the B
+// twist table.
 //
 // Thread mappings (t = thread id in 0..255, wave w = t >> 6):
 //   A map   : r = t & 63, m0 = t >> 6   owns coefficients n = 64 (m0 + 4 m1) + r, m1 < 8
@@ -223,7 +224,7 @@ GL_HD void c_twist_wave(uint64_t (*v)[8]) {
       c_twist<S, R0_PAIR[W][0]>(v[j]);
   }
 }
-// Wave-specialised twist of every task (NP >= 2), or the per-lane table (NP = 1).
+// Wave-specialised twist of every task (NP >= 2), or the per-lane table tab64 (NP = 1).
 template <int S, int NP>
 GL_HD void c_twist_all(uint64_t (*v)[8], int t, const uint64_t* tab64) {
   if constexpr (NP == 1) {
