@@ -24,8 +24,10 @@ constexpr uint64_t EPS = 0xFFFFFFFFull;  // 2^64 mod p
 
 GL_HD uint64_t canonical(uint64_t x) { return x >= P ? x - P : x; }
 
-// a + b (mod p) for canonical a, b (< p); canonical result.  a + b < 2p < 2^65: when
-// the 64-bit sum wrapped or is >= p, a + b - p is the 64-bit sum plus EPS (mod 2^64).
+// a + b (mod p), canonical, for any 64-bit a, b with a + b < 2p: canonical a and b, or the
+// reductions' any-64-bit a plus b <= p - 2^32.  When the 64-bit sum wrapped (it is then
+// a + b - 2^64 < p - EPS) or is >= p, a + b - p is the 64-bit sum plus EPS (mod 2^64),
+// and is < p.
 GL_HD uint64_t add(uint64_t a, uint64_t b) {
   uint64_t s, t;
   bool c1 = __builtin_add_overflow(a, b, &s);
@@ -68,33 +70,25 @@ GL_HD uint64_t sub(uint64_t a, uint64_t b) { return sub_fold(a, b); }
 GL_HD uint64_t neg(uint64_t a) { return sub(0, a); }
 
 // (lo + hi * 2^64) mod p, canonical.  hi * 2^64 = hh * 2^96 + hl * 2^64 == -hh + hl * EPS.
+// The final sum t0 + hl * EPS has t0 < 2^64 and hl * EPS <= p - 2^32, so one add() both
+// folds its carry and canonicalises.
 GL_HD uint64_t reduce128(uint64_t lo, uint64_t hi) {
   uint64_t hh = hi >> 32, hl = hi & EPS;
   uint64_t t0 = sub_fold(lo, hh);  // borrowed 2^64 == EPS; hh < 2^32: cannot wrap again
-  uint64_t t1 = (hl << 32) - hl;   // hl * EPS < p
-  uint64_t r;
-  if (__builtin_add_overflow(t0, t1, &r)) r += EPS;     // carried 2^64 == EPS; cannot wrap again
-  return canonical(r);
+  uint64_t t1 = (hl << 32) - hl;   // hl * EPS <= p - 2^32
+  return add(t0, t1);
 }
 
 // (lo + hl * 2^64) mod p for hl < 2^32, canonical: hl * 2^64 == hl * EPS (the 2^96 part is
-// zero, so no subtraction).  lo + hl * EPS - 2^64 + EPS < 2^64 - 2^32: one fold suffices.
-GL_HD uint64_t reduce96(uint64_t lo, uint64_t hl) {
-  uint64_t t1 = (hl << 32) - hl;
-  uint64_t r;
-  if (__builtin_add_overflow(lo, t1, &r)) r += EPS;
-  return canonical(r);
-}
+// zero, so no subtraction).
+GL_HD uint64_t reduce96(uint64_t lo, uint64_t hl) { return add(lo, (hl << 32) - hl); }
 
 // (lo + hi * 2^64 + c * 2^128) mod p for c < 2^32 - 1, canonical: 2^128 == -2^32, so c
 // joins the 2^96 part of hi as the high word of the subtrahend.
 GL_HD uint64_t reduce160(uint64_t lo, uint64_t hi, uint32_t c) {
   uint64_t hh = (hi >> 32) | ((uint64_t)c << 32), hl = hi & EPS;
   uint64_t t0 = sub_fold(lo, hh);  // hh < 2^64 - 2^32: cannot wrap again
-  uint64_t t1 = (hl << 32) - hl;
-  uint64_t r;
-  if (__builtin_add_overflow(t0, t1, &r)) r += EPS;
-  return canonical(r);
+  return add(t0, (hl << 32) - hl);
 }
 
 GL_HD void mul_full(uint64_t a, uint64_t b, uint64_t& lo, uint64_t& hi) {
