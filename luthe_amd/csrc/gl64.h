@@ -91,6 +91,36 @@ GL_HD uint64_t reduce160(uint64_t lo, uint64_t hi, uint32_t c) {
   return add(t0, (hl << 32) - hl);
 }
 
+// (lo, hi, c) += (plo, phi): a 128-bit add whose carry past 2^128 is counted in c.  On the
+// device one carry chain through the four words into c (five VALU operations, no compares).
+GL_HD void acc128(uint64_t& lo, uint64_t& hi, uint32_t& c, uint64_t plo, uint64_t phi) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint32_t l0 = (uint32_t)lo, l1 = (uint32_t)(lo >> 32), h0 = (uint32_t)hi, h1 = (uint32_t)(hi >> 32);
+  asm("v_add_co_u32 %0, vcc, %0, %5\n\t"
+      "s_nop 1\n\t"
+      "v_addc_co_u32 %1, vcc, %1, %6, vcc\n\t"
+      "s_nop 1\n\t"
+      "v_addc_co_u32 %2, vcc, %2, %7, vcc\n\t"
+      "s_nop 1\n\t"
+      "v_addc_co_u32 %3, vcc, %3, %8, vcc\n\t"
+      "s_nop 1\n\t"
+      "v_addc_co_u32 %4, vcc, %4, 0, vcc"
+      // early-clobber: an input may hold the same value as a word written before it is read
+      : "+&v"(l0), "+&v"(l1), "+&v"(h0), "+&v"(h1), "+&v"(c)
+      : "v"((uint32_t)plo), "v"((uint32_t)(plo >> 32)), "v"((uint32_t)phi), "v"((uint32_t)(phi >> 32))
+      : "vcc");
+  lo = ((uint64_t)l1 << 32) | l0;
+  hi = ((uint64_t)h1 << 32) | h0;
+#else
+  uint64_t nlo, t1, nhi;
+  const bool k1 = __builtin_add_overflow(lo, plo, &nlo);
+  const bool o1 = __builtin_add_overflow(hi, phi, &t1);
+  const bool o2 = __builtin_add_overflow(t1, (uint64_t)k1, &nhi);
+  lo = nlo, hi = nhi;
+  c += (uint32_t)o1 + (uint32_t)o2;
+#endif
+}
+
 GL_HD void mul_full(uint64_t a, uint64_t b, uint64_t& lo, uint64_t& hi) {
 #if defined(__HIP_DEVICE_COMPILE__)
   uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32), b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);

@@ -125,16 +125,10 @@ GL_HD void mac(uint64_t (*z)[8], const uint64_t* bsk_i, int t) {
         uint64_t b = bsk_i[((w0 * (K1 * L) + gl_) * K1 + j) * T + t];
         uint64_t plo, phi;
         gl::mul_full(z[gl_][w0], b, plo, phi);
-        if (gl_ == 0) {
+        if (gl_ == 0)
           lo = plo, hi = phi;
-        } else {
-          uint64_t nlo, t1, nhi;
-          const bool k1 = __builtin_add_overflow(lo, plo, &nlo);
-          const bool o1 = __builtin_add_overflow(hi, phi, &t1);
-          const bool o2 = __builtin_add_overflow(t1, (uint64_t)k1, &nhi);
-          lo = nlo, hi = nhi;
-          c += (uint32_t)o1 + (uint32_t)o2;
-        }
+        else
+          gl::acc128(lo, hi, c, plo, phi);
       }
       o[j] = gl::reduce160(lo, hi, c);
     }

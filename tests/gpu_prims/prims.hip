@@ -8,7 +8,7 @@
 #include "../../luthe_amd/csrc/pbs_core.h"
 
 namespace {
-constexpr int NOUT = 16;  // results per operand triple, in the order listed in prims_run
+constexpr int NOUT = 19;  // results per operand triple, in the order listed in prims_run
 
 __global__ void k_prims(const uint64_t* a, const uint64_t* b, const uint32_t* c, int n, uint64_t* out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -35,10 +35,16 @@ __global__ void k_prims(const uint64_t* a, const uint64_t* b, const uint32_t* c,
   int32_t d[1];
   pbs::decompose_hi<1>((uint32_t)(x >> 32), 23, d);
   o[15] = (uint64_t)(int64_t)d[0];
+  uint64_t lo = x, hi = y;
+  uint32_t cc = z & 3;
+  gl::acc128(lo, hi, cc, y, x);  // (x + y 2^64 + cc 2^128) + (y + x 2^64)
+  o[16] = lo;
+  o[17] = hi;
+  o[18] = cc;
 }
 }  // namespace
 
-// a, b, c: host arrays of n operands; out: host array of n * 16 results.  Returns 0 or the
+// a, b, c: host arrays of n operands; out: host array of n * NOUT results.  Returns 0 or the
 // HIP error code.
 extern "C" int prims_run(const uint64_t* a, const uint64_t* b, const uint32_t* c, int n, uint64_t* out) {
   uint64_t *da = nullptr, *db = nullptr, *dout = nullptr;
