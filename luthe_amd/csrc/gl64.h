@@ -203,6 +203,34 @@ GL_HD uint64_t from_signed32(int32_t d) {
   return d < 0 ? s - EPS : s;
 }
 
+// v mod p for a signed 64-bit v, canonical: (uint64)v = 2^64 + v when negative, and
+// 2^64 + v - EPS = p + v.
+GL_HD uint64_t lift64(int64_t v) {
+  const uint64_t u = (uint64_t)v;
+  return v < 0 ? u - EPS : u;
+}
+
+// d * 2^(12 M1) mod p for a digit |d| <= 2^22, canonical: the forward NTT's pre-twist
+// psi^(4 m1) (psi = 2^3) fused into the digit lift.  Products that fit a signed 64-bit word
+// are lifted directly (2^12, 2^24, 2^36); 2^72 is (d 2^8) * 2^64 == (d 2^8) * EPS, also
+// exact in 64 bits; 2^48, 2^60 and 2^84 are an exact product times a constant shift.
+template <int M1>
+GL_HD uint64_t pretwist_digit(int32_t d) {
+  const int64_t D = d;
+  if constexpr (M1 == 0) return lift64(D);
+  else if constexpr (M1 == 1) return lift64(D * (1ll << 12));
+  else if constexpr (M1 == 2) return lift64(D * (1ll << 24));
+  else if constexpr (M1 == 3) return lift64(D * (1ll << 36));
+  else if constexpr (M1 == 4) return mul_pow2c<24>(lift64(D * (1ll << 24)));
+  else if constexpr (M1 == 5) return mul_pow2c<24>(lift64(D * (1ll << 36)));
+  else if constexpr (M1 == 6) {
+    const int64_t w = D * (1ll << 8);  // |w| <= 2^30: w * EPS fits
+    return lift64(w * (1ll << 32) - w);
+  } else {
+    return mul_pow2c<72>(lift64(D * (1ll << 12)));
+  }
+}
+
 // p -> 2^64 modulus switch of the reference (mod_switch_to_2powerN.sv:60-64,132-135,
 // PRECISION_W = 96 per pe_pbs.sv:38): z = bits[159:96] of e*C + bit 95 (mod 2^64), with
 // C = floor(2^160 / p) = 2^96 + 2^64 - 2.  For e = eh*2^32 + el, e*C + 2^95 over 2^96 is

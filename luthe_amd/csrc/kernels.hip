@@ -23,14 +23,15 @@ constexpr int MAX_GRID_Y = 65535;  // launches split their per-op grid dimension
 // NTT kernels: NP polynomials per 256-thread workgroup.
 // ------------------------------------------------------------------------------------
 // tab: twiddle tables (ntt2048.h TABLE_WORDS layout).
-template <int NP>
+// PRE: the inputs are already pretwisted by psi^(4 m1) (blind-rotation digits, HI path).
+template <int NP, bool PRE = false>
 __device__ __forceinline__ void fwd_passes(uint64_t (*v)[8], uint64_t* lds, const uint64_t* tab, int t) {
   const int r = t & 63, m0 = t >> 6, k = t >> 3, j = t & 7;
   const int m0u = wave_uniform(m0);
   const uint64_t* tw = tab + TW_OFF;
   const uint64_t* c64 = tab + C64_OFF;
 #pragma unroll
-  for (int p = 0; p < NP; p++) fwd_a1_compute(v[p], m0u);
+  for (int p = 0; p < NP; p++) fwd_a1_compute(v[p], m0u, PRE);
 #pragma unroll
   for (int p = 0; p < NP; p++) fwd_a1_store(v[p], lds + p * N, r, m0);
   __syncthreads();
@@ -245,7 +246,8 @@ __global__ __launch_bounds__(256) void k_keyswitch(const uint64_t* __restrict__ 
 // ------------------------------------------------------------------------------------
 // Resident workgroups per CU requested from the register allocator: 32 KB of LDS per
 // workgroup at L = 1 allows 4 (4 waves per SIMD at <= 128 VGPRs); L = 2 needs 64 KB.
-// HI: L * base_log <= 31, the decomposition reads only the high word of each coefficient.
+// HI: L * base_log <= 31 and base_log <= 23: the decomposition reads only the high word of
+// each coefficient and its digit lift applies the forward NTT's pre-twist.
 template <int K1, int L, bool HI>
 __global__ __launch_bounds__(256, (L == 1 ? 4 : 2)) void k_blind_rotate(const uint64_t* __restrict__ bsk,
                                                       const uint64_t* __restrict__ luts,
@@ -287,7 +289,7 @@ __global__ __launch_bounds__(256, (L == 1 ? 4 : 2)) void k_blind_rotate(const ui
 #pragma unroll
       for (int g = 0; g < K1; g++) pbs::rotate_decompose<L, HI>(lds + g * N, acc[g], tt, rot, a.base_log, &v[g * L]);
       __syncthreads();
-      fwd_passes<NPF>(v, lds, tab, tt);
+      fwd_passes<NPF, HI>(v, lds, tab, tt);
       pbs::mac<K1, L>(v, bsk_i, tt);
       inv_passes<K1>(v, lds, tab, tt);
 #pragma unroll
@@ -370,7 +372,7 @@ __global__ __launch_bounds__(512, 1) void k_blind_rotate_lat(const uint64_t* __r
       uint64_t v[L][8];
       pbs::rotate_decompose<L, HI>(lds + h * N, acc, tt, rot, a.base_log, v);
       __syncthreads();
-      fwd_passes<L>(v, lds + h * L * N, tab, tt);
+      fwd_passes<L, HI>(v, lds + h * L * N, tab, tt);
       // exchange the digit NTTs: row gl = g * L + l at [(gl * 8 + w0) * T + t]
       __syncthreads();
 #pragma unroll
@@ -612,7 +614,7 @@ hipError_t launch_blind_rotate(const uint64_t* bsk, const uint64_t* luts, const 
   if (nct == 0) return hipSuccess;
   if (pbs_level != 1 && pbs_level != 2) return hipErrorInvalidValue;
   const bool latency = nct <= latency_max;
-  const bool hi = (uint32_t)pbs_level * a.base_log <= 31;
+  const bool hi = (uint32_t)pbs_level * a.base_log <= 31 && a.base_log <= 23;
   if (pbs_level == 1) {
     if (hi)
       br_launch<1, true>(bsk, luts, lwe2n, jobs, nct, out_base, tab, a, grid_override, s, latency);

@@ -152,14 +152,17 @@ GL_HD int wave_uniform(int x) {
 // A1: in regs v[m1] = p[64 (m0 + 4 m1) + r].  psi^(m0 + 4 m1) Omega^(k1 m0) is split into
 // a compile-time pre-twist psi^(4 m1) = 2^(12 m1) and a post-twist
 // psi^m0 Omega^(k1 m0) = 2^(3 m0 (1 + 2 k1)) that is uniform per wave.
-GL_HD void fwd_a1_compute(uint64_t* v, int m0u) {
-  v[1] = gl::mul_pow2c<12>(v[1]);
-  v[2] = gl::mul_pow2c<24>(v[2]);
-  v[3] = gl::mul_pow2c<36>(v[3]);
-  v[4] = gl::mul_pow2c<48>(v[4]);
-  v[5] = gl::mul_pow2c<60>(v[5]);
-  v[6] = gl::mul_pow2c<72>(v[6]);
-  v[7] = gl::mul_pow2c<84>(v[7]);
+// pretwisted: v already holds p[n] psi^(4 m1) (the blind rotation's digit lift applies it).
+GL_HD void fwd_a1_compute(uint64_t* v, int m0u, bool pretwisted = false) {
+  if (!pretwisted) {
+    v[1] = gl::mul_pow2c<12>(v[1]);
+    v[2] = gl::mul_pow2c<24>(v[2]);
+    v[3] = gl::mul_pow2c<36>(v[3]);
+    v[4] = gl::mul_pow2c<48>(v[4]);
+    v[5] = gl::mul_pow2c<60>(v[5]);
+    v[6] = gl::mul_pow2c<72>(v[6]);
+    v[7] = gl::mul_pow2c<84>(v[7]);
+  }
   dft8<24>(v);
   switch (m0u) {
     case 1: a1_post_twist<1>(v); break;

@@ -82,12 +82,23 @@ GL_HD void decompose_hi(uint32_t xh, uint32_t b, int32_t* d) {
 // and lifted into Z_p: out[l][m1].  acc_lds is the polynomial in plain order.
 // Rotation semantics of pep_mmacc_splitc_feed_read.sv:484-488 and feed_final.sv:203-330:
 // coefficient n reads ACC[(n - a) mod 2N], negated when that index lands in [N, 2N).
-// HI: L * base_log <= 31 (decompose_hi on the high word).
+// HI: L * base_log <= 31 and base_log <= 23 (decompose_hi on the high word; digits
+// |d| <= 2^22, so the lift also applies the forward NTT's pre-twist psi^(4 m1) and the
+// outputs are pretwisted, see fwd_a1_compute).
+template <int L, bool HI, int M1 = 0>
+GL_HD void rotate_decompose_from(const uint64_t* acc_lds, const uint64_t* acc_reg, int t, uint32_t a,
+                                 uint32_t base_log, uint64_t (*out)[8]);
 template <int L, bool HI>
 GL_HD void rotate_decompose(const uint64_t* acc_lds, const uint64_t* acc_reg, int t, uint32_t a,
                             uint32_t base_log, uint64_t (*out)[8]) {
-#pragma unroll
-  for (int m1 = 0; m1 < 8; m1++) {
+  rotate_decompose_from<L, HI, 0>(acc_lds, acc_reg, t, a, base_log, out);
+}
+// One coefficient per recursion step so that m1 is a compile-time constant (the pre-twist).
+template <int L, bool HI, int M1>
+GL_HD void rotate_decompose_from(const uint64_t* acc_lds, const uint64_t* acc_reg, int t, uint32_t a,
+                                 uint32_t base_log, uint64_t (*out)[8]) {
+  if constexpr (M1 < 8) {
+    constexpr int m1 = M1;
     int n = coef_index(t, m1);
     uint32_t idx = ((uint32_t)n + 2 * N - a) & (2 * N - 1);
     uint64_t val = acc_lds[idx & (N - 1)];
@@ -97,13 +108,14 @@ GL_HD void rotate_decompose(const uint64_t* acc_lds, const uint64_t* acc_reg, in
       int32_t d[L];
       decompose_hi<L>((uint32_t)(ct10 >> 32), base_log, d);
 #pragma unroll
-      for (int l = 0; l < L; l++) out[l][m1] = gl::from_signed32(d[l]);
+      for (int l = 0; l < L; l++) out[l][m1] = gl::pretwist_digit<M1>(d[l]);
     } else {
       int64_t d[L];
       decompose<L>(ct10, base_log, d);
 #pragma unroll
       for (int l = 0; l < L; l++) out[l][m1] = gl::from_signed(d[l]);
     }
+    rotate_decompose_from<L, HI, M1 + 1>(acc_lds, acc_reg, t, a, base_log, out);
   }
 }
 

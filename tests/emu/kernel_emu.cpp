@@ -117,6 +117,7 @@ void bsk_to_internal(int n, int K1, int L, const uint64_t* bsk_coef, uint64_t* o
 template <int K1, int L>
 void blind_rotate(int n, const uint64_t* bsk_int, const uint64_t* lut, const uint16_t* a2n, uint32_t base_log,
                   uint64_t* acc_out) {
+  const bool hi = L * base_log <= 31 && base_log <= 23;  // the kernels' HI variant
   constexpr int NPF = K1 * L;
   const Tables& tb = tables();
   std::vector<uint64_t> lds(NPF * N);
@@ -132,13 +133,13 @@ void blind_rotate(int n, const uint64_t* bsk_int, const uint64_t* lut, const uin
         for (int m1 = 0; m1 < 8; m1++) lds[g * N + pbs::coef_index(t, m1)] = acc[t][g][m1];
     for (int t = 0; t < T; t++)
       for (int g = 0; g < K1; g++) {
-        if (L * base_log <= 31)
+        if (hi)
           pbs::rotate_decompose<L, true>(lds.data() + g * N, acc[t][g], t, a, base_log, &v[t][g * L]);
         else
           pbs::rotate_decompose<L, false>(lds.data() + g * N, acc[t][g], t, a, base_log, &v[t][g * L]);
       }
     for (int t = 0; t < T; t++)
-      for (int p = 0; p < NPF; p++) fwd_a1_compute(v[t][p], t >> 6);
+      for (int p = 0; p < NPF; p++) fwd_a1_compute(v[t][p], t >> 6, hi);
     for (int t = 0; t < T; t++)
       for (int p = 0; p < NPF; p++) fwd_a1_store(v[t][p], lds.data() + p * N, t & 63, t >> 6);
     for (int t = 0; t < T; t++)
@@ -228,6 +229,9 @@ int emu_decompose_hi(uint64_t x, uint32_t b, int L, int32_t* d) {
   return 0;
 }
 uint64_t emu_from_signed32(int32_t d) { return gl::from_signed32(d); }
+// pretwist_digit<m1>(d) against mul_pow2c<12 m1>(from_signed32(d)) for every |d| <= 2^22 and
+// every m1; returns the number of mismatches.
+long emu_pretwist_mismatches();
 uint64_t emu_extract_coef(const uint64_t* acc, int g, int m, uint32_t b, uint32_t offs) {
   return pbs::extract_coef(acc, g, m, b, offs);
 }
@@ -244,3 +248,18 @@ void pow2c_all(uint64_t x, uint64_t* out) {
 }  // namespace
 
 extern "C" void emu_gl_mul_pow2c_all(uint64_t x, uint64_t* out) { pow2c_all<-192>(x, out); }
+
+namespace {
+template <int M1>
+long pretwist_check() {
+  long bad = 0;
+  for (int32_t d = -(1 << 22); d <= (1 << 22); d++)
+    bad += gl::pretwist_digit<M1>(d) != gl::mul_pow2c<12 * M1>(gl::from_signed32(d));
+  return bad;
+}
+}  // namespace
+
+extern "C" long emu_pretwist_mismatches() {
+  return pretwist_check<0>() + pretwist_check<1>() + pretwist_check<2>() + pretwist_check<3>() +
+         pretwist_check<4>() + pretwist_check<5>() + pretwist_check<6>() + pretwist_check<7>();
+}

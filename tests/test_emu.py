@@ -106,6 +106,14 @@ def test_kernel_blind_rotation_matches_oracle(emu, L, base_log):
     assert np.array_equal(acc, exp)
 
 
+def test_pretwisted_digit_lift(emu):
+    """The blind rotation lifts each digit already multiplied by the forward NTT's pre-twist
+    2^(12 m1); every digit |d| <= 2^22 and every m1 against lift-then-shift (itself checked
+    against exact integers above)."""
+    emu.emu_pretwist_mismatches.restype = C.c_long
+    assert emu.emu_pretwist_mismatches() == 0
+
+
 def test_kernel_decomposition_and_extract(emu):
     vp = C.c_void_p
     emu.emu_decompose.argtypes = [C.c_uint64, C.c_uint32, C.c_int, vp]
