@@ -67,6 +67,52 @@ GL_HD uint64_t sub_fold(uint64_t a, uint64_t b) {
 
 GL_HD uint64_t sub(uint64_t a, uint64_t b) { return sub_fold(a, b); }
 
+// Four independent subtractions r_i = a_i - b_i (mod p), canonical a_i, b_i: the sub_fold
+// sequence with the four carry chains interleaved, each chain on its own SGPR pairs.  Every
+// carry is read three instructions after it is written, which covers the two wait states
+// without s_nop (sub_fold needs three of them).
+GL_HD void sub4(uint64_t& r0, uint64_t& r1, uint64_t& r2, uint64_t& r3, uint64_t a0, uint64_t a1,
+                uint64_t a2, uint64_t a3, uint64_t b0, uint64_t b1, uint64_t b2, uint64_t b3) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint32_t l0 = (uint32_t)a0, h0 = (uint32_t)(a0 >> 32), l1 = (uint32_t)a1, h1 = (uint32_t)(a1 >> 32);
+  uint32_t l2 = (uint32_t)a2, h2 = (uint32_t)(a2 >> 32), l3 = (uint32_t)a3, h3 = (uint32_t)(a3 >> 32);
+  uint64_t w0, w1, w2, w3, c0, c1, c2, c3;
+  asm("v_sub_co_u32 %0, %8, %0, %16\n\t"
+      "v_sub_co_u32 %2, %9, %2, %18\n\t"
+      "v_sub_co_u32 %4, %10, %4, %20\n\t"
+      "v_sub_co_u32 %6, %11, %6, %22\n\t"
+      "v_subb_co_u32 %1, %8, %1, %17, %8\n\t"
+      "v_subb_co_u32 %3, %9, %3, %19, %9\n\t"
+      "v_subb_co_u32 %5, %10, %5, %21, %10\n\t"
+      "v_subb_co_u32 %7, %11, %7, %23, %11\n\t"
+      "v_addc_co_u32 %0, %12, %0, 0, %8\n\t"
+      "v_addc_co_u32 %2, %13, %2, 0, %9\n\t"
+      "v_addc_co_u32 %4, %14, %4, 0, %10\n\t"
+      "v_addc_co_u32 %6, %15, %6, 0, %11\n\t"
+      "v_addc_co_u32 %1, %12, %1, 0, %12\n\t"
+      "v_addc_co_u32 %3, %13, %3, 0, %13\n\t"
+      "v_addc_co_u32 %5, %14, %5, 0, %14\n\t"
+      "v_addc_co_u32 %7, %15, %7, 0, %15\n\t"
+      "v_subb_co_u32 %1, %8, %1, 0, %8\n\t"
+      "v_subb_co_u32 %3, %9, %3, 0, %9\n\t"
+      "v_subb_co_u32 %5, %10, %5, 0, %10\n\t"
+      "v_subb_co_u32 %7, %11, %7, 0, %11"
+      : "+&v"(l0), "+&v"(h0), "+&v"(l1), "+&v"(h1), "+&v"(l2), "+&v"(h2), "+&v"(l3), "+&v"(h3),
+        "=&s"(w0), "=&s"(w1), "=&s"(w2), "=&s"(w3), "=&s"(c0), "=&s"(c1), "=&s"(c2), "=&s"(c3)
+      : "v"((uint32_t)b0), "v"((uint32_t)(b0 >> 32)), "v"((uint32_t)b1), "v"((uint32_t)(b1 >> 32)),
+        "v"((uint32_t)b2), "v"((uint32_t)(b2 >> 32)), "v"((uint32_t)b3), "v"((uint32_t)(b3 >> 32)));
+  r0 = ((uint64_t)h0 << 32) | l0;
+  r1 = ((uint64_t)h1 << 32) | l1;
+  r2 = ((uint64_t)h2 << 32) | l2;
+  r3 = ((uint64_t)h3 << 32) | l3;
+#else
+  r0 = sub_fold(a0, b0);
+  r1 = sub_fold(a1, b1);
+  r2 = sub_fold(a2, b2);
+  r3 = sub_fold(a3, b3);
+#endif
+}
+
 GL_HD uint64_t neg(uint64_t a) { return sub(0, a); }
 
 // (lo + hi * 2^64) mod p, canonical.  hi * 2^64 = hh * 2^96 + hl * 2^64 == -hh + hl * EPS.

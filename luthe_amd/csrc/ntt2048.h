@@ -60,39 +60,49 @@ GL_HD int idx_u(int k, int w1, int r0) {
   return k * 64 + (x ^ (2 * (x >> 4)) ^ k);
 }
 
-// DFT-8 in natural order with root 2^S1 (S1 = 24).
+// DFT-8 in natural order with root 2^S1 (S1 = 24).  Each radix-2 stage forms its four sums
+// first, then its four differences in one gl::sub4 (the operands are dead after it).
 template <int S1>
 GL_HD void dft8(uint64_t* x) {
-  uint64_t a, b;
+  uint64_t s0, s1, s2, s3, d0, d1, d2, d3;
   // stage 1: pairs (j, j+4), twiddle root^j
-  a = x[0]; b = x[4]; x[0] = gl::add(a, b); x[4] = gl::sub(a, b);
-  a = x[1]; b = x[5]; x[1] = gl::add(a, b); x[5] = gl::mul_pow2c<S1>(gl::sub(a, b));
-  a = x[2]; b = x[6]; x[2] = gl::add(a, b); x[6] = gl::mul_pow2c<2 * S1>(gl::sub(a, b));
-  a = x[3]; b = x[7]; x[3] = gl::add(a, b); x[7] = gl::mul_pow2c<3 * S1>(gl::sub(a, b));
+  s0 = gl::add(x[0], x[4]); s1 = gl::add(x[1], x[5]); s2 = gl::add(x[2], x[6]); s3 = gl::add(x[3], x[7]);
+  gl::sub4(d0, d1, d2, d3, x[0], x[1], x[2], x[3], x[4], x[5], x[6], x[7]);
+  x[0] = s0; x[1] = s1; x[2] = s2; x[3] = s3;
+  x[4] = d0;
+  x[5] = gl::mul_pow2c<S1>(d1);
+  x[6] = gl::mul_pow2c<2 * S1>(d2);
+  x[7] = gl::mul_pow2c<3 * S1>(d3);
   // stage 2: pairs (h+j, h+j+2), twiddle root^(2j)
-  a = x[0]; b = x[2]; x[0] = gl::add(a, b); x[2] = gl::sub(a, b);
-  a = x[1]; b = x[3]; x[1] = gl::add(a, b); x[3] = gl::mul_pow2c<2 * S1>(gl::sub(a, b));
-  a = x[4]; b = x[6]; x[4] = gl::add(a, b); x[6] = gl::sub(a, b);
-  a = x[5]; b = x[7]; x[5] = gl::add(a, b); x[7] = gl::mul_pow2c<2 * S1>(gl::sub(a, b));
-  // stage 3
-  a = x[0]; b = x[1]; x[0] = gl::add(a, b); x[1] = gl::sub(a, b);
-  a = x[2]; b = x[3]; x[2] = gl::add(a, b); x[3] = gl::sub(a, b);
-  a = x[4]; b = x[5]; x[4] = gl::add(a, b); x[5] = gl::sub(a, b);
-  a = x[6]; b = x[7]; x[6] = gl::add(a, b); x[7] = gl::sub(a, b);
-  // bit-reversed -> natural
-  a = x[1]; x[1] = x[4]; x[4] = a;
-  a = x[3]; x[3] = x[6]; x[6] = a;
+  s0 = gl::add(x[0], x[2]); s1 = gl::add(x[1], x[3]); s2 = gl::add(x[4], x[6]); s3 = gl::add(x[5], x[7]);
+  gl::sub4(d0, d1, d2, d3, x[0], x[1], x[4], x[5], x[2], x[3], x[6], x[7]);
+  x[0] = s0; x[1] = s1; x[4] = s2; x[5] = s3;
+  x[2] = d0;
+  x[3] = gl::mul_pow2c<2 * S1>(d1);
+  x[6] = d2;
+  x[7] = gl::mul_pow2c<2 * S1>(d3);
+  // stage 3: pairs (2j, 2j+1); the outputs land bit-reversed (1 <-> 4, 3 <-> 6), which
+  // the assignment below undoes
+  s0 = gl::add(x[0], x[1]); s1 = gl::add(x[2], x[3]); s2 = gl::add(x[4], x[5]); s3 = gl::add(x[6], x[7]);
+  gl::sub4(d0, d1, d2, d3, x[0], x[2], x[4], x[6], x[1], x[3], x[5], x[7]);
+  x[0] = s0; x[4] = d0; x[2] = s1; x[6] = d1; x[1] = s2; x[5] = d2; x[3] = s3; x[7] = d3;
 }
 
-// DFT-4 in natural order with root 2^S2 (S2 = 48).
+// Two DFT-4s in natural order with root 2^S2 (S2 = 48), over x[0..3] and x[4..7]; each of
+// the two radix-2 stages takes its four differences in one gl::sub4.
 template <int S2>
-GL_HD void dft4(uint64_t* x) {
-  uint64_t a0 = gl::add(x[0], x[2]), a1 = gl::sub(x[0], x[2]);
-  uint64_t b0 = gl::add(x[1], x[3]), b1 = gl::mul_pow2c<S2>(gl::sub(x[1], x[3]));
+GL_HD void dft4x2(uint64_t* x) {
+  uint64_t a0 = gl::add(x[0], x[2]), b0 = gl::add(x[1], x[3]);
+  uint64_t c0 = gl::add(x[4], x[6]), e0 = gl::add(x[5], x[7]);
+  uint64_t a1, b1, c1, e1;
+  gl::sub4(a1, b1, c1, e1, x[0], x[1], x[4], x[5], x[2], x[3], x[6], x[7]);
+  b1 = gl::mul_pow2c<S2>(b1);
+  e1 = gl::mul_pow2c<S2>(e1);
   x[0] = gl::add(a0, b0);
-  x[2] = gl::sub(a0, b0);
   x[1] = gl::add(a1, b1);
-  x[3] = gl::sub(a1, b1);
+  x[4] = gl::add(c0, e0);
+  x[5] = gl::add(c1, e1);
+  gl::sub4(x[2], x[3], x[6], x[7], a0, a1, c0, c1, b0, b1, e0, e1);
 }
 
 // Inverse DFT-8 / DFT-4 (roots 2^-24, 2^-48): sum_j x[j] w^(-jk) is output (n - k) mod n
@@ -109,9 +119,10 @@ GL_HD void idft8(uint64_t* x) {
   swap_(x[2], x[6]);
   swap_(x[3], x[5]);
 }
-GL_HD void idft4(uint64_t* x) {
-  dft4<48>(x);
+GL_HD void idft4x2(uint64_t* x) {
+  dft4x2<48>(x);
   swap_(x[1], x[3]);
+  swap_(x[5], x[7]);
 }
 
 // Twists by compile-time powers of two over registers K..7 / K..3 (template recursion keeps
@@ -183,9 +194,9 @@ GL_HD void fwd_a2_load(uint64_t* v, const uint64_t* lds, int r, int q) {
     for (int m0 = 0; m0 < 4; m0++) v[h * 4 + m0] = lds[idx_y1(2 * q + h, m0, r)];
 }
 GL_HD void fwd_a2_compute(uint64_t* v, int r, int q, const uint64_t* tw) {
+  dft4x2<48>(v);
 #pragma unroll
   for (int h = 0; h < 2; h++) {
-    dft4<48>(v + h * 4);
 #pragma unroll
     for (int k0 = 0; k0 < 4; k0++) {
       int k = 2 * q + h + 8 * k0;
@@ -341,8 +352,7 @@ GL_HD void inv_a2_load(uint64_t* v, const uint64_t* lds, int r, int q) {
 // The psi^(-m0) part of the final untwist psi^-(m0 + 4 m1) is constant over IA1's
 // DFT-8 and moves here; both stages carry a factor -1 (the cheaper shift forms).
 GL_HD void inv_a2_compute(uint64_t* v, int qu) {
-  idft4(v);
-  idft4(v + 4);
+  idft4x2(v);
   switch (qu) {
     case 0: ia2_twist<0>(v); ia2_twist<1>(v + 4); break;
     case 1: ia2_twist<2>(v); ia2_twist<3>(v + 4); break;

@@ -8,7 +8,7 @@
 #include "../../luthe_amd/csrc/pbs_core.h"
 
 namespace {
-constexpr int NOUT = 19;  // results per operand triple, in the order listed in prims_run
+constexpr int NOUT = 23;  // results per operand triple, in the order listed in prims_run
 
 __global__ void k_prims(const uint64_t* a, const uint64_t* b, const uint32_t* c, int n, uint64_t* out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -41,6 +41,8 @@ __global__ void k_prims(const uint64_t* a, const uint64_t* b, const uint32_t* c,
   o[16] = lo;
   o[17] = hi;
   o[18] = cc;
+  // four interleaved subtractions with operands crossed so that every lane differs
+  gl::sub4(o[19], o[20], o[21], o[22], xc, yc, xc, gl::canonical(x ^ y), yc, xc, gl::canonical(~x), 0);
 }
 }  // namespace
 
