@@ -67,24 +67,23 @@ GL_HD uint64_t sub_fold(uint64_t a, uint64_t b) {
 
 GL_HD uint64_t sub(uint64_t a, uint64_t b) { return sub_fold(a, b); }
 
-// Four independent subtractions r_i = a_i - b_i (mod p), canonical a_i, b_i: the sub_fold
-// sequence with the four carry chains interleaved, each chain on its own SGPR pairs.  Every
-// carry is read three instructions after it is written, which covers the two wait states
-// without s_nop (sub_fold needs three of them).
+// Four independent r_i = sub_fold(a_i, b_i) (a_i - b_i (mod p) for canonical operands): the
+// sub_fold sequence with the four carry chains interleaved, each chain on its own SGPR pairs.
+// Every carry is read three instructions after it is written, which covers the two wait
+// states without s_nop (sub_fold needs three of them).
 GL_HD void sub4(uint64_t& r0, uint64_t& r1, uint64_t& r2, uint64_t& r3, uint64_t a0, uint64_t a1,
                 uint64_t a2, uint64_t a3, uint64_t b0, uint64_t b1, uint64_t b2, uint64_t b3) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  uint32_t l0 = (uint32_t)a0, h0 = (uint32_t)(a0 >> 32), l1 = (uint32_t)a1, h1 = (uint32_t)(a1 >> 32);
-  uint32_t l2 = (uint32_t)a2, h2 = (uint32_t)(a2 >> 32), l3 = (uint32_t)a3, h3 = (uint32_t)(a3 >> 32);
+  uint32_t l0, h0, l1, h1, l2, h2, l3, h3;
   uint64_t w0, w1, w2, w3, c0, c1, c2, c3;
-  asm("v_sub_co_u32 %0, %8, %0, %16\n\t"
-      "v_sub_co_u32 %2, %9, %2, %18\n\t"
-      "v_sub_co_u32 %4, %10, %4, %20\n\t"
-      "v_sub_co_u32 %6, %11, %6, %22\n\t"
-      "v_subb_co_u32 %1, %8, %1, %17, %8\n\t"
-      "v_subb_co_u32 %3, %9, %3, %19, %9\n\t"
-      "v_subb_co_u32 %5, %10, %5, %21, %10\n\t"
-      "v_subb_co_u32 %7, %11, %7, %23, %11\n\t"
+  asm("v_sub_co_u32 %0, %8, %16, %24\n\t"
+      "v_sub_co_u32 %2, %9, %18, %26\n\t"
+      "v_sub_co_u32 %4, %10, %20, %28\n\t"
+      "v_sub_co_u32 %6, %11, %22, %30\n\t"
+      "v_subb_co_u32 %1, %8, %17, %25, %8\n\t"
+      "v_subb_co_u32 %3, %9, %19, %27, %9\n\t"
+      "v_subb_co_u32 %5, %10, %21, %29, %10\n\t"
+      "v_subb_co_u32 %7, %11, %23, %31, %11\n\t"
       "v_addc_co_u32 %0, %12, %0, 0, %8\n\t"
       "v_addc_co_u32 %2, %13, %2, 0, %9\n\t"
       "v_addc_co_u32 %4, %14, %4, 0, %10\n\t"
@@ -97,9 +96,11 @@ GL_HD void sub4(uint64_t& r0, uint64_t& r1, uint64_t& r2, uint64_t& r3, uint64_t
       "v_subb_co_u32 %3, %9, %3, 0, %9\n\t"
       "v_subb_co_u32 %5, %10, %5, 0, %10\n\t"
       "v_subb_co_u32 %7, %11, %7, 0, %11"
-      : "+&v"(l0), "+&v"(h0), "+&v"(l1), "+&v"(h1), "+&v"(l2), "+&v"(h2), "+&v"(l3), "+&v"(h3),
+      : "=&v"(l0), "=&v"(h0), "=&v"(l1), "=&v"(h1), "=&v"(l2), "=&v"(h2), "=&v"(l3), "=&v"(h3),
         "=&s"(w0), "=&s"(w1), "=&s"(w2), "=&s"(w3), "=&s"(c0), "=&s"(c1), "=&s"(c2), "=&s"(c3)
-      : "v"((uint32_t)b0), "v"((uint32_t)(b0 >> 32)), "v"((uint32_t)b1), "v"((uint32_t)(b1 >> 32)),
+      : "v"((uint32_t)a0), "v"((uint32_t)(a0 >> 32)), "v"((uint32_t)a1), "v"((uint32_t)(a1 >> 32)),
+        "v"((uint32_t)a2), "v"((uint32_t)(a2 >> 32)), "v"((uint32_t)a3), "v"((uint32_t)(a3 >> 32)),
+        "v"((uint32_t)b0), "v"((uint32_t)(b0 >> 32)), "v"((uint32_t)b1), "v"((uint32_t)(b1 >> 32)),
         "v"((uint32_t)b2), "v"((uint32_t)(b2 >> 32)), "v"((uint32_t)b3), "v"((uint32_t)(b3 >> 32)));
   r0 = ((uint64_t)h0 << 32) | l0;
   r1 = ((uint64_t)h1 << 32) | l1;
@@ -167,6 +168,31 @@ GL_HD void acc128(uint64_t& lo, uint64_t& hi, uint32_t& c, uint64_t plo, uint64_
 #endif
 }
 
+// reduce128 over four lanes, the four borrow folds in one sub4.
+GL_HD void reduce128x4(uint64_t* r, const uint64_t* lo, const uint64_t* hi) {
+  uint64_t t[4];
+  sub4(t[0], t[1], t[2], t[3], lo[0], lo[1], lo[2], lo[3], hi[0] >> 32, hi[1] >> 32, hi[2] >> 32,
+       hi[3] >> 32);
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const uint64_t hl = hi[i] & EPS;
+    r[i] = add(t[i], (hl << 32) - hl);
+  }
+}
+
+// reduce160 over four lanes (c_i < 2^32 - 1), the four borrow folds in one sub4.
+GL_HD void reduce160x4(uint64_t* r, const uint64_t* lo, const uint64_t* hi, const uint32_t* c) {
+  uint64_t t[4], hh[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) hh[i] = (hi[i] >> 32) | ((uint64_t)c[i] << 32);
+  sub4(t[0], t[1], t[2], t[3], lo[0], lo[1], lo[2], lo[3], hh[0], hh[1], hh[2], hh[3]);
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const uint64_t hl = hi[i] & EPS;
+    r[i] = add(t[i], (hl << 32) - hl);
+  }
+}
+
 GL_HD void mul_full(uint64_t a, uint64_t b, uint64_t& lo, uint64_t& hi) {
 #if defined(__HIP_DEVICE_COMPILE__)
   uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32), b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
@@ -187,6 +213,14 @@ GL_HD uint64_t mul(uint64_t a, uint64_t b) {
   uint64_t lo, hi;
   mul_full(a, b, lo, hi);
   return reduce128(lo, hi);
+}
+
+// r_i = a_i * b_i mod p for four lanes (r may alias a or b).
+GL_HD void mul4(uint64_t* r, const uint64_t* a, const uint64_t* b) {
+  uint64_t lo[4], hi[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) mul_full(a[i], b[i], lo[i], hi[i]);
+  reduce128x4(r, lo, hi);
 }
 
 // x * 2^s mod p for 0 <= s < 96 and any 64-bit x; canonical result.

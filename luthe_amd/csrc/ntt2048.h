@@ -197,11 +197,10 @@ GL_HD void fwd_a2_compute(uint64_t* v, int r, int q, const uint64_t* tw) {
   dft4x2<48>(v);
 #pragma unroll
   for (int h = 0; h < 2; h++) {
+    uint64_t w[4];
 #pragma unroll
-    for (int k0 = 0; k0 < 4; k0++) {
-      int k = 2 * q + h + 8 * k0;
-      v[h * 4 + k0] = gl::mul(v[h * 4 + k0], tw[k * 64 + r]);
-    }
+    for (int k0 = 0; k0 < 4; k0++) w[k0] = tw[(2 * q + h + 8 * k0) * 64 + r];
+    gl::mul4(v + h * 4, v + h * 4, w);
   }
 }
 GL_HD void fwd_a2_store(const uint64_t* v, uint64_t* lds, int r, int q) {
@@ -323,8 +322,11 @@ GL_HD void inv_c1_compute(uint64_t (*v)[8], int t, const uint64_t* itw, const ui
   for (int j = 0; j < NP; j++) {
     idft8(v[j]);
     const int r0 = c1_r0<NP>(j, w, h);
+    uint64_t w[8];
 #pragma unroll
-    for (int r1 = 0; r1 < 8; r1++) v[j][r1] = gl::mul(v[j][r1], itw[(r0 + 8 * r1) * 32 + k]);
+    for (int r1 = 0; r1 < 8; r1++) w[r1] = itw[(r0 + 8 * r1) * 32 + k];
+    gl::mul4(v[j], v[j], w);
+    gl::mul4(v[j] + 4, v[j] + 4, w + 4);
   }
 }
 template <int NP>

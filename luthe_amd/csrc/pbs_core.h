@@ -124,28 +124,33 @@ GL_HD void rotate_decompose_from(const uint64_t* acc_lds, const uint64_t* acc_re
 // [w0][K1*L][K1][T].  Result written over z[j][w0] for j < K1.
 template <int K1, int L>
 GL_HD void mac(uint64_t (*z)[8], const uint64_t* bsk_i, int t) {
+  // Outputs in groups of four, (w0, j) with j fastest; a group covers whole w0 columns, so
+  // every input z[.][w0] of the group is read before its outputs are written over it.
+  static_assert(4 % K1 == 0, "mac groups four outputs over whole w0 columns");
+  const uint64_t* bt = bsk_i + t;
 #pragma unroll
-  for (int w0 = 0; w0 < 8; w0++) {
-    uint64_t o[K1];
+  for (int q = 0; q < 8 * K1; q += 4) {
+    uint64_t lo[4], hi[4], o[4];
+    uint32_t c[4];
 #pragma unroll
-    for (int j = 0; j < K1; j++) {
+    for (int u = 0; u < 4; u++) {
+      const int w0 = (q + u) / K1, j = (q + u) % K1;
       // sum of K1*L full 128-bit products, carries past 2^128 counted in c, one reduction
-      uint64_t lo = 0, hi = 0;
-      uint32_t c = 0;
+      c[u] = 0;
 #pragma unroll
       for (int gl_ = 0; gl_ < K1 * L; gl_++) {
-        uint64_t b = bsk_i[((w0 * (K1 * L) + gl_) * K1 + j) * T + t];
+        uint64_t b = bt[(size_t)((w0 * (K1 * L) + gl_) * K1 + j) * T];
         uint64_t plo, phi;
         gl::mul_full(z[gl_][w0], b, plo, phi);
         if (gl_ == 0)
-          lo = plo, hi = phi;
+          lo[u] = plo, hi[u] = phi;
         else
-          gl::acc128(lo, hi, c, plo, phi);
+          gl::acc128(lo[u], hi[u], c[u], plo, phi);
       }
-      o[j] = gl::reduce160(lo, hi, c);
     }
+    gl::reduce160x4(o, lo, hi, c);
 #pragma unroll
-    for (int j = 0; j < K1; j++) z[j][w0] = o[j];
+    for (int u = 0; u < 4; u++) z[(q + u) % K1][(q + u) / K1] = o[u];
   }
 }
 

@@ -8,7 +8,7 @@
 #include "../../luthe_amd/csrc/pbs_core.h"
 
 namespace {
-constexpr int NOUT = 23;  // results per operand triple, in the order listed in prims_run
+constexpr int NOUT = 31;  // results per operand triple, in the order listed in prims_run
 
 __global__ void k_prims(const uint64_t* a, const uint64_t* b, const uint32_t* c, int n, uint64_t* out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -43,6 +43,13 @@ __global__ void k_prims(const uint64_t* a, const uint64_t* b, const uint32_t* c,
   o[18] = cc;
   // four interleaved subtractions with operands crossed so that every lane differs
   gl::sub4(o[19], o[20], o[21], o[22], xc, yc, xc, gl::canonical(x ^ y), yc, xc, gl::canonical(~x), 0);
+  // four-lane products and 160-bit reductions
+  const uint64_t ma[4] = {x, y, x ^ y, ~x}, mb[4] = {y, x, y, x};
+  gl::mul4(o + 23, ma, mb);
+  const uint32_t zc = z == 0xFFFFFFFFu ? 0 : z;
+  const uint64_t rl[4] = {x, y, x, y}, rh[4] = {y, x, x, ~y};
+  const uint32_t rc[4] = {zc, zc >> 1, 0, 0xFFFFFFFEu};
+  gl::reduce160x4(o + 27, rl, rh, rc);
 }
 }  // namespace
 
