@@ -27,12 +27,13 @@ GL_HD uint64_t canonical(uint64_t x) { return x >= P ? x - P : x; }
 // a + b (mod p), canonical, for any 64-bit a, b with a + b < 2p: canonical a and b, or the
 // reductions' any-64-bit a plus b <= p - 2^32.  When the 64-bit sum wrapped (it is then
 // a + b - 2^64 < p - EPS) or is >= p, a + b - p is the 64-bit sum plus EPS (mod 2^64),
-// and is < p.
+// and is < p.  The select picks the 32-bit addend (EPS or 0), not one of two 64-bit sums:
+// one 32-bit select per add instead of two.
 GL_HD uint64_t add(uint64_t a, uint64_t b) {
-  uint64_t s, t;
-  bool c1 = __builtin_add_overflow(a, b, &s);
-  bool c2 = __builtin_add_overflow(s, EPS, &t);
-  return (c1 | c2) ? t : s;
+  uint64_t s;
+  const bool wrapped = __builtin_add_overflow(a, b, &s);
+  const uint32_t x = (wrapped | (s > P - 1)) ? 0xFFFFFFFFu : 0u;
+  return s + (uint64_t)x;
 }
 
 // a - b with the borrow folded back as 2^64 == EPS: (a - b) mod 2^64, minus EPS when the

@@ -261,6 +261,11 @@ __global__ __launch_bounds__(256, (L == 1 ? 4 : 2)) void k_blind_rotate(const ui
   const int r = t & 63, m0 = t >> 6;
   const size_t blen = (size_t)(K1 - 1) * N + 1;
   const size_t bsk_stride = (size_t)8 * NPF * K1 * T;
+  // The key through a buffer resource (its size in bytes, below 2^31, checked at launch):
+  // each key word is read at the lane offset 8 t plus a scalar offset, so the MAC's loads
+  // take no vector address arithmetic.
+  const __amdgpu_buffer_rsrc_t bsk_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)bsk, (short)0, (int)((size_t)a.n_lwe * bsk_stride * 8), 0x00020000);
   for (int c = blockIdx.x; c < nct; c += gridDim.x) {
     const PbsJob job = jobs[c];
     const uint16_t* a2n = lwe2n + (size_t)c * a.lwe_stride;
@@ -273,7 +278,6 @@ __global__ __launch_bounds__(256, (L == 1 ? 4 : 2)) void k_blind_rotate(const ui
     uint64_t v[NPF][8];
     for (int i = 0; i < a.n_lwe; i++) {
       const uint32_t rot = a2n[i];
-      const uint64_t* bsk_i = bsk + (size_t)i * bsk_stride;
       // Opaque copy of the thread id: keeps the compiler from hoisting every derived
       // shift amount, twiddle and LDS address out of the CMUX loop (which would hold
       // hundreds of registers live); they are recomputed per pass instead.
@@ -290,7 +294,12 @@ __global__ __launch_bounds__(256, (L == 1 ? 4 : 2)) void k_blind_rotate(const ui
       for (int g = 0; g < K1; g++) pbs::rotate_decompose<L, HI>(lds + g * N, acc[g], tt, rot, a.base_log, &v[g * L]);
       __syncthreads();
       fwd_passes<NPF, HI>(v, lds, tab, tt);
-      pbs::mac<K1, L>(v, bsk_i, tt);
+      const int slice_off = (int)((size_t)i * bsk_stride * 8);
+      const int lane_off = tt * 8;
+      pbs::mac_with<K1, L>(v, [&](int e) {
+        const auto w = __builtin_amdgcn_raw_buffer_load_b64(bsk_rsrc, lane_off, slice_off + e * T * 8, 0);
+        return ((uint64_t)w[1] << 32) | w[0];
+      });
       inv_passes<K1>(v, lds, tab, tt);
 #pragma unroll
       for (int g = 0; g < K1; g++)
@@ -613,6 +622,8 @@ hipError_t launch_blind_rotate(const uint64_t* bsk, const uint64_t* luts, const 
                                int pbs_level, int grid_override, hipStream_t s, int latency_max) {
   if (nct == 0) return hipSuccess;
   if (pbs_level != 1 && pbs_level != 2) return hipErrorInvalidValue;
+  // the batched kernel addresses the key through a buffer resource of at most 2^31 - 1 bytes
+  if ((size_t)a.n_lwe * 8 * (2 * pbs_level) * 2 * T * 8 > 0x7FFFFFFFull) return hipErrorInvalidValue;
   const bool latency = nct <= latency_max;
   const bool hi = (uint32_t)pbs_level * a.base_log <= 31 && a.base_log <= 23;
   if (pbs_level == 1) {

@@ -120,26 +120,32 @@ GL_HD void rotate_decompose_from(const uint64_t* acc_lds, const uint64_t* acc_re
 }
 
 // Pointwise external product for one thread: z[gl][w0] are the NTT-domain digits of
-// row gl = g*L + l; bsk_i is the key slice of this CMUX in the internal layout
-// [w0][K1*L][K1][T].  Result written over z[j][w0] for j < K1.
-template <int K1, int L>
-GL_HD void mac(uint64_t (*z)[8], const uint64_t* bsk_i, int t) {
+// row gl = g*L + l; the key slice of this CMUX is in the internal layout [w0][K1*L][K1][T],
+// and key(e) returns this thread's word e*T + t of it.  Result written over z[j][w0] for
+// j < K1.
+template <int K1, int L, class KeyWord>
+GL_HD void mac_with(uint64_t (*z)[8], KeyWord key) {
   // Outputs in groups of four, (w0, j) with j fastest; a group covers whole w0 columns, so
   // every input z[.][w0] of the group is read before its outputs are written over it.
   static_assert(4 % K1 == 0, "mac groups four outputs over whole w0 columns");
-  const uint64_t* bt = bsk_i + t;
 #pragma unroll
   for (int q = 0; q < 8 * K1; q += 4) {
     uint64_t lo[4], hi[4], o[4];
     uint32_t c[4];
+    // the group's key words first, so that their loads are in flight together
+    uint64_t kw[4][K1 * L];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int gl_ = 0; gl_ < K1 * L; gl_++) kw[u][gl_] = key((((q + u) / K1) * (K1 * L) + gl_) * K1 + (q + u) % K1);
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-      const int w0 = (q + u) / K1, j = (q + u) % K1;
+      const int w0 = (q + u) / K1;
       // sum of K1*L full 128-bit products, carries past 2^128 counted in c, one reduction
       c[u] = 0;
 #pragma unroll
       for (int gl_ = 0; gl_ < K1 * L; gl_++) {
-        uint64_t b = bt[(size_t)((w0 * (K1 * L) + gl_) * K1 + j) * T];
+        const uint64_t b = kw[u][gl_];
         uint64_t plo, phi;
         gl::mul_full(z[gl_][w0], b, plo, phi);
         if (gl_ == 0)
@@ -152,6 +158,13 @@ GL_HD void mac(uint64_t (*z)[8], const uint64_t* bsk_i, int t) {
 #pragma unroll
     for (int u = 0; u < 4; u++) z[(q + u) % K1][(q + u) / K1] = o[u];
   }
+}
+
+// mac_with over a plain pointer to the key slice of this CMUX (bsk_i).
+template <int K1, int L>
+GL_HD void mac(uint64_t (*z)[8], const uint64_t* bsk_i, int t) {
+  const uint64_t* bt = bsk_i + t;
+  mac_with<K1, L>(z, [bt](int e) { return bt[(size_t)e * T]; });
 }
 
 // Sample extract of one output coefficient with Many-LUT
