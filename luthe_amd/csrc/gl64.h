@@ -36,6 +36,30 @@ GL_HD uint64_t add(uint64_t a, uint64_t b) {
   return s + (uint64_t)x;
 }
 
+// t0 + hl * EPS (mod p), canonical, for any 64-bit t0 and hl < 2^32 (the tail of every
+// reduction: hl * EPS <= p - 2^32, so the sum is < 2p and add()'s rule applies).  On the
+// device the product and the sum are one v_mad_u64_u32, whose carry-out is the wrap flag;
+// the addend EPS or 0 is then added by a second v_mad_u64_u32 (x * 1 + s).  One wait state
+// separates the SALU-written select mask from its reader.
+GL_HD uint64_t add_eps_prod(uint64_t t0, uint32_t hl) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint64_t s, r, c, m, d;
+  uint32_t x;
+  asm("v_mad_u64_u32 %0, %3, %6, -1, %7\n\t"
+      "v_cmp_lt_u64_e64 %4, %8, %0\n\t"
+      "s_or_b64 %4, %4, %3\n\t"
+      "s_nop 0\n\t"
+      "v_cndmask_b32_e64 %2, 0, -1, %4\n\t"
+      "v_mad_u64_u32 %1, %5, %2, 1, %0"
+      : "=&v"(s), "=v"(r), "=&v"(x), "=&s"(c), "=&s"(m), "=&s"(d)
+      : "v"(hl), "v"(t0), "s"(P - 1)
+      : "scc");
+  return r;
+#else
+  return add(t0, ((uint64_t)hl << 32) - hl);
+#endif
+}
+
 // a - b with the borrow folded back as 2^64 == EPS: (a - b) mod 2^64, minus EPS when the
 // subtraction borrowed.  For canonical a, b this is a - b (mod p), canonical (a - b + p
 // when a < b); the reductions below use it on operands where the fold cannot wrap.
@@ -121,22 +145,21 @@ GL_HD uint64_t neg(uint64_t a) { return sub(0, a); }
 // The final sum t0 + hl * EPS has t0 < 2^64 and hl * EPS <= p - 2^32, so one add() both
 // folds its carry and canonicalises.
 GL_HD uint64_t reduce128(uint64_t lo, uint64_t hi) {
-  uint64_t hh = hi >> 32, hl = hi & EPS;
+  uint64_t hh = hi >> 32;
   uint64_t t0 = sub_fold(lo, hh);  // borrowed 2^64 == EPS; hh < 2^32: cannot wrap again
-  uint64_t t1 = (hl << 32) - hl;   // hl * EPS <= p - 2^32
-  return add(t0, t1);
+  return add_eps_prod(t0, (uint32_t)hi);  // + hl * EPS, hl * EPS <= p - 2^32
 }
 
 // (lo + hl * 2^64) mod p for hl < 2^32, canonical: hl * 2^64 == hl * EPS (the 2^96 part is
 // zero, so no subtraction).
-GL_HD uint64_t reduce96(uint64_t lo, uint64_t hl) { return add(lo, (hl << 32) - hl); }
+GL_HD uint64_t reduce96(uint64_t lo, uint64_t hl) { return add_eps_prod(lo, (uint32_t)hl); }
 
 // (lo + hi * 2^64 + c * 2^128) mod p for c < 2^32 - 1, canonical: 2^128 == -2^32, so c
 // joins the 2^96 part of hi as the high word of the subtrahend.
 GL_HD uint64_t reduce160(uint64_t lo, uint64_t hi, uint32_t c) {
   uint64_t hh = (hi >> 32) | ((uint64_t)c << 32), hl = hi & EPS;
   uint64_t t0 = sub_fold(lo, hh);  // hh < 2^64 - 2^32: cannot wrap again
-  return add(t0, (hl << 32) - hl);
+  return add_eps_prod(t0, (uint32_t)hl);
 }
 
 // (lo, hi, c) += (plo, phi): a 128-bit add whose carry past 2^128 is counted in c.  On the
@@ -175,10 +198,7 @@ GL_HD void reduce128x4(uint64_t* r, const uint64_t* lo, const uint64_t* hi) {
   sub4(t[0], t[1], t[2], t[3], lo[0], lo[1], lo[2], lo[3], hi[0] >> 32, hi[1] >> 32, hi[2] >> 32,
        hi[3] >> 32);
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const uint64_t hl = hi[i] & EPS;
-    r[i] = add(t[i], (hl << 32) - hl);
-  }
+  for (int i = 0; i < 4; i++) r[i] = add_eps_prod(t[i], (uint32_t)hi[i]);
 }
 
 // reduce160 over four lanes (c_i < 2^32 - 1), the four borrow folds in one sub4.
@@ -188,10 +208,7 @@ GL_HD void reduce160x4(uint64_t* r, const uint64_t* lo, const uint64_t* hi, cons
   for (int i = 0; i < 4; i++) hh[i] = (hi[i] >> 32) | ((uint64_t)c[i] << 32);
   sub4(t[0], t[1], t[2], t[3], lo[0], lo[1], lo[2], lo[3], hh[0], hh[1], hh[2], hh[3]);
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const uint64_t hl = hi[i] & EPS;
-    r[i] = add(t[i], (hl << 32) - hl);
-  }
+  for (int i = 0; i < 4; i++) r[i] = add_eps_prod(t[i], (uint32_t)hi[i]);
 }
 
 GL_HD void mul_full(uint64_t a, uint64_t b, uint64_t& lo, uint64_t& hi) {
