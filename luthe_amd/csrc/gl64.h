@@ -39,19 +39,20 @@ GL_HD uint64_t add(uint64_t a, uint64_t b) {
 // t0 + hl * EPS (mod p), canonical, for any 64-bit t0 and hl < 2^32 (the tail of every
 // reduction: hl * EPS <= p - 2^32, so the sum is < 2p and add()'s rule applies).  On the
 // device the product and the sum are one v_mad_u64_u32, whose carry-out is the wrap flag;
-// the addend EPS or 0 is then added by a second v_mad_u64_u32 (x * 1 + s).  One wait state
-// separates the SALU-written select mask from its reader.
+// the addend EPS or 0 is then added by a second v_mad_u64_u32 (x * 1 + s).  The select
+// mask is the SALU OR of the two VALU-written flags into a third register pair that no VALU
+// writes, so the select may read it at once (only VALU-written SGPRs need wait states before
+// a VALU reads them).
 GL_HD uint64_t add_eps_prod(uint64_t t0, uint32_t hl) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  uint64_t s, r, c, m, d;
+  uint64_t s, r, c, m, n, d;
   uint32_t x;
-  asm("v_mad_u64_u32 %0, %3, %6, -1, %7\n\t"
-      "v_cmp_lt_u64_e64 %4, %8, %0\n\t"
-      "s_or_b64 %4, %4, %3\n\t"
-      "s_nop 0\n\t"
-      "v_cndmask_b32_e64 %2, 0, -1, %4\n\t"
-      "v_mad_u64_u32 %1, %5, %2, 1, %0"
-      : "=&v"(s), "=v"(r), "=&v"(x), "=&s"(c), "=&s"(m), "=&s"(d)
+  asm("v_mad_u64_u32 %0, %3, %7, -1, %8\n\t"
+      "v_cmp_lt_u64_e64 %4, %9, %0\n\t"
+      "s_or_b64 %5, %4, %3\n\t"
+      "v_cndmask_b32_e64 %2, 0, -1, %5\n\t"
+      "v_mad_u64_u32 %1, %6, %2, 1, %0"
+      : "=&v"(s), "=v"(r), "=&v"(x), "=&s"(c), "=&s"(m), "=&s"(n), "=&s"(d)
       : "v"(hl), "v"(t0), "s"(P - 1)
       : "scc");
   return r;
