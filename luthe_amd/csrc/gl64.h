@@ -303,10 +303,11 @@ GL_HD uint64_t from_signed32(int32_t d) {
 }
 
 // v mod p for a signed 64-bit v, canonical: (uint64)v = 2^64 + v when negative, and
-// 2^64 + v - EPS = p + v.
+// 2^64 + v - EPS = p + v.  -EPS == 1 - 2^32 (mod 2^64) is the word pair (1, 2^32 - 1), i.e.
+// (b, m) with m the sign word of v and b = m & 1: one 64-bit add, no compare or select.
 GL_HD uint64_t lift64(int64_t v) {
-  const uint64_t u = (uint64_t)v;
-  return v < 0 ? u - EPS : u;
+  const uint32_t m = (uint32_t)(v >> 63);
+  return (uint64_t)v + (((uint64_t)m << 32) | (m & 1u));
 }
 
 // d * 2^(12 M1) mod p for a digit |d| <= 2^22, canonical: the forward NTT's pre-twist
