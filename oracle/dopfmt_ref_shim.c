@@ -1,5 +1,5 @@
 /*
- * Shim exposing the reference firmware's DOp template patching (patch_dop in
+ * Wrapper exposing the reference firmware's DOp template patching (patch_dop in
  * fw/ublaze/src/hpu_dop_fmt.h:206-229) through a flat C function, so tests can
  * compare this build's patching against the reference's own C code.  The header is
  * compiled from the reference checkout in place; nothing from it is copied here.

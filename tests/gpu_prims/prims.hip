@@ -8,7 +8,7 @@
 #include "../../luthe_amd/csrc/pbs_core.h"
 
 namespace {
-constexpr int NOUT = 31;  // results per operand triple, in the order listed in prims_run
+constexpr int NOUT = 33;  // results per operand triple, in the order listed in prims_run
 
 __global__ void k_prims(const uint64_t* a, const uint64_t* b, const uint32_t* c, int n, uint64_t* out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -50,6 +50,9 @@ __global__ void k_prims(const uint64_t* a, const uint64_t* b, const uint32_t* c,
   const uint64_t rl[4] = {x, y, x, y}, rh[4] = {y, x, x, ~y};
   const uint32_t rc[4] = {zc, zc >> 1, 0, 0xFFFFFFFEu};
   gl::reduce160x4(o + 27, rl, rh, rc);
+  // signed lift into Z_p, full range and a shifted (small-magnitude) value
+  o[31] = gl::lift64((int64_t)x);
+  o[32] = gl::lift64((int64_t)x >> 17);
 }
 }  // namespace
 

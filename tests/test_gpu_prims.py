@@ -40,8 +40,8 @@ def test_device_primitives_exact(prims, oracle):
     a = np.array([p[0] for p in pairs], np.uint64)
     b = np.array([p[1] for p in pairs], np.uint64)
     c = np.array([zs[i % len(zs)] for i in range(n)], np.uint32)
-    out = np.zeros((n, 31), np.uint64)
-    print(f"case: {n} operand pairs through 20 device primitives", flush=True)
+    out = np.zeros((n, 33), np.uint64)
+    print(f"case: {n} operand pairs through 21 device primitives", flush=True)
     assert prims.prims_run(a.ctypes.data, b.ctypes.data, c.ctypes.data, n, out.ctypes.data) == 0
     for i, (x, y) in enumerate(pairs):
         z = int(c[i])
@@ -60,5 +60,7 @@ def test_device_primitives_exact(prims, oracle):
         exp += [x * y % P, y * x % P, (x ^ y) * y % P, nx * x % P]
         exp += [(x + (y << 64) + (zz << 128)) % P, (y + (x << 64) + ((zz >> 1) << 128)) % P,
                 (x + (x << 64)) % P, (y + (ny << 64) + (0xFFFFFFFE << 128)) % P]
+        xs = x - 2**64 if x >= 2**63 else x  # x as a signed 64-bit value
+        exp += [xs % P, (xs >> 17) % P]
         got = [int(v) for v in out[i]]
-        assert got == exp, (hex(x), hex(y), z, [k for k in range(31) if got[k] != exp[k]])
+        assert got == exp, (hex(x), hex(y), z, [k for k in range(33) if got[k] != exp[k]])
