@@ -2,7 +2,7 @@
 //
 //  k_ntt_*        batched negacyclic NTT / INTT / NTT-multiply-INTT, N = 2048
 //  k_bsk_prepare  bootstrapping key -> NTT domain, blind-rotation layout
-//  k_keyswitch    batched LWE key switch (integer GEMM mod 2^32 on VALU, 24-bit MACs)
+//  (k_keyswitch, the batched LWE key switch, is in keyswitch.hip)
 //  k_blind_rotate persistent blind rotation: one ciphertext per 256-thread workgroup,
 //                 accumulator in registers, NTT work buffer in LDS, BSK streamed from
 //                 HBM/L2 in a coalesced per-thread layout; sample extract fused
@@ -170,73 +170,6 @@ __global__ __launch_bounds__(256) void k_bsk_prepare(const uint64_t* __restrict_
 #pragma unroll
   for (int w0 = 0; w0 < 8; w0++)
     dst[(((i * 8 + w0) * rows + gl_) * K1 + j) * T + t] = gl::canonical(v[0][w0]);
-}
-
-// ------------------------------------------------------------------------------------
-// Key switch.  Workgroup = 32 ciphertexts x 256 output coefficients; K dimension =
-// (k*N mask coefficients) x KSL levels, streamed in chunks of 64 coefficients whose
-// digits are decomposed once into LDS and broadcast to all threads.
-// ------------------------------------------------------------------------------------
-constexpr int KS_CT = 32;
-constexpr int KS_CHUNK = 64;
-
-template <int KSL>
-__global__ __launch_bounds__(256) void k_keyswitch(const uint64_t* __restrict__ base, const PbsJob* __restrict__ jobs,
-                                                   int nct, const uint32_t* __restrict__ ksk,
-                                                   uint16_t* __restrict__ lwe2n, KsArgs a) {
-  __shared__ int32_t dig[KS_CHUNK][KSL][KS_CT / 4];  // 4 signed 8-bit digits per word
-  const int t = threadIdx.x;
-  const int c0 = blockIdx.y * KS_CT;
-  const int j = blockIdx.x * 256 + t;
-  const int n1 = a.n_lwe + 1;
-  const size_t blen = (size_t)a.kN + 1;
-  uint32_t acc[KS_CT];  // sums mod 2^32 (the result is taken mod 2^w, w <= 24)
-#pragma unroll
-  for (int c = 0; c < KS_CT; c++) acc[c] = 0;
-  int8_t* dig8 = reinterpret_cast<int8_t*>(&dig[0][0][0]);
-  for (int i0 = 0; i0 < a.kN; i0 += KS_CHUNK) {
-    __syncthreads();
-    // 32 x 64 coefficients, 8 per thread; a row's coefficients are contiguous.
-#pragma unroll
-    for (int e = 0; e < (KS_CT * KS_CHUNK) / 256; e++) {
-      int idx = e * 256 + t;
-      int c = idx / KS_CHUNK, ii = idx % KS_CHUNK;
-      uint64_t x = 0;
-      if (c0 + c < nct && i0 + ii < a.kN) x = base[(size_t)jobs[c0 + c].in_row * blen + i0 + ii];
-      int64_t d[KSL];
-      pbs::decompose<KSL>(x, a.base_log, d);
-#pragma unroll
-      for (int l = 0; l < KSL; l++) dig8[(ii * KSL + l) * KS_CT + c] = (int8_t)d[l];
-    }
-    __syncthreads();
-    if (j < n1) {
-      const int iimax = (a.kN - i0) < KS_CHUNK ? (a.kN - i0) : KS_CHUNK;
-      for (int ii = 0; ii < iimax; ii++) {
-#pragma unroll
-        for (int l = 0; l < KSL; l++) {
-          int32_t kv = (int32_t)ksk[((size_t)(i0 + ii) * KSL + l) * n1 + j];
-#pragma unroll
-          for (int c4 = 0; c4 < KS_CT / 4; c4++) {
-            int32_t w = dig[ii][l][c4];
-#pragma unroll
-            for (int e = 0; e < 4; e++) {
-              int32_t dv = (int32_t)(int8_t)(w >> (8 * e));
-              acc[c4 * 4 + e] += (uint32_t)__mul24(dv, kv);
-            }
-          }
-        }
-      }
-    }
-  }
-  if (j < n1) {
-#pragma unroll
-    for (int c = 0; c < KS_CT; c++) {
-      if (c0 + c >= nct) break;
-      uint32_t body = 0;
-      if (j == a.n_lwe) body = pbs::ks_body(base[(size_t)jobs[c0 + c].in_row * blen + a.kN], a.mod_log);
-      lwe2n[(size_t)(c0 + c) * a.lwe_stride + j] = (uint16_t)pbs::ks_finish((uint32_t)acc[c], body, a.mod_log, a.lc);
-    }
-  }
 }
 
 // ------------------------------------------------------------------------------------
@@ -553,27 +486,6 @@ hipError_t launch_bsk_prepare(const uint64_t* src, uint64_t* dst, int n_lwe, int
                               const uint64_t* tab, hipStream_t s) {
   k_bsk_prepare<<<dim3((unsigned)((size_t)n_lwe * rows * K1)), 256, 0, s>>>(src, dst, rows, K1, is_ntt, tab);
   return hipGetLastError();
-}
-
-hipError_t launch_keyswitch(const uint64_t* base, const PbsJob* jobs, int nct, const uint32_t* ksk, uint16_t* lwe2n,
-                            const KsArgs& a, int ks_level, hipStream_t s) {
-  if (ks_level < 1 || ks_level > 8) return hipErrorInvalidValue;
-  // grid.y is limited to 65535: chunks of at most 65535 * KS_CT ciphertexts
-  for (int c0 = 0; c0 < nct; c0 += MAX_GRID_Y * KS_CT) {
-    const int nc = (nct - c0) < MAX_GRID_Y * KS_CT ? (nct - c0) : MAX_GRID_Y * KS_CT;
-    dim3 grid((unsigned)((a.n_lwe + 1 + 255) / 256), (unsigned)((nc + KS_CT - 1) / KS_CT));
-    const PbsJob* jb = jobs + c0;
-    uint16_t* out = lwe2n + (size_t)c0 * a.lwe_stride;
-    switch (ks_level) {
-#define KS_CASE(L) \
-  case L: k_keyswitch<L><<<grid, 256, 0, s>>>(base, jb, nc, ksk, out, a); break;
-      KS_CASE(1) KS_CASE(2) KS_CASE(3) KS_CASE(4) KS_CASE(5) KS_CASE(6) KS_CASE(7) KS_CASE(8)
-#undef KS_CASE
-    }
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-  }
-  return hipSuccess;
 }
 
 template <int K1, int L>
