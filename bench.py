@@ -1,16 +1,16 @@
 """Benchmark: batched PBS throughput on one MI355X.
 
 Workload (one "step"): one batched PBS (key switch + blind rotation + sample extract)
-over 8192 ciphertexts with the config-0 parameters (n=724, k=1, N=2048, PBS base 2^23
-level 1, KS base 2^3 level 5, w_ks=21), seeded keys, uniform 4-bit plaintexts, identity
+over 8192 ciphertexts with n=724, k=1, N=2048, PBS base 2^23 level 1, KS base 2^3
+level 5, w_ks=21, seeded keys, uniform 4-bit plaintexts, identity
 LUT; inputs resident in HBM before the timed region.  Beside it:
   * the plain path: the same kernels with one ciphertext per launch;
   * the blind-rotation kernel's own duration (HIP events on the engine stream) and its
     roofline: algorithmic modmul-equivalents per launch / duration against the measured
     full-occupancy rate of the same Goldilocks modmul primitive;
-  * config 1 (batched NTT core), config 3 (uneven graph-captured batches, PBS level 2)
-    and config 4 (serving: Poisson arrivals of radix IOps through the continuous-batching
-    server, PBS/s with p50/p99 latency per IOp);
+  * the NTT product (4096 polynomials: NTT, pointwise product, INTT), uneven
+    graph-captured batches (64 batches, PBS level 2) and serving (Poisson arrivals of radix
+    IOps through the continuous-batching server, PBS/s with p50/p99 latency per IOp);
   * cpu_baseline: the C oracle (oracle/luthe_ref.c, OpenMP over ciphertexts) timed on a
     bounded sample of the same batch; its outputs are compared bit for bit with the GPU's.
 
@@ -69,8 +69,8 @@ def bsk_bytes(p):
     return p.lwe_dim * k1 * p.pbs_level * k1 * p.poly_size * 8
 
 
-def run_config1(la, hip, ctx, peak, npoly=4096, reps=5):
-    """Config 1: NTT -> pointwise multiply by an NTT-domain set -> INTT, 4096 x 2048."""
+def run_ntt_product(la, hip, ctx, peak, npoly=4096, reps=5):
+    """NTT -> pointwise multiply by an NTT-domain set -> INTT, 4096 polynomials x 2048."""
     rng = np.random.default_rng(1)
     a = rng.integers(0, la.P, (npoly, 2048), dtype=np.uint64)
     bn = rng.integers(0, la.P, (npoly, 2048), dtype=np.uint64)
@@ -91,15 +91,15 @@ def run_config1(la, hip, ctx, peak, npoly=4096, reps=5):
             "gmodmul_per_s": round(work / (ms * 1e-3) / 1e9, 1), "valu_frac": round(work / (ms * 1e-3) / 1e9 / peak, 4)}
 
 
-def run_config3(la, hip, seed=3):
-    """Config 3: 64 batches, sizes uniform in [1, 4096]; per batch an ADD and a MAC then a
+def run_uneven_batches(la, hip, seed=3):
+    """64 batches, sizes uniform in [1, 4096]; per batch an ADD and a MAC then a
     PBS with a per-ciphertext LUT (random 16-entry table); PBS level 2, base 2^15; the
     whole 64-batch program captured as one HIP graph and replayed."""
     prm = la.default_params(pbs_base_log=15, pbs_level=2, lut_slots=4096)
     rng = np.random.default_rng(seed)
     sizes = [int(x) for x in rng.integers(1, 4097, 64)]
     total = sum(sizes)
-    log(f"bench: config 3, {total} PBS in 64 batches, keygen L=2")
+    log(f"bench: uneven batches, {total} PBS in 64 batches, keygen L=2")
     keys = la.Keys(prm, seed=5678)
     ctx = la.Context(prm, device=0)
     ctx.upload_bsk(keys.bsk)
@@ -152,8 +152,8 @@ def _pct(v, q):
     return float(np.percentile(np.asarray(v), q, method="nearest")) if len(v) else None
 
 
-def run_config4(la, ctx, keys, rates=(8, 20, 40, 60), seconds=5.0, burst=96, seed=4, cohort_max=0):
-    """Config 4: serving.  Poisson arrivals (seeded) of radix IOps drawn uniformly from
+def run_serving(la, ctx, keys, rates=(8, 20, 40, 60), seconds=5.0, burst=96, seed=4, cohort_max=0):
+    """Serving.  Poisson arrivals (seeded) of radix IOps drawn uniformly from
     {ADD, CMP_GT, MUL} x {8, 16, 32} blocks of 2 bits, through the continuous-batching
     server (HBM arena admission, one PBS cohort per step); latency = submit -> result on
     the host.  Plus a burst of `burst` requests submitted at once (saturation PBS/s)."""
@@ -166,7 +166,7 @@ def run_config4(la, ctx, keys, rates=(8, 20, 40, 60), seconds=5.0, burst=96, see
     name = {la.IOP_ADD: "ADD", la.IOP_CMP_GT: "CMP_GT", la.IOP_MUL: "MUL"}
     mean_pbs = float(np.mean([npbs[k] for k in mix]))
     res = {"workload": "Poisson arrivals of radix IOps, uniform over {ADD, CMP_GT, MUL} x {8, 16, 32} blocks x 2 "
-                       "bits, config-0 parameters, keys resident, continuous batching",
+                       "bits, n=724 PBS 2^23 x 1 parameters, keys resident, continuous batching",
            "pbs_per_iop": {f"{name[o]}{n}": npbs[(o, n)] for o, n in mix}, "mean_pbs_per_request": mean_pbs}
     srv = la.Server(ctx, cohort_max=cohort_max)
     res["cohort_max"] = srv.stats()["cohort_max"]
@@ -194,7 +194,7 @@ def run_config4(la, ctx, keys, rates=(8, 20, 40, 60), seconds=5.0, burst=96, see
     ids = [(mix[k][0], mix[k][1], srv.submit(mix[k][0], pool_a[:mix[k][1]], pool_b[:mix[k][1]])) for k in kinds]
     infos = [(op, nb, srv.wait(rid)[1]) for op, nb, rid in ids]
     res["burst"] = summarize(infos, time.perf_counter() - t0, srv.stats())
-    log(f"bench: config 4 burst of {burst}: {res['burst']['pbs_per_s']} PBS/s, "
+    log(f"bench: serving burst of {burst}: {res['burst']['pbs_per_s']} PBS/s, "
         f"p50 {res['burst']['p50_ms']} ms, p99 {res['burst']['p99_ms']} ms")
     res["rates"] = []
     for rate in rates:
@@ -218,7 +218,7 @@ def run_config4(la, ctx, keys, rates=(8, 20, 40, 60), seconds=5.0, burst=96, see
         row["offered_iops_per_s"] = rate
         row["offered_load"] = round(rate * mean_pbs / max(1.0, res["burst"]["pbs_per_s"]), 3)
         res["rates"].append(row)
-        log(f"bench: config 4 rate {rate}/s: {row['pbs_per_s']} PBS/s, p50 {row['p50_ms']} ms, "
+        log(f"bench: serving rate {rate}/s: {row['pbs_per_s']} PBS/s, p50 {row['p50_ms']} ms, "
             f"p99 {row['p99_ms']} ms, mean cohort {row['mean_cohort']}")
     srv.close()
     return res
@@ -234,10 +234,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--br-grid", type=int, default=0, help="blind-rotation workgroups (0 = occupancy x CUs)")
-    ap.add_argument("--no-extra", action="store_true", help="skip the config-1, -3 and -4 measurements")
-    ap.add_argument("--serve-seconds", type=float, default=5.0, help="arrival window per rate (config 4)")
-    ap.add_argument("--serve-cohort-max", type=int, default=0, help="PBS per cohort (config 4; 0 = server default)")
-    ap.add_argument("--extras", default="c1,c3,c4", help="extra configurations to run beside the headline")
+    ap.add_argument("--no-extra", action="store_true", help="skip the NTT-product, uneven-batch and serving measurements")
+    ap.add_argument("--serve-seconds", type=float, default=5.0, help="arrival window per rate (serving)")
+    ap.add_argument("--serve-cohort-max", type=int, default=0, help="PBS per cohort (serving; 0 = server default)")
+    ap.add_argument("--extras", default="ntt,uneven,serving",
+                    help="extra workloads to run beside the headline (ntt, uneven, serving)")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (library defaults)")
     ap.add_argument("--tuning-json", default="", help="where the autotuner writes its choices")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -263,7 +264,7 @@ def main():
         return
 
     prm = la.default_params()
-    log(f"bench: config-2 batch={args.batch} n={prm.lwe_dim} N={prm.poly_size}: keygen")
+    log(f"bench: batched PBS batch={args.batch} n={prm.lwe_dim} N={prm.poly_size}: keygen")
     keys = la.Keys(prm, seed=1234)
     ctx = la.Context(prm, device=0)
     if args.br_grid:
@@ -364,15 +365,15 @@ def main():
     }
 
     extras = set() if args.no_extra else {e.strip() for e in args.extras.split(",") if e.strip()}
-    if "c1" in extras:
-        out["config1_ntt"] = run_config1(la, hip, ctx, peak)
-        log(f"bench: config 1 {out['config1_ntt']}")
-    if "c3" in extras:
-        out["config3_uneven"] = run_config3(la, hip)
-        log(f"bench: config 3 {out['config3_uneven']}")
-    if "c4" in extras:
-        out["config4_serving"] = run_config4(la, ctx, keys, seconds=args.serve_seconds,
-                                             cohort_max=args.serve_cohort_max)
+    if "ntt" in extras:
+        out["ntt_product_4096"] = run_ntt_product(la, hip, ctx, peak)
+        log(f"bench: NTT product {out['ntt_product_4096']}")
+    if "uneven" in extras:
+        out["uneven_batches"] = run_uneven_batches(la, hip)
+        log(f"bench: uneven batches {out['uneven_batches']}")
+    if "serving" in extras:
+        out["serving"] = run_serving(la, ctx, keys, seconds=args.serve_seconds,
+                                     cohort_max=args.serve_cohort_max)
 
     if not args.no_cpu_baseline:
         from oracle import ref  # CPU baseline leg only

@@ -86,7 +86,7 @@ typedef struct luthe_counters {
 
 /* ---------------------------------------------------------------- context */
 
-/* Config-0 defaults: n=724, k=1, N=2048, PBS 2^23 x 1, KS 2^3 x 5, w_ks=21,
+/* Defaults: n=724, k=1, N=2048, PBS 2^23 x 1, KS 2^3 x 5, w_ks=21,
  * 2+2 bit blocks, noise 2^-15 / 2^-51, 64 registers, 4096 ct slots, 256 LUT slots. */
 void luthe_params_default(luthe_params* p);
 

@@ -81,7 +81,7 @@ def test_other_parameters(oracle, over):
 
 
 def test_ragged_batch_4097(oracle):
-    print("case: batch of 4097 (config-0 parameters), sampled against the oracle", flush=True)
+    print("case: batch of 4097 (n=724, PBS 2^23 x 1), sampled against the oracle", flush=True)
     prm, keys, ctx, rp, bsk_ntt = _setup(oracle, 408)
     ctx.write_lut(0, la.lut_encode(prm, np.arange(16, dtype=np.uint8)))
     n = 4097

@@ -1,4 +1,4 @@
-"""GPU: config-3 parameters (PBS level 2, base 2^15), Many-LUT PBS, batched ALU,
+"""GPU: PBS level-2 parameters (base 2^15 x 2), Many-LUT PBS, batched ALU,
 HIP-graph capture of an uneven batch program, IOp submission and error paths --
 all bit-exact against the oracle through the C ABI."""
 import os
@@ -14,8 +14,8 @@ THREADS = min(16, os.cpu_count() or 1)
 
 
 @pytest.fixture(scope="module")
-def cfg3(oracle):
-    """Config 3 parameters: n=724, k=1, N=2048, PBS base 2^15 level 2, KS 2^3 x 5."""
+def prm_l2(oracle):
+    """PBS level-2 parameters: n=724, k=1, N=2048, PBS base 2^15 level 2, KS 2^3 x 5."""
     prm = la.default_params(pbs_base_log=15, pbs_level=2, lut_slots=512)
     ctx = la.Context(prm, device=0)
     keys = la.Keys(prm, seed=303)
@@ -27,9 +27,9 @@ def cfg3(oracle):
     ctx.close()
 
 
-def test_pbs_level2_vs_oracle(cfg3, oracle):
+def test_pbs_level2_vs_oracle(prm_l2, oracle):
     print("case: PBS L=2 B=2^15, 12 ciphertexts, per-ciphertext LUTs", flush=True)
-    prm, ctx, keys, rp, bsk_ntt = cfg3
+    prm, ctx, keys, rp, bsk_ntt = prm_l2
     rng = np.random.default_rng(31)
     tables = rng.integers(0, 16, (12, 16), dtype=np.uint8)
     luts = np.stack([la.lut_encode(prm, t) for t in tables])
@@ -44,9 +44,9 @@ def test_pbs_level2_vs_oracle(cfg3, oracle):
 
 
 @pytest.mark.parametrize("log_nb", [1, 2, 3])
-def test_many_lut_vs_oracle(cfg3, oracle, log_nb):
+def test_many_lut_vs_oracle(prm_l2, oracle, log_nb):
     print(f"case: Many-LUT log_lut_nb={log_nb}", flush=True)
-    prm, ctx, keys, rp, bsk_ntt = cfg3
+    prm, ctx, keys, rp, bsk_ntt = prm_l2
     rng = np.random.default_rng(40 + log_nb)
     nb, tlen = 1 << log_nb, 16 >> log_nb
     tabs = rng.integers(0, 16, (nb, tlen), dtype=np.uint8)
@@ -65,9 +65,9 @@ def test_many_lut_vs_oracle(cfg3, oracle, log_nb):
     assert (dec == tabs[:, msgs].T).mean() >= 0.75
 
 
-def test_alu_batch_vs_oracle(cfg3, oracle):
+def test_alu_batch_vs_oracle(prm_l2, oracle):
     print("case: ALU batch, all opcodes", flush=True)
-    prm, ctx, _, _, _ = cfg3
+    prm, ctx, _, _, _ = prm_l2
     rng = np.random.default_rng(50)
     n = 33
     a = rng.integers(0, 2**64, (n, prm.blen), dtype=np.uint64)
@@ -83,13 +83,13 @@ def test_alu_batch_vs_oracle(cfg3, oracle):
             assert np.array_equal(got[c], exp), hex(op)
 
 
-def test_uneven_batches_graph_replay(cfg3, oracle):
-    """Config-3 shape at reduced size: batches of uneven sizes, per-ciphertext LUTs, an
+def test_uneven_batches_graph_replay(prm_l2, oracle):
+    """The uneven-batch workload at reduced size: batches of uneven sizes, per-ciphertext LUTs, an
     ADD and a MAC before each PBS, captured once as a HIP graph and replayed three times
     with eager work in between; every replay equals the eager result, which equals the
     oracle on sampled ciphertexts."""
     print("case: uneven batches captured as a HIP graph", flush=True)
-    prm, ctx, keys, rp, bsk_ntt = cfg3
+    prm, ctx, keys, rp, bsk_ntt = prm_l2
     rng = np.random.default_rng(60)
     sizes = [1, 37, 5, 120, 64, 3]
     total = sum(sizes)
@@ -139,10 +139,10 @@ def test_uneven_batches_graph_replay(cfg3, oracle):
         assert np.array_equal(eager[c], exp), c
 
 
-def test_iop_submission(cfg3, oracle):
+def test_iop_submission(prm_l2, oracle):
     """An IOp with a registered DOp template program behaves like the patched DOps."""
     print("case: IOp path", flush=True)
-    prm, ctx, keys, rp, bsk_ntt = cfg3
+    prm, ctx, keys, rp, bsk_ntt = prm_l2
     rng = np.random.default_rng(70)
     lut = la.lut_encode(prm, (np.arange(16) * 7 % 16).astype(np.uint8))
     ctx.write_lut(300, lut)

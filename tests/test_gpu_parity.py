@@ -1,6 +1,6 @@
 """GPU parity: every HIP kernel path against the CPU oracle, bit for bit, through the C ABI.
 
-Sizes are chosen so that the oracle finishes in seconds; full-size configurations are
+Sizes are chosen so that the oracle finishes in seconds; full-size workloads are
 checked through size-independent properties (round trips, batch independence).
 """
 import os
@@ -22,8 +22,8 @@ def _ctx(**over):
 
 
 @pytest.fixture(scope="module")
-def cfg0(oracle):
-    """Config 0: n=724, k=1, N=2048, PBS 2^23 x 1, KS 2^3 x 5, w_ks=21, seeded keys."""
+def prm_l1(oracle):
+    """PBS level-1 parameters: n=724, k=1, N=2048, PBS 2^23 x 1, KS 2^3 x 5, w_ks=21, seeded keys."""
     prm, ctx = _ctx()
     keys = la.Keys(prm, seed=11)
     ctx.upload_bsk(keys.bsk, la.BSK_COEF)
@@ -34,7 +34,7 @@ def cfg0(oracle):
     ctx.close()
 
 
-# ------------------------------------------------------------------ NTT core (config 1 kernel)
+# ------------------------------------------------------------------ NTT core (4096-polynomial product kernel)
 def test_ntt_forward_inverse_vs_oracle(oracle):
     print("case: NTT fwd/inv, 32 polys", flush=True)
     _, ctx = _ctx(lwe_dim=8)
@@ -69,8 +69,8 @@ def test_ntt_polymul_vs_oracle(oracle):
     ctx.close()
 
 
-def test_ntt_polymul_full_config1(oracle):
-    """Config 1 at full size: 4096 polynomials; 48 sampled rows against the oracle and
+def test_ntt_polymul_full_4096(oracle):
+    """The NTT product at full size: 4096 polynomials; 48 sampled rows against the oracle and
     the size-independent identity  polymul(a, NTT(1)) == a  on every row."""
     print("case: NTT polymul 4096 x 2048", flush=True)
     _, ctx = _ctx(lwe_dim=8)
@@ -89,9 +89,9 @@ def test_ntt_polymul_full_config1(oracle):
 
 
 # ------------------------------------------------------------------ key switch
-def test_keyswitch_vs_oracle(cfg0, oracle):
-    print("case: KS 40 ciphertexts config 0", flush=True)
-    prm, ctx, keys, rp, _ = cfg0
+def test_keyswitch_vs_oracle(prm_l1, oracle):
+    print("case: KS 40 ciphertexts, n=724 PBS 2^23 x 1 parameters", flush=True)
+    prm, ctx, keys, rp, _ = prm_l1
     rng = np.random.default_rng(4)
     blwe = rng.integers(0, 2**64, (40, prm.blen), dtype=np.uint64)
     blwe[0] = 0
@@ -103,9 +103,9 @@ def test_keyswitch_vs_oracle(cfg0, oracle):
 
 
 # ------------------------------------------------------------------ blind rotation
-def test_blind_rotate_vs_oracle(cfg0, oracle):
-    print("case: BR 6 ciphertexts config 0 (incl. rotations 0, 2047, 2048, 4095)", flush=True)
-    prm, ctx, keys, rp, bsk_ntt = cfg0
+def test_blind_rotate_vs_oracle(prm_l1, oracle):
+    print("case: BR 6 ciphertexts, n=724 PBS 2^23 x 1 (incl. rotations 0, 2047, 2048, 4095)", flush=True)
+    prm, ctx, keys, rp, bsk_ntt = prm_l1
     rng = np.random.default_rng(5)
     luts = rng.integers(0, 2**64, (2, prm.lut_len), dtype=np.uint64)
     ctx.write_lut(0, luts)
@@ -120,9 +120,9 @@ def test_blind_rotate_vs_oracle(cfg0, oracle):
 
 
 # ------------------------------------------------------------------ full PBS
-def test_pbs_vs_oracle_and_decrypt(cfg0, oracle):
-    print("case: PBS 16 ciphertexts config 0, per-ciphertext LUTs", flush=True)
-    prm, ctx, keys, rp, bsk_ntt = cfg0
+def test_pbs_vs_oracle_and_decrypt(prm_l1, oracle):
+    print("case: PBS 16 ciphertexts, n=724 PBS 2^23 x 1, per-ciphertext LUTs", flush=True)
+    prm, ctx, keys, rp, bsk_ntt = prm_l1
     rng = np.random.default_rng(6)
     tables = rng.integers(0, 16, (4, 16), dtype=np.uint8)
     tables[0] = np.arange(16)
@@ -139,10 +139,10 @@ def test_pbs_vs_oracle_and_decrypt(cfg0, oracle):
     assert ok >= 0.75, f"decryption success {ok}"
 
 
-def test_pbs_batch_independence(cfg0):
+def test_pbs_batch_independence(prm_l1):
     """Same ciphertext, alone and inside a large batch: identical bits."""
     print("case: PBS alone vs in a batch of 1500", flush=True)
-    prm, ctx, keys, _, _ = cfg0
+    prm, ctx, keys, _, _ = prm_l1
     lut = la.lut_encode(prm, np.arange(16, dtype=np.uint8))
     ctx.write_lut(0, lut)
     cts = keys.encrypt_msg(np.arange(1500) % 16, seed=21)
@@ -154,9 +154,9 @@ def test_pbs_batch_independence(cfg0):
 
 
 # ------------------------------------------------------------------ DOp stream
-def test_dop_stream_vs_oracle(cfg0, oracle):
+def test_dop_stream_vs_oracle(prm_l1, oracle):
     print("case: DOp stream LD/ALU/PBS/ST", flush=True)
-    prm, ctx, keys, rp, bsk_ntt = cfg0
+    prm, ctx, keys, rp, bsk_ntt = prm_l1
     lut = la.lut_encode(prm, (np.arange(16) * 5 % 16).astype(np.uint8))
     ctx.write_lut(3, lut)
     cts = keys.encrypt_msg([1, 2], seed=31)

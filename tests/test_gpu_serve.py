@@ -80,7 +80,7 @@ def small(oracle):
 
 @pytest.fixture(scope="module")
 def full():
-    """Config-0 shapes (n = 724, N = 2048, PBS 2^23 x 1, KS 2^3 x 5) with low key-switch noise."""
+    """The PBS level-1 shapes (n = 724, N = 2048, PBS 2^23 x 1, KS 2^3 x 5) with low key-switch noise."""
     prm = la.default_params(lwe_noise_std=2.0**-25)
     ctx = la.Context(prm, device=0)
     keys = la.Keys(prm, seed=616)

@@ -90,7 +90,7 @@ def _poly_eval(p, x):
 
 def test_ntt_output_order_is_evaluation_at_odd_powers():
     """ntt() with cut [5,6]: index 64k + w holds P(zeta^(2(k + 32 w) + 1)), zeta the 4096th
-    root -- a closed form that fixes the NTT-domain order used by BSK and config 1."""
+    root -- a closed form that fixes the NTT-domain order used by the BSK and the NTT product."""
     L = _gl()
     zeta = L.ref_power_of_rou(4096, 1)
     p = rng.integers(0, P, 2048, dtype=np.uint64)
