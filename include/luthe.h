@@ -225,7 +225,11 @@ int luthe_reserve(luthe_ctx* ctx, size_t max_batch);
 /* HIP graph capture of batch calls issued on the context stream (stream argument NULL):
  * begin, issue luthe_pbs_batch / luthe_alu_batch / NTT calls, end; then replay with
  * luthe_graph_launch.  DOp submissions cannot be captured.  Shapes and pointers are
- * fixed at capture time. */
+ * fixed at capture time.  A graph keeps using the scratch buffers it was captured with:
+ * while any graph of the context exists, scratch that a later call grows is kept (not
+ * freed) and is released when the last graph is destroyed.  A graph replays only on the
+ * context it was captured on; destroying the context first leaves the graph valid only for
+ * luthe_graph_destroy. */
 typedef struct luthe_graph luthe_graph;
 int luthe_capture_begin(luthe_ctx* ctx);
 int luthe_capture_end(luthe_ctx* ctx, luthe_graph** out);

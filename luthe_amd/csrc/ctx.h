@@ -18,6 +18,7 @@ struct Ticket {
 struct luthe_graph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  luthe_ctx* ctx = nullptr;  // cleared when the context is destroyed first
 };
 
 struct luthe_ctx {
@@ -46,6 +47,10 @@ struct luthe_ctx {
   uint8_t* h_stage = nullptr;  // pinned mirror
   size_t h_stage_cap = 0;
   hipEvent_t stage_copied = nullptr;
+  // Captured graphs hold raw pointers to the scratch above: while any exists, scratch that
+  // grows is retired here instead of freed, and freed when the last graph is destroyed.
+  std::vector<luthe_graph*> graphs;
+  std::vector<std::pair<void*, size_t>> retired;  // (pointer, bytes)
   uint64_t dev_bytes = 0;
   luthe_counters cnt{};
   uint64_t next_ticket = 1;
@@ -68,6 +73,8 @@ namespace lx {
 // Records the message on the context and returns `code`.
 int fail(luthe_ctx* c, int code, const char* fmt, ...);
 int dev_alloc(luthe_ctx* c, void** p, size_t bytes);
+// Frees a scratch buffer that is being replaced, or retires it while captured graphs exist.
+void retire_scratch(luthe_ctx* c, void* p, size_t bytes);
 int ensure_lwe2n(luthe_ctx* c, size_t n);
 int ensure_jobs(luthe_ctx* c, size_t n);
 int ensure_rename_rows(luthe_ctx* c, size_t n);
@@ -75,6 +82,8 @@ int ensure_rename_rows(luthe_ctx* c, size_t n);
 // device; rows are relative to in_base / out_base.
 int run_pbs(luthe_ctx* c, const uint64_t* in_base, const lk::PbsJob* d_jobs, int n, uint64_t* out_base,
             hipStream_t s);
+// Width of the ALU scalar constant in the body: msg + carry + 1 padding bit.
+uint32_t useful_bits(const luthe_ctx* c);
 // Largest cohort that runs the 512-thread latency kernel.
 int latency_max(luthe_ctx* c);
 

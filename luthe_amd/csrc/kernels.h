@@ -57,10 +57,11 @@ int blind_rotate_occupancy(int pbs_level);
 // Ciphertexts the latency variant holds resident at once (its workgroups per CU x CUs).
 int blind_rotate_latency_slots(int pbs_level);
 hipError_t launch_modmul_peak(uint64_t* out, int blocks, int iters, hipStream_t s);
-hipError_t launch_alu(uint64_t* base, const AluOp* ops, int nops, int blen, hipStream_t s);
+// useful_bits = msg + carry + 1 padding bit: the ADDS/SUBS/SSUB constant's width in the body.
+hipError_t launch_alu(uint64_t* base, const AluOp* ops, int nops, int blen, uint32_t useful_bits, hipStream_t s);
 hipError_t launch_copy(uint64_t* base, const CopyOp* ops, int nops, int blen, hipStream_t s);
 hipError_t launch_alu_batch(uint32_t opcode, const uint64_t* a, const uint64_t* b, uint64_t* out, uint32_t mul_factor,
-                            uint32_t msg_cst, int n, int blen, hipStream_t s);
+                            uint32_t msg_cst, int n, int blen, uint32_t useful_bits, hipStream_t s);
 hipError_t launch_make_jobs(PbsJob* jobs, const uint32_t* gid, int n, uint32_t log_lut_nb, uint32_t lut_slots,
                             hipStream_t s);
 

@@ -362,14 +362,22 @@ __global__ __launch_bounds__(512, 1) void k_blind_rotate_lat(const uint64_t* __r
 // ------------------------------------------------------------------------------------
 // ALU DOps (pea_alu_core.sv:55-160) and LD/ST row copies.
 // ------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_alu(uint64_t* __restrict__ base, const AluOp* __restrict__ ops, int blen) {
+// The scalar constant of ADDS/SUBS/SSUB occupies the top `useful` bits of the body only:
+// msg_cst[useful-1:0] << (64 - useful), useful = msg + carry + padding bits
+// (pea_alu_core.sv:59, param_tfhe_pkg.sv:32); 1 <= useful <= 11 (checked by the host).
+__device__ __forceinline__ uint64_t alu_body_cst(uint32_t msg_cst, uint32_t useful) {
+  return ((uint64_t)msg_cst & ((1ull << useful) - 1)) << (64 - useful);
+}
+
+__global__ __launch_bounds__(256) void k_alu(uint64_t* __restrict__ base, const AluOp* __restrict__ ops, int blen,
+                                             uint32_t useful) {
   const AluOp op = ops[blockIdx.y];
   const int e = blockIdx.x * 256 + threadIdx.x;
   if (e >= blen) return;
   const uint64_t x = base[(size_t)op.src0_row * blen + e];
   const uint64_t y = (op.opcode == 0x01 || op.opcode == 0x02 || op.opcode == 0x05)
                          ? base[(size_t)op.src1_row * blen + e] : 0;
-  const uint64_t cst = (e == blen - 1) ? ((uint64_t)(op.msg_cst & 31u) << 59) : 0;
+  const uint64_t cst = (e == blen - 1) ? alu_body_cst(op.msg_cst, useful) : 0;
   uint64_t z;
   switch (op.opcode) {
     case 0x01: z = x + y; break;
@@ -387,13 +395,13 @@ __global__ __launch_bounds__(256) void k_alu(uint64_t* __restrict__ base, const 
 
 __global__ __launch_bounds__(256) void k_alu_batch(uint32_t opcode, const uint64_t* __restrict__ a,
                                                    const uint64_t* __restrict__ b, uint64_t* __restrict__ out,
-                                                   uint32_t mul_factor, uint32_t msg_cst, int blen) {
+                                                   uint32_t mul_factor, uint32_t msg_cst, int blen, uint32_t useful) {
   const int e = blockIdx.x * 256 + threadIdx.x;
   if (e >= blen) return;
   const size_t off = (size_t)blockIdx.y * blen + e;
   const uint64_t x = a[off];
   const uint64_t y = (opcode == 0x01 || opcode == 0x02 || opcode == 0x05) ? b[off] : 0;
-  const uint64_t cst = (e == blen - 1) ? ((uint64_t)(msg_cst & 31u) << 59) : 0;
+  const uint64_t cst = (e == blen - 1) ? alu_body_cst(msg_cst, useful) : 0;
   uint64_t z;
   switch (opcode) {
     case 0x01: z = x + y; break;
@@ -560,10 +568,11 @@ int blind_rotate_latency_slots(int pbs_level) {
   return pbs_level == 1 ? br_lat_max_grid<2, 1>() : br_lat_max_grid<2, 2>();
 }
 
-hipError_t launch_alu(uint64_t* base, const AluOp* ops, int nops, int blen, hipStream_t s) {
+hipError_t launch_alu(uint64_t* base, const AluOp* ops, int nops, int blen, uint32_t useful_bits, hipStream_t s) {
+  if (useful_bits < 1 || useful_bits > 11) return hipErrorInvalidValue;
   for (int o = 0; o < nops; o += MAX_GRID_Y) {
     const int n = (nops - o) < MAX_GRID_Y ? (nops - o) : MAX_GRID_Y;
-    k_alu<<<dim3((unsigned)((blen + 255) / 256), (unsigned)n), 256, 0, s>>>(base, ops + o, blen);
+    k_alu<<<dim3((unsigned)((blen + 255) / 256), (unsigned)n), 256, 0, s>>>(base, ops + o, blen, useful_bits);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
@@ -571,12 +580,13 @@ hipError_t launch_alu(uint64_t* base, const AluOp* ops, int nops, int blen, hipS
 }
 
 hipError_t launch_alu_batch(uint32_t opcode, const uint64_t* a, const uint64_t* b, uint64_t* out, uint32_t mul_factor,
-                            uint32_t msg_cst, int n, int blen, hipStream_t s) {
+                            uint32_t msg_cst, int n, int blen, uint32_t useful_bits, hipStream_t s) {
+  if (useful_bits < 1 || useful_bits > 11) return hipErrorInvalidValue;
   for (int o = 0; o < n; o += MAX_GRID_Y) {
     const int m = (n - o) < MAX_GRID_Y ? (n - o) : MAX_GRID_Y;
     const size_t off = (size_t)o * blen;
     k_alu_batch<<<dim3((unsigned)((blen + 255) / 256), (unsigned)m), 256, 0, s>>>(
-        opcode, a + off, b ? b + off : nullptr, out + off, mul_factor, msg_cst, blen);
+        opcode, a + off, b ? b + off : nullptr, out + off, mul_factor, msg_cst, blen, useful_bits);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }

@@ -69,6 +69,8 @@ int check_params(const luthe_params* p, std::string& why) {
   if (p->ks_base_log < 1 || p->ks_base_log > 7) return why = "ks_base_log must be in [1, 7]", -1;
   if (p->ks_base_log * p->ks_level >= 64) return why = "ks_base_log*ks_level must be < 64", -1;
   if (p->ks_mod_log < 13 || p->ks_mod_log > 24) return why = "ks_mod_log must be in [13, 24]", -1;
+  // the gadget weights 2^(w - b*(L-l)) of the KSK live mod 2^w
+  if (p->ks_base_log * p->ks_level > p->ks_mod_log) return why = "ks_base_log*ks_level must be <= ks_mod_log", -1;
   if (p->msg_bits + p->carry_bits < 1 || p->msg_bits + p->carry_bits > 8) return why = "msg_bits+carry_bits in [1, 8]", -1;
   if (p->regfile_size < 1 || p->regfile_size > 128) return why = "regfile_size in [1, 128]", -1;
   if (p->ct_slots > 65536) return why = "ct_slots <= 65536", -1;
@@ -76,16 +78,29 @@ int check_params(const luthe_params* p, std::string& why) {
   return 0;
 }
 
+void retire_scratch(luthe_ctx* c, void* p, size_t bytes) {
+  if (!p) return;
+  if (!c->graphs.empty()) {
+    c->retired.emplace_back(p, bytes);  // a captured graph may still read or write it
+    return;
+  }
+  (void)hipFree(p);
+  c->dev_bytes -= bytes;
+}
+
+// Scratch grows by allocating the new buffer first: on failure the context keeps the old
+// buffer and capacity.
 int ensure_lwe2n(luthe_ctx* c, size_t n) {
   if (n <= c->lwe2n_cap) return LUTHE_OK;
-  size_t cap = std::max(n, c->lwe2n_cap * 2);
+  const size_t cap = std::max(n, c->lwe2n_cap * 2);
+  uint16_t* nb = nullptr;
+  int rc = dev_alloc(c, (void**)&nb, cap * c->lwe_stride * sizeof(uint16_t));
+  if (rc) return rc;
   if (c->d_lwe2n) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipFree(c->d_lwe2n));
-    c->dev_bytes -= c->lwe2n_cap * c->lwe_stride * sizeof(uint16_t);
+    retire_scratch(c, c->d_lwe2n, c->lwe2n_cap * c->lwe_stride * sizeof(uint16_t));
   }
-  int rc = dev_alloc(c, (void**)&c->d_lwe2n, cap * c->lwe_stride * sizeof(uint16_t));
-  if (rc) return rc;
+  c->d_lwe2n = nb;
   c->lwe2n_cap = cap;
   return LUTHE_OK;
 }
@@ -110,14 +125,15 @@ int ensure_rename_rows(luthe_ctx* c, size_t n) {
 
 int ensure_jobs(luthe_ctx* c, size_t n) {
   if (n <= c->jobs_cap) return LUTHE_OK;
-  size_t cap = std::max(n, c->jobs_cap * 2);
+  const size_t cap = std::max(n, c->jobs_cap * 2);
+  PbsJob* nb = nullptr;
+  int rc = dev_alloc(c, (void**)&nb, cap * sizeof(PbsJob));
+  if (rc) return rc;
   if (c->d_jobs) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipFree(c->d_jobs));
-    c->dev_bytes -= c->jobs_cap * sizeof(PbsJob);
+    retire_scratch(c, c->d_jobs, c->jobs_cap * sizeof(PbsJob));
   }
-  int rc = dev_alloc(c, (void**)&c->d_jobs, cap * sizeof(PbsJob));
-  if (rc) return rc;
+  c->d_jobs = nb;
   c->jobs_cap = cap;
   return LUTHE_OK;
 }
@@ -129,20 +145,24 @@ int stage_upload(luthe_ctx* c, const std::vector<uint8_t>& bytes) {
   // the pinned buffer may still be read by the previous submission's copy
   HIPCHK(c, hipEventSynchronize(c->stage_copied));
   if (n > c->h_stage_cap) {
-    if (c->h_stage) HIPCHK(c, hipHostFree(c->h_stage));
-    size_t cap = std::max(n, c->h_stage_cap * 2);
-    HIPCHK(c, hipHostMalloc((void**)&c->h_stage, cap, hipHostMallocDefault));
+    const size_t cap = std::max(n, c->h_stage_cap * 2);
+    uint8_t* nh = nullptr;
+    HIPCHK(c, hipHostMalloc((void**)&nh, cap, hipHostMallocDefault));
+    if (c->h_stage) (void)hipHostFree(c->h_stage);
+    c->h_stage = nh;
     c->h_stage_cap = cap;
   }
   if (n > c->stage_cap) {
+    const size_t cap = std::max(n, c->stage_cap * 2);
+    uint8_t* nd = nullptr;
+    int rc = dev_alloc(c, (void**)&nd, cap);
+    if (rc) return rc;
     if (c->d_stage) {
       HIPCHK(c, hipStreamSynchronize(c->stream));
-      HIPCHK(c, hipFree(c->d_stage));
+      (void)hipFree(c->d_stage);
       c->dev_bytes -= c->stage_cap;
     }
-    size_t cap = std::max(n, c->stage_cap * 2);
-    int rc = dev_alloc(c, (void**)&c->d_stage, cap);
-    if (rc) return rc;
+    c->d_stage = nd;
     c->stage_cap = cap;
   }
   memcpy(c->h_stage, bytes.data(), n);
@@ -152,6 +172,8 @@ int stage_upload(luthe_ctx* c, const std::vector<uint8_t>& bytes) {
 }
 
 hipStream_t pick_stream(luthe_ctx* c, void* s) { return s ? (hipStream_t)s : c->stream; }
+
+uint32_t useful_bits(const luthe_ctx* c) { return c->prm.msg_bits + c->prm.carry_bits + 1; }
 
 int latency_max(luthe_ctx* c) {
   if (c->br_latency_max < 0) {
@@ -273,6 +295,8 @@ int luthe_destroy(luthe_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto& kv : c->tickets)
     if (kv.second.done) (void)hipEventDestroy(kv.second.done);
+  for (luthe_graph* g : c->graphs) g->ctx = nullptr;  // their scratch is freed below
+  for (auto& r : c->retired) (void)hipFree(r.first);
   void* bufs[] = {c->d_mem, c->d_luts, c->d_bsk, c->d_ksk, c->d_tab, c->d_lwe2n, c->d_jobs, c->d_stage};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -491,7 +515,7 @@ int luthe_alu_batch(luthe_ctx* c, uint32_t opcode, const uint64_t* d_a, const ui
   if (two && n && !d_b) return LUTHE_ERR_ARG;
   if (n > (size_t)1 << 28) return fail(c, LUTHE_ERR_ARG, "batch too large");
   HIPCHK(c, lk::launch_alu_batch(opcode, d_a, d_b, d_out, mul_factor, msg_cst, (int)n, (int)c->blen,
-                                 pick_stream(c, stream)));
+                                 useful_bits(c), pick_stream(c, stream)));
   c->cnt.alu += n;
   return LUTHE_OK;
 }
@@ -522,21 +546,38 @@ int luthe_capture_end(luthe_ctx* c, luthe_graph** out) {
     luthe_graph_destroy(g);
     return fail(c, LUTHE_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
   }
+  g->ctx = c;
+  c->graphs.push_back(g);
   *out = g;
   return LUTHE_OK;
 }
 
 int luthe_graph_launch(luthe_ctx* c, luthe_graph* g) {
   if (!c || !g || !g->exec) return LUTHE_ERR_ARG;
+  if (g->ctx != c) return fail(c, LUTHE_ERR_ARG, "graph was captured on another context");
   HIPCHK(c, hipGraphLaunch(g->exec, c->stream));
   return LUTHE_OK;
 }
 
 int luthe_graph_destroy(luthe_graph* g) {
   if (!g) return LUTHE_ERR_ARG;
+  luthe_ctx* c = g->ctx;
+  if (c) {
+    // a replay may still be running on the context stream
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    c->graphs.erase(std::remove(c->graphs.begin(), c->graphs.end(), g), c->graphs.end());
+  }
   if (g->exec) (void)hipGraphExecDestroy(g->exec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
   delete g;
+  if (c && c->graphs.empty()) {
+    for (auto& r : c->retired) {
+      (void)hipFree(r.first);
+      c->dev_bytes -= r.second;
+    }
+    c->retired.clear();
+  }
   return LUTHE_OK;
 }
 
@@ -651,7 +692,8 @@ static int execute_program(luthe_ctx* c, const std::vector<dop::Dop>& prog, uint
     if (lp.ncopy)
       HIPCHK(c, lk::launch_copy(c->d_mem, (const CopyOp*)(c->d_stage + lp.copy_off), lp.ncopy, blen, c->stream));
     if (lp.nalu)
-      HIPCHK(c, lk::launch_alu(c->d_mem, (const AluOp*)(c->d_stage + lp.alu_off), lp.nalu, blen, c->stream));
+      HIPCHK(c, lk::launch_alu(c->d_mem, (const AluOp*)(c->d_stage + lp.alu_off), lp.nalu, blen, useful_bits(c),
+                               c->stream));
     if (lp.npbs) {
       rc = run_pbs(c, c->d_mem, (const PbsJob*)(c->d_stage + lp.pbs_off), lp.npbs, c->d_mem, c->stream);
       if (rc) return rc;

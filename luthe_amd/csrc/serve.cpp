@@ -339,7 +339,8 @@ int issue_step(luthe_server* s, std::vector<Request*>& admitted) {
   if (off) SHIP(s, hipMemcpyAsync(sl.dev, sl.host, off, hipMemcpyHostToDevice, st));
   uint64_t nalu = 0;
   for (size_t i = 0; i < pre.size(); i++) {
-    SHIP(s, lk::launch_alu(s->d_arena, (const AluOp*)(sl.dev + pre_off[i].first), (int)pre_off[i].second, blen, st));
+    SHIP(s, lk::launch_alu(s->d_arena, (const AluOp*)(sl.dev + pre_off[i].first), (int)pre_off[i].second, blen,
+                             lx::useful_bits(c), st));
     nalu += pre_off[i].second;
   }
   if (!jobs.empty()) {
@@ -347,7 +348,8 @@ int issue_step(luthe_server* s, std::vector<Request*>& admitted) {
     if (rc) return sfail(s, rc, "cohort: %s", c->err.c_str());
   }
   for (size_t i = 0; i < post.size(); i++) {
-    SHIP(s, lk::launch_alu(s->d_arena, (const AluOp*)(sl.dev + post_off[i].first), (int)post_off[i].second, blen, st));
+    SHIP(s, lk::launch_alu(s->d_arena, (const AluOp*)(sl.dev + post_off[i].first), (int)post_off[i].second, blen,
+                             lx::useful_bits(c), st));
     nalu += post_off[i].second;
   }
   if (!gathers.empty())

@@ -706,13 +706,17 @@ void ref_bsk_to_ntt(const ref_params* prm, const uint32_t* cuts, uint32_t ncut, 
 }
 
 /* ==================================================================================== */
-/* ALU: pea_alu_core.sv:55-160 (USEFUL_BIT = 5, param_tfhe_pkg.sv:32)                     */
+/* ALU: pea_alu_core.sv:55-160.  The scalar constant sits in the top USEFUL_BIT bits of the  */
+/* body: in_msg_cst[USEFUL_BIT-1:0] << (64 - USEFUL_BIT) (pea_alu_core.sv:59), USEFUL_BIT =   */
+/* PAYLOAD_BIT + PADDING_BIT (param_tfhe_pkg.sv:32), i.e. msg + carry + 1 padding bit.      */
 /* ==================================================================================== */
 int ref_alu(uint32_t opcode, size_t len, const uint64_t* a0, const uint64_t* a1, uint32_t mul_factor,
-            uint32_t msg_cst, uint64_t* out) {
+            uint32_t msg_cst, uint32_t useful_bits, uint64_t* out) {
+  if (useful_bits < 1 || useful_bits > 11) return -1;
+  const uint64_t cst_body = ((uint64_t)msg_cst & ((1ull << useful_bits) - 1)) << (64 - useful_bits);
   for (size_t i = 0; i < len; i++) {
     int is_body = (i == len - 1);
-    uint64_t cst = is_body ? ((uint64_t)(msg_cst & 31u) << 59) : 0; /* pe_alu.sv:469-475 */
+    uint64_t cst = is_body ? cst_body : 0; /* body only: pe_alu.sv:469-475 */
     uint64_t x = a0[i];
     switch (opcode) {
       case 0x01: out[i] = x + a1[i]; break;                          /* ADD  */

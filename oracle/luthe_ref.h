@@ -119,9 +119,9 @@ void ref_pbs_batch(const ref_params* prm, const uint32_t* cuts, uint32_t ncut,
 void ref_bsk_to_ntt(const ref_params* prm, const uint32_t* cuts, uint32_t ncut,
                     const uint64_t* bsk_coef, uint64_t* bsk_ntt, int threads);
 
-/* ---- ALU DOps on BLWE (k*N+1 u64), opcode = 6-bit DOp code ---- */
+/* ---- ALU DOps on BLWE (k*N+1 u64), opcode = 6-bit DOp code; useful_bits = msg + carry + 1 ---- */
 int ref_alu(uint32_t opcode, size_t blwe_len, const uint64_t* a0, const uint64_t* a1,
-            uint32_t mul_factor, uint32_t msg_cst, uint64_t* out);
+            uint32_t mul_factor, uint32_t msg_cst, uint32_t useful_bits, uint64_t* out);
 
 #ifdef __cplusplus
 }

@@ -53,7 +53,7 @@ def lib():
             ("ref_ks_to_2n", u32, [u32, u32, u32]),
             ("ref_monomial_mult", None, [u32, u64p, u32, u64p]),
             ("ref_sample_extract", None, [u32, u32, u64p, u32, u32, u64p]),
-            ("ref_alu", C.c_int, [u32, sz, u64p, u64p, u32, u32, u64p]),
+            ("ref_alu", C.c_int, [u32, sz, u64p, u64p, u32, u32, u32, u64p]),
         ]:
             f = getattr(L, name)
             f.restype = res
@@ -170,11 +170,12 @@ def sample_extract(k, N, acc, b2n, log_lut_nb):
     return out
 
 
-def alu(opcode, a0, a1, mul_factor=0, msg_cst=0):
+def alu(opcode, a0, a1, mul_factor=0, msg_cst=0, useful_bits=5):
+    """useful_bits = message + carry bits + 1 padding bit (5 for 2+2-bit blocks)."""
     a0 = _u64(a0)
     a1 = _u64(a1 if a1 is not None else np.zeros_like(a0))
     out = np.empty_like(a0)
-    assert lib().ref_alu(opcode, len(a0), a0, a1, mul_factor, msg_cst, out) == 0
+    assert lib().ref_alu(opcode, len(a0), a0, a1, mul_factor, msg_cst, useful_bits, out) == 0
     return out
 
 

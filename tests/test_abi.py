@@ -307,6 +307,8 @@ def test_keyswitch_parameter_limits():
     """Digits are int8 and the key-switch MACs are 24-bit: parameters outside that are refused
     (the reference's sets use KS_B_W 2-3, KS_L 5-8, MOD_KSK_W 21)."""
     h = C.c_void_p()
-    for over in (dict(ks_base_log=8, ks_level=4), dict(ks_mod_log=25), dict(ks_mod_log=32), dict(ks_base_log=0)):
+    # b*L > w: the gadget weight 2^(w - b*(L-l)) of the top level would need a negative shift
+    for over in (dict(ks_base_log=8, ks_level=4), dict(ks_mod_log=25), dict(ks_mod_log=32), dict(ks_base_log=0),
+                 dict(ks_base_log=7, ks_level=8, ks_mod_log=21), dict(ks_base_log=3, ks_level=8, ks_mod_log=23)):
         p = la.default_params(**over)
         assert la.lib().luthe_create(C.byref(p), 0, C.byref(h)) == -2, over

@@ -193,3 +193,39 @@ def test_error_paths(oracle):
     clamped = ctx.pbs_batch(keys.encrypt_msg([3]), np.array([999999], np.uint32))
     assert np.array_equal(ok, clamped)
     ctx.close()
+
+
+@pytest.mark.parametrize("msg,carry", [(1, 1), (3, 3)])
+def test_alu_constant_other_block_widths(oracle, msg, carry):
+    """ADDS/SUBS/SSUB place the constant in the top msg + carry + 1 bits of the body
+    (pea_alu_core.sv:59, USEFUL_BIT of param_tfhe_pkg.sv:32): checked for blocks other than
+    2+2 bits, on the batched ALU and on the DOp path, against the oracle."""
+    useful = msg + carry + 1
+    print(f"case: ALU constant, {msg}+{carry}-bit blocks (useful {useful} bits)", flush=True)
+    prm = la.default_params(lwe_dim=16, msg_bits=msg, carry_bits=carry, ct_slots=64)
+    ctx = la.Context(prm, device=0)
+    try:
+        rng = np.random.default_rng(60 + useful)
+        n = 5
+        a = rng.integers(0, 2**64, (n, prm.blen), dtype=np.uint64)
+        b = rng.integers(0, 2**64, (n, prm.blen), dtype=np.uint64)
+        da, db = la.DeviceBuffer.from_numpy(a), la.DeviceBuffer.from_numpy(b)
+        dout = la.DeviceBuffer((n, prm.blen), np.uint64)
+        cst = 0x5B3  # 11-bit field, wider than the useful width
+        for op in (dop.ADDS, dop.SUBS, dop.SSUB, dop.MULS):
+            ctx.alu_batch_dev(op, da, db, dout, n, msg_cst=cst)
+            ctx.sync()
+            got = dout.numpy()
+            for c in range(n):
+                exp = oracle.alu(op, a[c], None, msg_cst=cst, useful_bits=useful)
+                assert np.array_equal(got[c], exp), (hex(op), c)
+        # DOp path: LD, ADDS / SSUB on registers, ST
+        ctx.write_ct(0, a[:2])
+        words = [dop.ld(1, 0), dop.ld(2, 1), dop.adds(3, 1, cst), dop.ssub(4, 2, cst),
+                 dop.st(10, 3), dop.st(11, 4)]
+        ctx.wait(ctx.submit_dops(words))
+        got = ctx.read_ct(10, 2)
+        assert np.array_equal(got[0], oracle.alu(dop.ADDS, a[0], None, msg_cst=cst, useful_bits=useful))
+        assert np.array_equal(got[1], oracle.alu(dop.SSUB, a[1], None, msg_cst=cst, useful_bits=useful))
+    finally:
+        ctx.close()
