@@ -218,6 +218,22 @@ int luthe_ntt_inverse_batch(luthe_ctx* ctx, const uint64_t* d_in, uint64_t* d_ou
 int luthe_alu_batch(luthe_ctx* ctx, uint32_t opcode, const uint64_t* d_a, const uint64_t* d_b, uint64_t* d_out,
                     uint32_t mul_factor, uint32_t msg_cst, size_t n, void* stream);
 
+/* luthe_pbs_batch without any allocation: no hipMalloc, hipFree or allocating library call,
+ * at any batch size it accepts.  Same kernels and results as luthe_pbs_batch; its scratch
+ * (job descriptors and mod-switched LWEs) is a fixed device array of 16.125 MiB that each GPU
+ * holds for the life of the process, so batches are limited to n <= LUTHE_NOALLOC_MAX_BATCH and
+ * n * lwe_stride <= LUTHE_NOALLOC_MAX_LWE_WORDS, with lwe_stride = lwe_dim + 1 rounded up to a
+ * multiple of 8 (n <= 8192 at lwe_dim <= 1016); larger batches return LUTHE_ERR_ARG before any
+ * work.  All calls on one GPU share that array: calls that overlap in time on one GPU (any
+ * stream, context or thread) race and must be serialised by the caller.  Capturable in a HIP
+ * graph without luthe_reserve.  No counterpart in the reference (its PBS unit has fixed
+ * on-chip buffers); for callers that cannot allocate: code under graph capture, tight serving
+ * loops, harnesses that forbid allocation. */
+#define LUTHE_NOALLOC_MAX_BATCH 8192
+#define LUTHE_NOALLOC_MAX_LWE_WORDS 8388608
+int luthe_pbs_batch_noalloc(luthe_ctx* ctx, const uint64_t* d_in, uint64_t* d_out, const uint32_t* d_gid,
+                            uint32_t log_lut_nb, size_t n, void* stream);
+
 /* Pre-size the per-context batch scratch for batches of up to max_batch ciphertexts,
  * so later batch calls allocate nothing (required before graph capture). */
 int luthe_reserve(luthe_ctx* ctx, size_t max_batch);

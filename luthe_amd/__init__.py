@@ -163,6 +163,7 @@ def lib():
         "luthe_register_iop": (i32, [vp, C.c_uint8, C.c_uint8, vp, sz]),
         "luthe_submit_iop": (i32, [vp, vp, sz, C.POINTER(u64)]),
         "luthe_pbs_batch": (i32, [vp, vp, vp, vp, u32, sz, vp]),
+        "luthe_pbs_batch_noalloc": (i32, [vp, vp, vp, vp, u32, sz, vp]),
         "luthe_keyswitch_batch": (i32, [vp, vp, vp, sz, vp]),
         "luthe_blind_rotate_batch": (i32, [vp, vp, vp, vp, u32, sz, vp]),
         "luthe_ntt_polymul_batch": (i32, [vp, vp, vp, vp, sz, vp]),

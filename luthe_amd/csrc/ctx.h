@@ -82,6 +82,11 @@ int ensure_rename_rows(luthe_ctx* c, size_t n);
 // device; rows are relative to in_base / out_base.
 int run_pbs(luthe_ctx* c, const uint64_t* in_base, const lk::PbsJob* d_jobs, int n, uint64_t* out_base,
             hipStream_t s);
+// The same with the mod-switched LWEs in caller-provided device memory; allocates nothing.
+int run_pbs_on(luthe_ctx* c, const uint64_t* in_base, const lk::PbsJob* d_jobs, int n, uint64_t* out_base,
+               hipStream_t s, uint16_t* lwe2n);
+// Looks up the no-allocation entry point's device scratch on `device` (current).
+int noalloc_init(luthe_ctx* c, int device);
 // Width of the ALU scalar constant in the body: msg + carry + 1 padding bit.
 uint32_t useful_bits(const luthe_ctx* c);
 // Largest cohort that runs the 512-thread latency kernel.

@@ -183,22 +183,31 @@ int latency_max(luthe_ctx* c) {
   return c->br_latency_max;
 }
 
-// KS + BR over jobs already on the device.
-int run_pbs(luthe_ctx* c, const uint64_t* in_base, const PbsJob* d_jobs, int n, uint64_t* out_base, hipStream_t s) {
+// KS + BR over jobs already on the device, with the mod-switched LWEs in `lwe2n`
+// (n rows of lwe_stride values).  Allocates nothing.
+int run_pbs_on(luthe_ctx* c, const uint64_t* in_base, const PbsJob* d_jobs, int n, uint64_t* out_base, hipStream_t s,
+               uint16_t* lwe2n) {
   if (!c->bsk_ok || !c->ksk_ok) return fail(c, LUTHE_ERR_STATE, "keys not uploaded");
   if (n == 0) return LUTHE_OK;
-  int rc = ensure_lwe2n(c, (size_t)n);
-  if (rc) return rc;
   const luthe_params& p = c->prm;
   lk::KsArgs ka{(int)(p.glwe_k * p.poly_size), (int)p.lwe_dim, c->lwe_stride, p.ks_base_log, p.ks_mod_log, 12};
-  HIPCHK(c, lk::launch_keyswitch(in_base, d_jobs, n, c->d_ksk, c->d_lwe2n, ka, (int)p.ks_level, s));
+  HIPCHK(c, lk::launch_keyswitch(in_base, d_jobs, n, c->d_ksk, lwe2n, ka, (int)p.ks_level, s));
   lk::BrArgs ba{(int)p.lwe_dim, c->lwe_stride, p.pbs_base_log};
-  HIPCHK(c, lk::launch_blind_rotate(c->d_bsk, c->d_luts, c->d_lwe2n, d_jobs, n, out_base, c->d_tab, ba,
+  HIPCHK(c, lk::launch_blind_rotate(c->d_bsk, c->d_luts, lwe2n, d_jobs, n, out_base, c->d_tab, ba,
                                     (int)p.pbs_level, c->br_grid_override, s, latency_max(c)));
   c->cnt.pbs += (uint64_t)n;
   c->cnt.pbs_batches += 1;
   c->cnt.pbs_max_batch = std::max<uint64_t>(c->cnt.pbs_max_batch, (uint64_t)n);
   return LUTHE_OK;
+}
+
+// KS + BR over jobs already on the device, on the context's (grown as needed) scratch.
+int run_pbs(luthe_ctx* c, const uint64_t* in_base, const PbsJob* d_jobs, int n, uint64_t* out_base, hipStream_t s) {
+  if (!c->bsk_ok || !c->ksk_ok) return fail(c, LUTHE_ERR_STATE, "keys not uploaded");
+  if (n == 0) return LUTHE_OK;
+  int rc = ensure_lwe2n(c, (size_t)n);
+  if (rc) return rc;
+  return run_pbs_on(c, in_base, d_jobs, n, out_base, s, c->d_lwe2n);
 }
 
 void trace_push(luthe_ctx* c, const luthe_trace_event& e) {
@@ -274,6 +283,7 @@ int luthe_create(const luthe_params* p, int device, luthe_ctx** out) {
   };
   if (hipSetDevice(device) != hipSuccess) return bail(LUTHE_ERR_HIP);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail(LUTHE_ERR_HIP);
+  if ((rc = noalloc_init(c, device))) return bail(rc);
   if (hipEventCreateWithFlags(&c->stage_copied, hipEventDisableTiming) != hipSuccess) return bail(LUTHE_ERR_HIP);
   size_t rows = (size_t)p->regfile_size + p->ct_slots;
   if ((rc = dev_alloc(c, (void**)&c->d_mem, rows * c->blen * 8))) return bail(rc);
