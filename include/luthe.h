@@ -82,6 +82,7 @@ typedef struct luthe_counters {
   uint64_t iops;           /* IOps accepted */
   uint64_t tickets;        /* submissions */
   uint64_t levels;         /* dependency levels scheduled */
+  uint64_t groups;         /* DOp groups launched (submissions scheduled together) */
 } luthe_counters;
 
 /* ---------------------------------------------------------------- context */
@@ -135,15 +136,42 @@ int luthe_read_reg(luthe_ctx* ctx, uint32_t rid, uint64_t* blwe, uint32_t count)
  * opcodes of hpu_common_instruction_pkg.sv:42-68).  The whole stream is validated
  * first (nothing runs if any word is invalid); memory operands must be concrete
  * (MEM_ADDR) and immediates constant (IMM_CST).  DOps run as soon as their register
- * and slot dependencies allow, PBS DOps of the same dependency level sharing one
- * batched launch; results equal in-order execution.  SYNC words are accepted and
+ * and slot dependencies allow, PBS DOps of the same dependency level -- of this and of
+ * the other submissions launched with it (luthe_set_dop_batching) -- sharing one batched
+ * launch; results equal in-order execution of all submissions.  SYNC words are accepted and
  * order nothing further (completion is per ticket).  Replaces the firmware's DOp
  * stream to the scheduler (ucore_fw.c:96-110) and its ack path (ucore_fw.c:224-232). */
 int luthe_submit_dops(luthe_ctx* ctx, const uint32_t* words, size_t n, uint64_t* ticket);
-/* 1 = done, 0 = pending, < 0 = error. */
+/* 1 = done, 0 = pending, < 0 = error.  Polling a ticket that has not been launched yet
+ * launches its group. */
 int luthe_poll(luthe_ctx* ctx, uint64_t ticket);
 /* Block until the ticket completes; timeout_us < 0 waits forever. */
 int luthe_wait(luthe_ctx* ctx, uint64_t ticket, int64_t timeout_us);
+
+/* Continuous batching across submissions.  DOp and IOp submissions join an open group; a
+ * group is scheduled as one program (the streams concatenated in submission order, so
+ * results equal running the submissions one at a time) and the PBS of all its tickets that
+ * are ready at one dependency level share one batched launch.  The open group is launched
+ * when fewer than max_groups launched groups are still running and
+ *   fill_pbs = 0 (default): at once -- submissions made while a group runs merge into the next;
+ *   fill_pbs > 0: once it holds fill_pbs PBS, a PBS carries the flush bit (PBS_F, docs/dop.md:
+ *     108-111), or its oldest submission has waited timeout_us (0 = no timeout);
+ * and in any case when the host needs a result: luthe_poll / luthe_wait on one of its
+ * tickets, luthe_flush, luthe_sync, ct / register / LUT reads and writes, graph capture.  The
+ * rules are evaluated whenever the host calls into the context (there is no timer thread).
+ * Replaces the BPIP batch former's start conditions (pep_sequencer.sv:584-608: full batch,
+ * timeout, flush) and its opportunistic mode (bpip_use_opportunism / bpip_timeout,
+ * hpu_config.toml.j2:9-11) over the instruction scheduler's DOp pool
+ * (instruction_scheduler.sv:1-20). */
+typedef struct luthe_dop_batching {
+  uint32_t fill_pbs;    /* 0 = launch whenever a group slot is free */
+  uint32_t timeout_us;  /* with fill_pbs > 0: launch once the oldest submission waited this long */
+  uint32_t max_groups;  /* launched groups allowed to run at once, 1..64 (default 1) */
+} luthe_dop_batching;
+int luthe_set_dop_batching(luthe_ctx* ctx, const luthe_dop_batching* b);
+int luthe_get_dop_batching(const luthe_ctx* ctx, luthe_dop_batching* out);
+/* Launch the open group now (the host-side flush trigger). */
+int luthe_flush(luthe_ctx* ctx);
 
 /* ---------------------------------------------------------------- IOp path */
 

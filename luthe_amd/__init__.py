@@ -56,7 +56,11 @@ class Params(C.Structure):
 
 class Counters(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("pbs", "pbs_batches", "pbs_max_batch", "alu", "mem", "dops",
-                                          "iops", "tickets", "levels")]
+                                          "iops", "tickets", "levels", "groups")]
+
+
+class DopBatching(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("fill_pbs", "timeout_us", "max_groups")]
 
 
 class IopInfo(C.Structure):
@@ -164,6 +168,9 @@ def lib():
         "luthe_submit_iop": (i32, [vp, vp, sz, C.POINTER(u64)]),
         "luthe_pbs_batch": (i32, [vp, vp, vp, vp, u32, sz, vp]),
         "luthe_pbs_batch_noalloc": (i32, [vp, vp, vp, vp, u32, sz, vp]),
+        "luthe_flush": (i32, [vp]),
+        "luthe_set_dop_batching": (i32, [vp, C.POINTER(DopBatching)]),
+        "luthe_get_dop_batching": (i32, [vp, C.POINTER(DopBatching)]),
         "luthe_keyswitch_batch": (i32, [vp, vp, vp, sz, vp]),
         "luthe_blind_rotate_batch": (i32, [vp, vp, vp, vp, u32, sz, vp]),
         "luthe_ntt_polymul_batch": (i32, [vp, vp, vp, vp, sz, vp]),
@@ -435,6 +442,19 @@ class Context:
 
     def wait(self, ticket, timeout_us=-1):
         self._chk(lib().luthe_wait(self._h, ticket, timeout_us), "wait")
+
+    def flush(self):
+        """Launch the open group of DOp/IOp submissions now."""
+        self._chk(lib().luthe_flush(self._h), "flush")
+
+    def set_dop_batching(self, fill_pbs=0, timeout_us=0, max_groups=1):
+        b = DopBatching(fill_pbs, timeout_us, max_groups)
+        self._chk(lib().luthe_set_dop_batching(self._h, C.byref(b)), "set_dop_batching")
+
+    def dop_batching(self):
+        b = DopBatching()
+        self._chk(lib().luthe_get_dop_batching(self._h, C.byref(b)), "get_dop_batching")
+        return {n: getattr(b, n) for n, _ in DopBatching._fields_}
 
     def sync(self):
         self._chk(lib().luthe_sync(self._h), "sync")

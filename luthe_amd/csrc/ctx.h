@@ -9,10 +9,19 @@
 #include <vector>
 
 #include "../../include/luthe.h"
+#include "dop.h"
 #include "kernels.h"
 
+// A submission: in the open group (group 0) until sealed, then in a launched group.
 struct Ticket {
+  uint64_t group = 0;
+  int status = 0;  // error of a group that failed to launch
+};
+
+// Tickets scheduled and launched together; `done` is recorded after its last level.
+struct DopGroup {
   hipEvent_t done = nullptr;
+  uint32_t tickets = 0;  // tickets not yet released
 };
 
 struct luthe_graph {
@@ -55,6 +64,18 @@ struct luthe_ctx {
   luthe_counters cnt{};
   uint64_t next_ticket = 1;
   std::map<uint64_t, Ticket> tickets;
+  // DOp/IOp submissions: the open group (streams concatenated in submission order) and the
+  // launched groups; groups_running lists launched groups not yet seen complete, oldest first.
+  std::vector<dop::Dop> open_prog;
+  std::vector<uint64_t> open_dop_ticket;  // ticket of each open DOp
+  std::vector<uint64_t> open_ids;         // tickets of the open group
+  uint32_t open_pbs = 0;
+  bool open_flush = false;                // a PBS of the open group carries the flush bit
+  std::chrono::steady_clock::time_point open_since;
+  std::map<uint64_t, DopGroup> groups;
+  std::deque<uint64_t> groups_running;
+  uint64_t next_group = 1;
+  luthe_dop_batching batching{0, 0, 1};
   std::map<uint32_t, std::vector<uint32_t>> iop_table;  // key = max_align << 8 | opcode
   int br_grid_override = 0;
   int br_latency_max = -1;  // cohorts up to this size use the latency kernel; -1 = its resident capacity
@@ -87,6 +108,9 @@ int run_pbs_on(luthe_ctx* c, const uint64_t* in_base, const lk::PbsJob* d_jobs, 
                hipStream_t s, uint16_t* lwe2n);
 // Looks up the no-allocation entry point's device scratch on `device` (current).
 int noalloc_init(luthe_ctx* c, int device);
+// DOp groups: seal (launch) the open group now; check the sealing rules (force: seal).
+int dop_seal(luthe_ctx* c);
+int dop_progress(luthe_ctx* c, bool force);
 // Width of the ALU scalar constant in the body: msg + carry + 1 padding bit.
 uint32_t useful_bits(const luthe_ctx* c);
 // Largest cohort that runs the 512-thread latency kernel.

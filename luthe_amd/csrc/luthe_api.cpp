@@ -303,7 +303,7 @@ int luthe_destroy(luthe_ctx* c) {
   if (c->server) return fail(c, LUTHE_ERR_STATE, "destroy the attached server first");
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (auto& kv : c->tickets)
+  for (auto& kv : c->groups)  // open (unlaunched) tickets are dropped
     if (kv.second.done) (void)hipEventDestroy(kv.second.done);
   for (luthe_graph* g : c->graphs) g->ctx = nullptr;  // their scratch is freed below
   for (auto& r : c->retired) (void)hipFree(r.first);
@@ -388,6 +388,8 @@ int luthe_microbench_modmul(luthe_ctx* c, double* gmodmul_per_s) {
 
 int luthe_sync(luthe_ctx* c) {
   if (!c) return LUTHE_ERR_ARG;
+  int rc = dop_progress(c, true);
+  if (rc) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return LUTHE_OK;
 }
@@ -437,6 +439,8 @@ static int rows_io(luthe_ctx* c, uint64_t* dbase, uint32_t first, uint32_t count
   if (!c || (!hin && !hout)) return LUTHE_ERR_ARG;
   if ((uint64_t)first + count > limit) return fail(c, LUTHE_ERR_RANGE, "%s %u+%u out of range %u", what, first, count, limit);
   HIPCHK(c, hipSetDevice(c->device));
+  int rc = dop_progress(c, true);  // submitted DOps come first, in submission order
+  if (rc) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (hin)
     HIPCHK(c, hipMemcpy(dbase + (size_t)first * len, hin, (size_t)count * len * 8, hipMemcpyHostToDevice));
@@ -542,6 +546,8 @@ int luthe_capture_begin(luthe_ctx* c) {
   if (!c) return LUTHE_ERR_ARG;
   if (c->server) return fail(c, LUTHE_ERR_BUSY, "context stream is owned by an attached server");
   HIPCHK(c, hipSetDevice(c->device));
+  int rc = dop_progress(c, true);  // submitted DOps are launched, not captured
+  if (rc) return rc;
   HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   return LUTHE_OK;
 }
@@ -608,9 +614,27 @@ int luthe_ntt_inverse_batch(luthe_ctx* c, const uint64_t* d_in, uint64_t* d_out,
   return LUTHE_OK;
 }
 
+}  // extern "C"
+
 // ---------------------------------------------------------------- DOp stream
-static int execute_program(luthe_ctx* c, const std::vector<dop::Dop>& prog, uint64_t* ticket) {
-  if (c->server) return fail(c, LUTHE_ERR_BUSY, "context stream is owned by an attached server");
+// Tickets are not launched one by one: submissions join an open group, and a group is
+// scheduled as one DOp program (the tickets' streams concatenated in submission order), so
+// the ready PBS of every ticket in it share one key-switch + blind-rotation launch per
+// dependency level.  Concatenation keeps in-order semantics across tickets (register
+// renaming and slot hazards span the whole group), and every PBS is a pure function of its
+// input and LUT, so each ticket's results equal running the tickets one at a time.
+// A group is sealed (launched) when a group slot is free (fewer than max_groups sealed
+// groups still running) and the fill rule holds -- always when fill_pbs = 0, else once it
+// holds fill_pbs PBS, a PBS carries the flush bit, or its oldest ticket waited timeout_us --
+// and at once when the host needs a result (poll or wait on one of its tickets, luthe_flush,
+// memory reads and writes, luthe_sync, capture, destroy).  The rules are checked whenever
+// the host calls into the context.  This is the BPIP batch former (pep_sequencer.sv:584-608:
+// full batch, timeout, flush) with opportunistic start (bpip_use_opportunism,
+// hpu_config.toml.j2:9-11) applied to whole dependency-level schedules.
+
+// Schedules and launches one group; returns its id (0 for an empty program).
+static int execute_group(luthe_ctx* c, const std::vector<dop::Dop>& prog, const std::vector<uint64_t>& dop_ticket,
+                         uint64_t* group_id) {
   const luthe_params& p = c->prm;
   const uint32_t R = p.regfile_size;
   // Register renaming: each register write lands in a fresh row after the ciphertext
@@ -676,25 +700,21 @@ static int execute_program(luthe_ctx* c, const std::vector<dop::Dop>& prog, uint
   rc = stage_upload(c, blob);
   if (rc) return rc;
   const int blen = (int)c->blen;
-  // trace: REFILL of every DOp now, ISSUE per level as it is launched, RETIRE at completion
-  const uint64_t id = c->next_ticket;
+  // trace: ISSUE per level as it is launched; RETIRE events (recorded at submission) are
+  // stamped when their ticket completes
   std::vector<std::vector<uint32_t>> by_level;
   if (c->trace_cap) {
     by_level.resize(nlev);
-    const uint32_t now = trace_now(c);
-    std::vector<luthe_trace_event>& ret = c->trace_retire[id];
+    std::map<uint64_t, size_t> nth;  // position of the DOp in its own submission
     for (size_t i = 0; i < prog.size(); i++) {
-      luthe_trace_event e{};
-      e.timestamp = now;
-      e.insn = prog[i].word;
-      e.cmd = LUTHE_TRC_REFILL;
-      e.sync_id = (uint8_t)(id & 15);
-      e.vld = 1;
-      trace_push(c, e);
-      if (level[i] >= 0) by_level[level[i]].push_back((uint32_t)i);
-      e.cmd = LUTHE_TRC_RETIRE;
-      e.pe_reserved = (level[i] >= 0 && prog[i].kind == dop::KIND_PBS) ? (uint32_t)pbss[level[i]].size() : 0;
-      ret.push_back(e);
+      const size_t k = nth[dop_ticket[i]]++;
+      if (level[i] < 0) continue;
+      by_level[level[i]].push_back((uint32_t)i);
+      if (prog[i].kind == dop::KIND_PBS) {  // RETIRE of a PBS: the size of the launch it ran in
+        auto it = c->trace_retire.find(dop_ticket[i]);
+        if (it != c->trace_retire.end() && k < it->second.size())
+          it->second[k].pe_reserved = (uint32_t)pbss[level[i]].size();
+      }
     }
   }
   for (int L = 0; L < nlev; L++) {
@@ -717,7 +737,7 @@ static int execute_program(luthe_ctx* c, const std::vector<dop::Dop>& prog, uint
         e.timestamp = now;
         e.insn = prog[i].word;
         e.cmd = LUTHE_TRC_ISSUE;
-        e.sync_id = (uint8_t)(id & 15);
+        e.sync_id = (uint8_t)(dop_ticket[i] & 15);
         e.vld = 1;
         e.pdg = 1;
         trace_push(c, e);
@@ -725,16 +745,138 @@ static int execute_program(luthe_ctx* c, const std::vector<dop::Dop>& prog, uint
     }
   }
   c->cnt.levels += (uint64_t)nlev;
-  c->cnt.dops += prog.size();
-  c->cnt.tickets += 1;
-  Ticket tk;
-  HIPCHK(c, hipEventCreateWithFlags(&tk.done, hipEventDisableTiming));
-  HIPCHK(c, hipEventRecord(tk.done, c->stream));
-  c->next_ticket++;
-  c->tickets[id] = tk;
-  if (ticket) *ticket = id;
+  c->cnt.groups += 1;
+  DopGroup g;
+  HIPCHK(c, hipEventCreateWithFlags(&g.done, hipEventDisableTiming));
+  HIPCHK(c, hipEventRecord(g.done, c->stream));
+  const uint64_t gid = c->next_group++;
+  c->groups[gid] = g;
+  c->groups_running.push_back(gid);
+  *group_id = gid;
   return LUTHE_OK;
 }
+
+namespace lx {
+
+// Launches the open group now.  On failure its tickets complete with the error.
+int dop_seal(luthe_ctx* c) {
+  if (c->open_ids.empty()) return LUTHE_OK;
+  std::vector<dop::Dop> prog;
+  std::vector<uint64_t> ids, dop_ticket;
+  prog.swap(c->open_prog);
+  ids.swap(c->open_ids);
+  dop_ticket.swap(c->open_dop_ticket);
+  c->open_pbs = 0;
+  c->open_flush = false;
+  uint64_t gid = 0;
+  const int rc = execute_group(c, prog, dop_ticket, &gid);
+  for (uint64_t id : ids) {
+    Ticket& t = c->tickets[id];
+    if (rc) {
+      t.status = rc;
+    } else {
+      t.group = gid;
+      c->groups[gid].tickets++;
+    }
+  }
+  return rc;
+}
+
+// Checks the sealing rules; force seals the open group whatever they say.
+int dop_progress(luthe_ctx* c, bool force) {
+  while (!c->groups_running.empty()) {  // sealed groups that have finished free their slot
+    auto it = c->groups.find(c->groups_running.front());
+    if (it != c->groups.end() && hipEventQuery(it->second.done) == hipErrorNotReady) break;
+    c->groups_running.pop_front();
+  }
+  if (c->open_ids.empty()) return LUTHE_OK;
+  if (!force) {
+    const luthe_dop_batching& b = c->batching;
+    if (c->groups_running.size() >= b.max_groups) return LUTHE_OK;
+    if (b.fill_pbs) {
+      const double waited = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() -
+                                                                      c->open_since).count();
+      const bool due = c->open_pbs >= b.fill_pbs || c->open_flush || (b.timeout_us && waited >= b.timeout_us);
+      if (!due) return LUTHE_OK;
+    }
+  }
+  return dop_seal(c);
+}
+
+}  // namespace lx
+
+// Appends a validated stream to the open group as a new ticket.
+static int enqueue_ticket(luthe_ctx* c, std::vector<dop::Dop>& prog, uint64_t* ticket) {
+  int rc = dop_progress(c, false);  // frees finished group slots; may seal the open group
+  if (rc) return rc;
+  const uint64_t id = c->next_ticket++;
+  if (c->open_ids.empty()) c->open_since = std::chrono::steady_clock::now();
+  if (c->trace_cap) {  // REFILL: the DOps enter the pool
+    const uint32_t now = trace_now(c);
+    std::vector<luthe_trace_event>& ret = c->trace_retire[id];
+    for (const dop::Dop& d : prog) {
+      luthe_trace_event e{};
+      e.timestamp = now;
+      e.insn = d.word;
+      e.cmd = LUTHE_TRC_REFILL;
+      e.sync_id = (uint8_t)(id & 15);
+      e.vld = 1;
+      trace_push(c, e);
+      e.cmd = LUTHE_TRC_RETIRE;
+      ret.push_back(e);
+    }
+  }
+  for (const dop::Dop& d : prog) {
+    c->open_dop_ticket.push_back(id);
+    if (d.kind == dop::KIND_PBS) {
+      c->open_pbs++;
+      if (d.flush) c->open_flush = true;
+    }
+  }
+  c->open_prog.insert(c->open_prog.end(), prog.begin(), prog.end());
+  c->open_ids.push_back(id);
+  c->tickets[id] = Ticket{};
+  c->cnt.dops += prog.size();
+  c->cnt.tickets += 1;
+  if (ticket) *ticket = id;
+  if (c->open_prog.size() >= ((size_t)1 << 20)) return dop_seal(c);  // bounds one schedule's size
+  return dop_progress(c, false);
+}
+
+// Releases a completed ticket (and its group's event once no ticket refers to it).
+static void release_ticket(luthe_ctx* c, std::map<uint64_t, Ticket>::iterator it) {
+  const uint64_t gid = it->second.group;
+  const uint64_t id = it->first;
+  c->tickets.erase(it);
+  trace_retire(c, id);
+  auto g = c->groups.find(gid);
+  if (g != c->groups.end() && --g->second.tickets == 0) {
+    (void)hipEventDestroy(g->second.done);
+    c->groups_running.erase(std::remove(c->groups_running.begin(), c->groups_running.end(), gid),
+                            c->groups_running.end());
+    c->groups.erase(g);
+  }
+}
+
+// The ticket's group, sealing the open group first if the ticket is in it.  Returns the
+// ticket's error status if its group failed to launch.
+static int ticket_group(luthe_ctx* c, uint64_t ticket, std::map<uint64_t, Ticket>::iterator& it) {
+  it = c->tickets.find(ticket);
+  if (it == c->tickets.end()) return fail(c, LUTHE_ERR_STATE, "unknown ticket %llu", (unsigned long long)ticket);
+  if (it->second.group == 0 && it->second.status == 0) {
+    (void)dop_seal(c);
+    it = c->tickets.find(ticket);
+  }
+  const int st = it->second.status;
+  if (st) {
+    c->tickets.erase(it);
+    c->trace_retire.erase(ticket);
+    return st;
+  }
+  return LUTHE_OK;
+}
+
+extern "C" {
 
 int luthe_submit_dops(luthe_ctx* c, const uint32_t* words, size_t n, uint64_t* ticket) {
   if (!c || (!words && n)) return LUTHE_ERR_ARG;
@@ -743,32 +885,34 @@ int luthe_submit_dops(luthe_ctx* c, const uint32_t* words, size_t n, uint64_t* t
   std::string why;
   int rc = dop::decode_stream(words, n, c->prm.regfile_size, c->prm.ct_slots, c->prm.lut_slots, prog, why);
   if (rc) return fail(c, rc, "%s", why.c_str());
-  return execute_program(c, prog, ticket);
+  return enqueue_ticket(c, prog, ticket);
 }
 
 int luthe_poll(luthe_ctx* c, uint64_t ticket) {
   if (!c) return LUTHE_ERR_ARG;
-  auto it = c->tickets.find(ticket);
-  if (it == c->tickets.end()) return fail(c, LUTHE_ERR_STATE, "unknown ticket %llu", (unsigned long long)ticket);
-  hipError_t e = hipEventQuery(it->second.done);
+  std::map<uint64_t, Ticket>::iterator it;
+  int rc = ticket_group(c, ticket, it);
+  if (rc) return rc;
+  (void)dop_progress(c, false);
+  hipError_t e = hipEventQuery(c->groups[it->second.group].done);
   if (e == hipErrorNotReady) return 0;
   if (e != hipSuccess) return fail(c, LUTHE_ERR_HIP, "ticket: %s", hipGetErrorString(e));
-  (void)hipEventDestroy(it->second.done);
-  c->tickets.erase(it);
-  trace_retire(c, ticket);
+  release_ticket(c, it);
   return 1;
 }
 
 int luthe_wait(luthe_ctx* c, uint64_t ticket, int64_t timeout_us) {
   if (!c) return LUTHE_ERR_ARG;
-  auto it = c->tickets.find(ticket);
-  if (it == c->tickets.end()) return fail(c, LUTHE_ERR_STATE, "unknown ticket %llu", (unsigned long long)ticket);
+  std::map<uint64_t, Ticket>::iterator it;
+  int rc = ticket_group(c, ticket, it);
+  if (rc) return rc;
+  hipEvent_t done = c->groups[it->second.group].done;
   if (timeout_us < 0) {
-    HIPCHK(c, hipEventSynchronize(it->second.done));
+    HIPCHK(c, hipEventSynchronize(done));
   } else {
     auto t0 = std::chrono::steady_clock::now();
     for (;;) {
-      hipError_t e = hipEventQuery(it->second.done);
+      hipError_t e = hipEventQuery(done);
       if (e == hipSuccess) break;
       if (e != hipErrorNotReady) return fail(c, LUTHE_ERR_HIP, "ticket: %s", hipGetErrorString(e));
       auto us = std::chrono::duration_cast<std::chrono::microseconds>(std::chrono::steady_clock::now() - t0).count();
@@ -777,9 +921,28 @@ int luthe_wait(luthe_ctx* c, uint64_t ticket, int64_t timeout_us) {
       std::this_thread::sleep_for(std::chrono::microseconds(20));
     }
   }
-  (void)hipEventDestroy(it->second.done);
-  c->tickets.erase(it);
-  trace_retire(c, ticket);
+  release_ticket(c, it);
+  (void)dop_progress(c, false);  // a group slot is free: the open group may start
+  return LUTHE_OK;
+}
+
+int luthe_flush(luthe_ctx* c) {
+  if (!c) return LUTHE_ERR_ARG;
+  return dop_progress(c, true);
+}
+
+int luthe_set_dop_batching(luthe_ctx* c, const luthe_dop_batching* b) {
+  if (!c || !b) return LUTHE_ERR_ARG;
+  if (b->max_groups < 1 || b->max_groups > 64) return fail(c, LUTHE_ERR_ARG, "max_groups must be in [1, 64]");
+  int rc = dop_progress(c, true);  // the open group was formed under the old rules
+  if (rc) return rc;
+  c->batching = *b;
+  return LUTHE_OK;
+}
+
+int luthe_get_dop_batching(const luthe_ctx* c, luthe_dop_batching* out) {
+  if (!c || !out) return LUTHE_ERR_ARG;
+  *out = c->batching;
   return LUTHE_OK;
 }
 
@@ -881,7 +1044,7 @@ int luthe_submit_iop(luthe_ctx* c, const uint32_t* words, size_t n, uint64_t* ti
                           why);
   if (rc) return fail(c, rc, "%s", why.c_str());
   c->cnt.iops += 1;
-  return execute_program(c, prog, ticket);
+  return enqueue_ticket(c, prog, ticket);
 }
 
 }  // extern "C"

@@ -527,6 +527,10 @@ int luthe_server_create(luthe_ctx* c, const luthe_server_config* cfg_in, luthe_s
   if (!c || !out) return LUTHE_ERR_ARG;
   *out = nullptr;
   if (c->server) return lx::fail(c, LUTHE_ERR_STATE, "a server is already attached to this context");
+  {
+    int rc0 = lx::dop_progress(c, true);  // submitted DOps run before the server owns the stream
+    if (rc0) return rc0;
+  }
   if (!c->bsk_ok || !c->ksk_ok) return lx::fail(c, LUTHE_ERR_STATE, "keys not uploaded");
   const luthe_params& p = c->prm;
   if (p.msg_bits != 2 || p.carry_bits != 2)
