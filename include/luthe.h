@@ -221,6 +221,25 @@ int luthe_iop_translate(const uint32_t* iop_words, size_t n_iop, const uint32_t*
 int luthe_pbs_batch(luthe_ctx* ctx, const uint64_t* d_in, uint64_t* d_out, const uint32_t* d_gid,
                     uint32_t log_lut_nb, size_t n, void* stream);
 
+/* Several independent batches -- each with its own input, output and LUT-id arrays, sizes and
+ * Many-LUT count -- as one key-switch launch and one persistent blind-rotation launch, so
+ * uneven batch sizes do not leave the GPU partly idle at each batch's tail.  d_descs: `count`
+ * descriptors in device memory (read by the device, so the call can be captured in a HIP graph
+ * and replayed; a descriptor's log_lut_nb is clamped to 3).  total must be the sum of the
+ * descriptors' n: PBS beyond total are not run, and slots of total that no descriptor fills
+ * run nothing.  Results equal luthe_pbs_batch on each batch.  Replaces the HPU's batch
+ * commands over its PBS pool (pep_common_param_pkg.sv:95-159), which mix PBS of any origin in
+ * one batch. */
+typedef struct luthe_pbs_batch_desc {
+  const uint64_t* d_in;   /* n BLWEs */
+  uint64_t* d_out;        /* n << log_lut_nb BLWEs */
+  const uint32_t* d_gid;  /* n LUT ids (NULL: LUT 0) */
+  uint32_t n;
+  uint32_t log_lut_nb;
+} luthe_pbs_batch_desc;
+int luthe_pbs_batch_list(luthe_ctx* ctx, const luthe_pbs_batch_desc* d_descs, uint32_t count, size_t total,
+                         void* stream);
+
 /* Key switch + mod switch to 2N only: d_lwe2n[c*(n+1) + j] in [0, 2N)
  * (pep_key_switch: pep_ks_blram_format.sv:159-246, pep_ks_out_process.sv:276-341). */
 int luthe_keyswitch_batch(luthe_ctx* ctx, const uint64_t* d_blwe, uint16_t* d_lwe2n, size_t n, void* stream);
@@ -248,7 +267,7 @@ int luthe_alu_batch(luthe_ctx* ctx, uint32_t opcode, const uint64_t* d_a, const 
 
 /* luthe_pbs_batch without any allocation: no hipMalloc, hipFree or allocating library call,
  * at any batch size it accepts.  Same kernels and results as luthe_pbs_batch; its scratch
- * (job descriptors and mod-switched LWEs) is a fixed device array of 16.125 MiB that each GPU
+ * (job descriptors and mod-switched LWEs) is a fixed device array of 16.1875 MiB that each GPU
  * holds for the life of the process, so batches are limited to n <= LUTHE_NOALLOC_MAX_BATCH and
  * n * lwe_stride <= LUTHE_NOALLOC_MAX_LWE_WORDS, with lwe_stride = lwe_dim + 1 rounded up to a
  * multiple of 8 (n <= 8192 at lwe_dim <= 1016); larger batches return LUTHE_ERR_ARG before any
@@ -364,7 +383,7 @@ int luthe_lut_encode_many(const luthe_params* p, const uint8_t* tables, uint32_t
  *    s * slot_bytes (page aligned, pem_common_param_pkg.sv:24-36); register-file word w
  *    sends coefficients [w*RC + p*RC/PC, w*RC + (p+1)*RC/PC) to channel p; the body is
  *    the first u64 of one extra AXI word of channel 0 (pem_load.sv:90-156).
- * The KSK image (ksk_if block-column format) is not converted. */
+ *  - KSK: see luthe_hpu_ksk_layout below. */
 typedef struct luthe_hpu_layout {
   uint32_t axi_bytes;    /* AXI4 data width of the ciphertext channels in bytes (8..64) */
   uint32_t pem_pc;       /* PEM_PC: ciphertext pseudo-channels, 1 or 2 */
@@ -394,6 +413,34 @@ int luthe_hpu_ct_from_image(const luthe_params* p, const luthe_hpu_layout* lay, 
 /* LUT bodies (count x N) -> full GLWEs (count x (k+1)N, zero mask). */
 int luthe_hpu_lut_from_image(const luthe_params* p, const uint64_t* bodies, uint32_t count, uint64_t* glwe);
 int luthe_write_lut_hpu_image(luthe_ctx* ctx, uint32_t gid, const uint64_t* bodies, uint32_t count);
+
+/* KSK image: one byte stream per KSK pseudo-channel, in the order ksk_if reads the key
+ * (hw/module/ksk/module/ksk_if/simu/rtl/tb_ksk_if.sv:470-525): for block column c < ceil((n+1)/
+ * LBX), column x < LBX, block line l < ceil(k*N/LBY), level group lg < ceil(KS_L/LBZ), line y of
+ * the channel's cuts (channel pc owns y in [pc*LBY/KSK_PC, (pc+1)*LBY/KSK_PC)), level z < LBZ:
+ * one ks_mod_log-bit coefficient; each (c, x) run is zero-padded to whole AXI words; an AXI word
+ * of axi_bytes (little-endian) holds axi_bytes*8 / KSK_ACS_W slots of LBZ coefficients, KSK_ACS_W
+ * = 64 if LBZ * ks_mod_log > 32 else 32 (ksk_if_common_param_pkg.sv:20-21).  Output column
+ * j = c*LBX + x (j = n: the body), level lg*LBZ + z (this engine's level order, least
+ * significant digit first), and line l*LBY + y = g*N + (N-1-m) holds mask coefficient m of
+ * GLWE polynomial g ("reverse order in the BLWE_K dimension, on N basis", ksk_manager.sv:11).
+ * Requires KSK_CUT_NB % KSK_PC == 0 and LBY % KSK_CUT_NB == 0 (the shipped configurations). */
+typedef struct luthe_hpu_ksk_layout {
+  uint32_t axi_bytes;  /* AXI4 data width of the KSK channels in bytes (8..128) */
+  uint32_t ksk_pc;     /* KSK_PC: pseudo-channels */
+  uint32_t ksk_cut_nb; /* KSK_CUT_NB: cuts of the LBY lines over the channels */
+  uint32_t lbx, lby, lbz; /* key-switch block: columns, lines, levels (pep_ks_common_param_pkg.sv:25-35) */
+} luthe_hpu_ksk_layout;
+/* Bytes of one channel's image (0 if the layout is invalid for these parameters). */
+size_t luthe_hpu_ksk_image_bytes(const luthe_params* p, const luthe_hpu_ksk_layout* lay);
+/* ksk: this engine's [k*N][ks_level][n+1] layout; images[pc]: ksk_pc buffers of
+ * luthe_hpu_ksk_image_bytes each. */
+int luthe_hpu_ksk_to_image(const luthe_params* p, const luthe_hpu_ksk_layout* lay, const uint32_t* ksk,
+                           uint8_t* const* images);
+int luthe_hpu_ksk_from_image(const luthe_params* p, const luthe_hpu_ksk_layout* lay, const uint8_t* const* images,
+                             uint32_t* ksk);
+/* Upload a KSK given as an HPU image (converted on the host, then luthe_upload_ksk). */
+int luthe_upload_ksk_hpu_image(luthe_ctx* ctx, const luthe_hpu_ksk_layout* lay, const uint8_t* const* images);
 
 /* ---------------------------------------------------------------- trace
  * Instruction-scheduler trace records (isc_trace_t, isc_common_param_pkg.sv:35-48):

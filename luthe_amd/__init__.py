@@ -78,6 +78,15 @@ class HpuLayout(C.Structure):
                 ("bsk_chunk", C.c_uint32), ("ntt_ncut", C.c_uint32), ("ntt_cut", C.c_uint32 * 8)]
 
 
+class HpuKskLayout(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("axi_bytes", "ksk_pc", "ksk_cut_nb", "lbx", "lby", "lbz")]
+
+
+def hpu_ksk_layout(axi_bytes=64, ksk_pc=1, ksk_cut_nb=1, lbx=3, lby=64, lbz=3):
+    """KSK image geometry: AXI width, KSK_PC, KSK_CUT_NB and the key-switch block LBX x LBY x LBZ."""
+    return HpuKskLayout(axi_bytes, ksk_pc, ksk_cut_nb, lbx, lby, lbz)
+
+
 def hpu_layout(axi_bytes=64, pem_pc=2, regf_coef_nb=32, bsk_chunk=64, ntt_cut=(5, 6)):
     lay = HpuLayout()
     lay.axi_bytes, lay.pem_pc, lay.regf_coef_nb, lay.bsk_chunk = axi_bytes, pem_pc, regf_coef_nb, bsk_chunk
@@ -168,6 +177,7 @@ def lib():
         "luthe_submit_iop": (i32, [vp, vp, sz, C.POINTER(u64)]),
         "luthe_pbs_batch": (i32, [vp, vp, vp, vp, u32, sz, vp]),
         "luthe_pbs_batch_noalloc": (i32, [vp, vp, vp, vp, u32, sz, vp]),
+        "luthe_pbs_batch_list": (i32, [vp, vp, u32, sz, vp]),
         "luthe_flush": (i32, [vp]),
         "luthe_set_dop_batching": (i32, [vp, C.POINTER(DopBatching)]),
         "luthe_get_dop_batching": (i32, [vp, C.POINTER(DopBatching)]),
@@ -215,6 +225,10 @@ def lib():
         "luthe_hpu_ct_from_image": (i32, [P_, C.POINTER(HpuLayout), vp, u32, vp]),
         "luthe_hpu_lut_from_image": (i32, [P_, vp, u32, vp]),
         "luthe_write_lut_hpu_image": (i32, [vp, u32, vp, u32]),
+        "luthe_hpu_ksk_image_bytes": (sz, [P_, C.POINTER(HpuKskLayout)]),
+        "luthe_hpu_ksk_to_image": (i32, [P_, C.POINTER(HpuKskLayout), vp, C.POINTER(vp)]),
+        "luthe_hpu_ksk_from_image": (i32, [P_, C.POINTER(HpuKskLayout), C.POINTER(vp), vp]),
+        "luthe_upload_ksk_hpu_image": (i32, [vp, C.POINTER(HpuKskLayout), C.POINTER(vp)]),
         "luthe_trace_record_bits": (i32, [u32]),
         "luthe_trace_pack": (i32, [u32, vp, sz, vp]),
         "luthe_trace_unpack": (i32, [u32, vp, sz, vp]),
@@ -340,6 +354,18 @@ def _vp(a):
     if isinstance(a, int):
         return C.c_void_p(a)
     return a.ctypes.data_as(C.c_void_p)
+
+
+# luthe_pbs_batch_desc: d_in, d_out, d_gid (device addresses), n, log_lut_nb
+PBS_BATCH_DESC = np.dtype([("d_in", "<u8"), ("d_out", "<u8"), ("d_gid", "<u8"), ("n", "<u4"), ("log_lut_nb", "<u4")])
+
+
+def batch_descs(batches):
+    """Descriptor array for luthe_pbs_batch_list from (in_addr, out_addr, gid_addr, n, log_lut_nb) tuples."""
+    d = np.zeros(len(batches), PBS_BATCH_DESC)
+    for i, (a, o, g, n, lg) in enumerate(batches):
+        d[i] = (a, o, g, n, lg)
+    return d
 
 
 def default_params(**over):
@@ -520,6 +546,11 @@ class Context:
         img = np.ascontiguousarray(image, np.uint64)
         self._chk(lib().luthe_upload_bsk_hpu_image(self._h, C.byref(layout), _vp(img)), "upload_bsk_hpu_image")
 
+    def upload_ksk_hpu_image(self, layout, images):
+        imgs = [np.ascontiguousarray(i, np.uint8) for i in images]
+        ptrs = (C.c_void_p * len(imgs))(*[i.ctypes.data for i in imgs])
+        self._chk(lib().luthe_upload_ksk_hpu_image(self._h, C.byref(layout), ptrs), "upload_ksk_hpu_image")
+
     def write_lut_hpu_image(self, gid, bodies):
         b = np.ascontiguousarray(bodies, np.uint64).reshape(-1, self.params.poly_size)
         self._chk(lib().luthe_write_lut_hpu_image(self._h, gid, _vp(b), b.shape[0]), "write_lut_hpu_image")
@@ -542,6 +573,11 @@ class Context:
     def pbs_batch_dev(self, d_in, d_out, d_gid, n, log_lut_nb=0, stream=None):
         self._chk(lib().luthe_pbs_batch(self._h, _vp(d_in), _vp(d_out), _vp(d_gid), log_lut_nb, n, stream),
                   "pbs_batch")
+
+    def pbs_batch_list_dev(self, d_descs, count, total, stream=None):
+        """One KS + blind-rotation launch over `count` batches described by PBS_BATCH_DESC
+        records in device memory (see batch_descs)."""
+        self._chk(lib().luthe_pbs_batch_list(self._h, _vp(d_descs), count, total, stream), "pbs_batch_list")
 
     def pbs_batch(self, blwe, gids, log_lut_nb=0):
         """Host convenience: upload, run luthe_pbs_batch, download."""
@@ -878,6 +914,34 @@ def hpu_bsk_from_image(params, layout, image):
     if rc != 0:
         raise LutheError(rc, "hpu_bsk_from_image")
     return out.reshape(params.bsk_shape())
+
+
+def hpu_ksk_image_bytes(params, layout):
+    return lib().luthe_hpu_ksk_image_bytes(C.byref(params), C.byref(layout))
+
+
+def hpu_ksk_to_image(params, layout, ksk):
+    """One uint8 array per KSK pseudo-channel."""
+    k = np.ascontiguousarray(ksk, np.uint32)
+    nb = hpu_ksk_image_bytes(params, layout)
+    if nb == 0:
+        raise LutheError(-1, "invalid KSK layout")
+    imgs = [np.zeros(nb, np.uint8) for _ in range(layout.ksk_pc)]
+    ptrs = (C.c_void_p * len(imgs))(*[i.ctypes.data for i in imgs])
+    rc = lib().luthe_hpu_ksk_to_image(C.byref(params), C.byref(layout), _vp(k), ptrs)
+    if rc != 0:
+        raise LutheError(rc, "hpu_ksk_to_image")
+    return imgs
+
+
+def hpu_ksk_from_image(params, layout, images):
+    imgs = [np.ascontiguousarray(i, np.uint8) for i in images]
+    ptrs = (C.c_void_p * len(imgs))(*[i.ctypes.data for i in imgs])
+    out = np.zeros(params.ksk_shape(), np.uint32)
+    rc = lib().luthe_hpu_ksk_from_image(C.byref(params), C.byref(layout), ptrs, _vp(out))
+    if rc != 0:
+        raise LutheError(rc, "hpu_ksk_from_image")
+    return out
 
 
 def hpu_ct_slot_bytes(params, layout):

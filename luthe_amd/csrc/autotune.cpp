@@ -43,12 +43,12 @@ struct Timer {
 int time_pbs(luthe_ctx* c, const uint64_t* d_in, uint64_t* d_out, int n, int grid, int lat_max, float& ms) {
   c->br_grid_override = grid;
   c->br_latency_max = lat_max;
-  HIPCHK(c, lk::launch_make_jobs(c->d_jobs, nullptr, n, 0, c->prm.lut_slots, c->stream));
-  int rc = lx::run_pbs(c, d_in, c->d_jobs, n, d_out, c->stream);
+  HIPCHK(c, lk::launch_make_jobs(c->d_jobs, d_in, d_out, nullptr, n, 0, c->prm.lut_slots, (int)c->blen, c->stream));
+  int rc = lx::run_pbs(c, c->d_jobs, n, c->stream);
   if (rc) return rc;
   Timer t(c->stream);
   t.start();
-  rc = lx::run_pbs(c, d_in, c->d_jobs, n, d_out, c->stream);
+  rc = lx::run_pbs(c, c->d_jobs, n, c->stream);
   ms = t.stop();
   return rc;
 }

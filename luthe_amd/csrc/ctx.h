@@ -100,12 +100,10 @@ int ensure_lwe2n(luthe_ctx* c, size_t n);
 int ensure_jobs(luthe_ctx* c, size_t n);
 int ensure_rename_rows(luthe_ctx* c, size_t n);
 // Key switch + blind rotation + sample extract of n jobs whose descriptors are on the
-// device; rows are relative to in_base / out_base.
-int run_pbs(luthe_ctx* c, const uint64_t* in_base, const lk::PbsJob* d_jobs, int n, uint64_t* out_base,
-            hipStream_t s);
+// device (their BLWE addresses inside).
+int run_pbs(luthe_ctx* c, const lk::PbsJob* d_jobs, int n, hipStream_t s);
 // The same with the mod-switched LWEs in caller-provided device memory; allocates nothing.
-int run_pbs_on(luthe_ctx* c, const uint64_t* in_base, const lk::PbsJob* d_jobs, int n, uint64_t* out_base,
-               hipStream_t s, uint16_t* lwe2n);
+int run_pbs_on(luthe_ctx* c, const lk::PbsJob* d_jobs, int n, hipStream_t s, uint16_t* lwe2n);
 // Looks up the no-allocation entry point's device scratch on `device` (current).
 int noalloc_init(luthe_ctx* c, int device);
 // DOp groups: seal (launch) the open group now; check the sealing rules (force: seal).

@@ -11,8 +11,19 @@
 //   pem_common_param_pkg.sv:24-36); register-file word w of REGF_COEF_NB coefficients
 //   sends coefficients [w*RC + p*RC/PC, w*RC + (p+1)*RC/PC) to channel p; channel 0 holds
 //   one extra AXI word with the body (pem_load.sv:90-156).
+// * KSK image: per pseudo-channel pc, the order in which ksk_if reads the key from HBM, as the
+//   ksk_if testbench writes it (hw/module/ksk/module/ksk_if/simu/rtl/tb_ksk_if.sv:470-525):
+//   for block column c, column x < LBX, block line l, level group lg, line y of this channel's
+//   cuts, level z < LBZ, one MOD_KSK_W-bit coefficient; each (c, x) run is padded to whole AXI
+//   words; an AXI word holds AXI4_DATA_W / KSK_ACS_W slots of LBZ coefficients
+//   (ksk_if_common_param_pkg.sv:20-21).  Output column j = c*LBX + x (j = n is the body),
+//   level = lg*LBZ + z (the decomposition's digit index, least significant first), and line
+//   l*LBY + y = g*N + (N - 1 - m) holds mask coefficient m of polynomial g: "reverse order in
+//   the BLWE_K dimension, on N basis" (ksk_manager.sv:11).  Columns, lines and levels beyond
+//   n + 1, k*N and KS_L, and the padding, are zero.
 // * Trace record: isc_trace_t (isc_common_param_pkg.sv:35-48), packed MSB-first,
 //   written as 32-bit AXI words, least significant word first (trace_manager.sv:245-264).
+#include <stdint.h>
 #include <string.h>
 
 #include <vector>
@@ -137,9 +148,130 @@ struct Bits {  // little-endian bit vector of 32-bit words
   }
 };
 
+// KSK image geometry (tb_ksk_if.sv:40-43,63-98 with the cut counts dividing evenly).
+struct KskGeom {
+  uint32_t W = 0, acs_w = 0, cpw = 0;         // MOD_KSK_W, KSK_ACS_W, slots per AXI word
+  uint32_t cols = 0, lines = 0, lgs = 0;      // KS_BLOCK_COL_NB, KS_BLOCK_LINE_NB, KS_LG_NB
+  uint32_t y_per_pc = 0;                      // lines y of one channel (LBY / KSK_PC)
+  size_t run = 0, run_padded = 0;             // coefficients per (c, x) run of one channel
+  size_t pc_bytes = 0;
+};
+
+int ksk_geom(const luthe_params* p, const luthe_hpu_ksk_layout* lay, KskGeom& g) {
+  if (!p || !lay) return LUTHE_ERR_ARG;
+  if (lay->lbx < 1 || lay->lby < 1 || lay->lbz < 1 || lay->ksk_pc < 1 || lay->ksk_cut_nb < 1) return LUTHE_ERR_ARG;
+  if (lay->ksk_cut_nb % lay->ksk_pc || lay->lby % lay->ksk_cut_nb) return LUTHE_ERR_ARG;  // even cuts
+  if (log2_exact(lay->axi_bytes) < 3 || lay->axi_bytes > 128) return LUTHE_ERR_ARG;
+  g.W = p->ks_mod_log;
+  if ((uint64_t)lay->lbz * g.W > 64) return LUTHE_ERR_ARG;
+  g.acs_w = lay->lbz * g.W > 32 ? 64 : 32;
+  if (lay->axi_bytes * 8 < g.acs_w) return LUTHE_ERR_ARG;
+  g.cpw = lay->axi_bytes * 8 / g.acs_w;
+  g.cols = (p->lwe_dim + 1 + lay->lbx - 1) / lay->lbx;
+  g.lines = (p->glwe_k * p->poly_size + lay->lby - 1) / lay->lby;
+  g.lgs = (p->ks_level + lay->lbz - 1) / lay->lbz;
+  g.y_per_pc = lay->lby / lay->ksk_pc;
+  g.run = (size_t)g.y_per_pc * g.lines * g.lgs * lay->lbz;
+  const size_t per_word = (size_t)g.cpw * lay->lbz;
+  g.run_padded = (g.run + per_word - 1) / per_word * per_word;
+  g.pc_bytes = (size_t)g.cols * lay->lbx * (g.run_padded / per_word) * lay->axi_bytes;
+  return LUTHE_OK;
+}
+
+// Visits every coefficient position of channel pc's image in stream order: f(byte offset of
+// the coefficient's slot, bit offset inside the slot, KSK index or SIZE_MAX for zero fill).
+template <class F>
+void ksk_walk(const luthe_params* p, const luthe_hpu_ksk_layout* lay, const KskGeom& g, uint32_t pc, F f) {
+  const uint32_t N = p->poly_size, kN = p->glwe_k * N, n1 = p->lwe_dim + 1, L = p->ks_level;
+  const size_t per_word = (size_t)g.cpw * lay->lbz;
+  size_t word = 0;
+  for (uint32_t c = 0; c < g.cols; c++)
+    for (uint32_t x = 0; x < lay->lbx; x++) {
+      size_t e = 0;  // coefficient index inside this run
+      auto put = [&](size_t idx) {
+        const size_t w = word + e / per_word, in = e % per_word;
+        f(w * lay->axi_bytes + (in / lay->lbz) * (g.acs_w / 8), (uint32_t)(in % lay->lbz) * g.W, idx);
+        e++;
+      };
+      const uint32_t j = c * lay->lbx + x;
+      for (uint32_t l = 0; l < g.lines; l++)
+        for (uint32_t lg = 0; lg < g.lgs; lg++)
+          for (uint32_t yy = 0; yy < g.y_per_pc; yy++)
+            for (uint32_t z = 0; z < lay->lbz; z++) {
+              const uint32_t line = l * lay->lby + pc * g.y_per_pc + yy, lev = lg * lay->lbz + z;
+              size_t idx = SIZE_MAX;
+              if (j < n1 && line < kN && lev < L) {
+                const uint32_t gpoly = line / N, m = N - 1 - line % N;  // reversed on N basis
+                idx = ((size_t)(gpoly * N + m) * L + lev) * n1 + j;
+              }
+              put(idx);
+            }
+      while (e < g.run_padded) put(SIZE_MAX);
+      word += g.run_padded / per_word;
+    }
+}
+
 }  // namespace
 
 extern "C" {
+
+size_t luthe_hpu_ksk_image_bytes(const luthe_params* p, const luthe_hpu_ksk_layout* lay) {
+  KskGeom g;
+  return ksk_geom(p, lay, g) ? 0 : g.pc_bytes;
+}
+
+int luthe_hpu_ksk_to_image(const luthe_params* p, const luthe_hpu_ksk_layout* lay, const uint32_t* ksk,
+                           uint8_t* const* images) {
+  KskGeom g;
+  int rc = ksk_geom(p, lay, g);
+  if (rc) return rc;
+  if (!ksk || !images) return LUTHE_ERR_ARG;
+  const uint64_t wmask = (1ull << g.W) - 1;
+  for (uint32_t pc = 0; pc < lay->ksk_pc; pc++) {
+    if (!images[pc]) return LUTHE_ERR_ARG;
+    memset(images[pc], 0, g.pc_bytes);
+    uint8_t* img = images[pc];
+    ksk_walk(p, lay, g, pc, [&](size_t byte, uint32_t bit, size_t idx) {
+      if (idx == SIZE_MAX) return;
+      uint64_t slot = 0;
+      memcpy(&slot, img + byte, g.acs_w / 8);
+      slot |= ((uint64_t)ksk[idx] & wmask) << bit;
+      memcpy(img + byte, &slot, g.acs_w / 8);  // little-endian AXI words
+    });
+  }
+  return LUTHE_OK;
+}
+
+int luthe_hpu_ksk_from_image(const luthe_params* p, const luthe_hpu_ksk_layout* lay, const uint8_t* const* images,
+                             uint32_t* ksk) {
+  KskGeom g;
+  int rc = ksk_geom(p, lay, g);
+  if (rc) return rc;
+  if (!ksk || !images) return LUTHE_ERR_ARG;
+  const uint64_t wmask = (1ull << g.W) - 1;
+  for (uint32_t pc = 0; pc < lay->ksk_pc; pc++) {
+    if (!images[pc]) return LUTHE_ERR_ARG;
+    const uint8_t* img = images[pc];
+    ksk_walk(p, lay, g, pc, [&](size_t byte, uint32_t bit, size_t idx) {
+      if (idx == SIZE_MAX) return;
+      uint64_t slot = 0;
+      memcpy(&slot, img + byte, g.acs_w / 8);
+      ksk[idx] = (uint32_t)((slot >> bit) & wmask);
+    });
+  }
+  return LUTHE_OK;
+}
+
+int luthe_upload_ksk_hpu_image(luthe_ctx* ctx, const luthe_hpu_ksk_layout* lay, const uint8_t* const* images) {
+  if (!ctx) return LUTHE_ERR_ARG;
+  luthe_params p;
+  luthe_get_params(ctx, &p);
+  std::vector<uint32_t> ksk((size_t)p.glwe_k * p.poly_size * p.ks_level * (p.lwe_dim + 1));
+  int rc = luthe_hpu_ksk_from_image(&p, lay, images, ksk.data());
+  if (rc) return rc;
+  return luthe_upload_ksk(ctx, ksk.data());
+}
+
 
 int luthe_hpu_bsk_order(const uint32_t* cut, uint32_t ncut, uint32_t* order) {
   if (!cut || !order || ncut < 1 || ncut > 16) return LUTHE_ERR_ARG;

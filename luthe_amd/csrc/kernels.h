@@ -6,12 +6,22 @@
 
 namespace lk {
 
-// One PBS of a batch: BLWE row to read (key switch input), first BLWE row to write
-// (lut_nb consecutive rows for Many-LUT), LUT id, log2 of the number of LUTs.
+// One PBS of a batch: device address of its input BLWE (key switch input) and of its first
+// output BLWE (lut_nb consecutive BLWEs for Many-LUT), LUT id, log2 of the number of LUTs.
+// Absolute addresses let one launch span batches in different allocations.
 struct PbsJob {
-  uint32_t in_row;
-  uint32_t out_row;
+  uint64_t in_addr;
+  uint64_t out_addr;
   uint32_t gid;
+  uint32_t log_lut_nb;
+};
+
+// One batch of a batch list, in device memory (the layout of luthe_pbs_batch_desc).
+struct PbsBatchDesc {
+  const uint64_t* in;
+  uint64_t* out;
+  const uint32_t* gid;
+  uint32_t n;
   uint32_t log_lut_nb;
 };
 
@@ -47,11 +57,11 @@ hipError_t launch_ntt(int mode, const uint64_t* a, const uint64_t* bn, uint64_t*
                       const uint64_t* tab, hipStream_t s, int polys_per_wg = 2);
 hipError_t launch_bsk_prepare(const uint64_t* src, uint64_t* dst, int n_lwe, int rows, int K1, int is_ntt,
                               const uint64_t* tab, hipStream_t s);
-hipError_t launch_keyswitch(const uint64_t* base, const PbsJob* jobs, int nct, const uint32_t* ksk, uint16_t* lwe2n,
+hipError_t launch_keyswitch(const PbsJob* jobs, int nct, const uint32_t* ksk, uint16_t* lwe2n,
                             const KsArgs& a, int ks_level, hipStream_t s);
 // Cohorts of at most latency_max ciphertexts run the 512-thread latency variant.
 hipError_t launch_blind_rotate(const uint64_t* bsk, const uint64_t* luts, const uint16_t* lwe2n, const PbsJob* jobs,
-                               int nct, uint64_t* out_base, const uint64_t* tab, const BrArgs& a,
+                               int nct, const uint64_t* tab, const BrArgs& a,
                                int pbs_level, int grid_override, hipStream_t s, int latency_max = 0);
 int blind_rotate_occupancy(int pbs_level);
 // Ciphertexts the latency variant holds resident at once (its workgroups per CU x CUs).
@@ -62,7 +72,12 @@ hipError_t launch_alu(uint64_t* base, const AluOp* ops, int nops, int blen, uint
 hipError_t launch_copy(uint64_t* base, const CopyOp* ops, int nops, int blen, hipStream_t s);
 hipError_t launch_alu_batch(uint32_t opcode, const uint64_t* a, const uint64_t* b, uint64_t* out, uint32_t mul_factor,
                             uint32_t msg_cst, int n, int blen, uint32_t useful_bits, hipStream_t s);
-hipError_t launch_make_jobs(PbsJob* jobs, const uint32_t* gid, int n, uint32_t log_lut_nb, uint32_t lut_slots,
+// Jobs of `count` batches described in device memory, `total` job slots.
+hipError_t launch_make_jobs_list(PbsJob* jobs, const PbsBatchDesc* descs, int count, int total, uint32_t lut_slots,
+                                 int blen, hipStream_t s);
+// Jobs of one contiguous batch: input c at in + c * blen, outputs at out + (c << log_lut_nb) * blen.
+hipError_t launch_make_jobs(PbsJob* jobs, const uint64_t* in, uint64_t* out, const uint32_t* gid, int n,
+                            uint32_t log_lut_nb, uint32_t lut_slots, int blen,
                             hipStream_t s);
 
 }  // namespace lk

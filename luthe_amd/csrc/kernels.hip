@@ -186,7 +186,6 @@ __global__ __launch_bounds__(256, (L == 1 ? 4 : 2)) void k_blind_rotate(const ui
                                                       const uint64_t* __restrict__ luts,
                                                       const uint16_t* __restrict__ lwe2n,
                                                       const PbsJob* __restrict__ jobs, int nct,
-                                                      uint64_t* __restrict__ out_base,
                                                       const uint64_t* __restrict__ tab, BrArgs a) {
   constexpr int NPF = K1 * L;
   __shared__ uint64_t lds[NPF * N];
@@ -201,6 +200,7 @@ __global__ __launch_bounds__(256, (L == 1 ? 4 : 2)) void k_blind_rotate(const ui
       (void*)bsk, (short)0, (int)((size_t)a.n_lwe * bsk_stride * 8), 0x00020000);
   for (int c = blockIdx.x; c < nct; c += gridDim.x) {
     const PbsJob job = jobs[c];
+    if (job.out_addr == 0) continue;  // an empty slot of a batch list (uniform per workgroup)
     const uint16_t* a2n = lwe2n + (size_t)c * a.lwe_stride;
     uint64_t acc[K1][8];
     const uint64_t* lut = luts + (size_t)job.gid * K1 * N;
@@ -250,7 +250,7 @@ __global__ __launch_bounds__(256, (L == 1 ? 4 : 2)) void k_blind_rotate(const ui
     const int lut_nb = 1 << job.log_lut_nb;
     for (int lu = 0; lu < lut_nb; lu++) {
       const uint32_t offs = (uint32_t)(lu * (N >> job.log_lut_nb));
-      uint64_t* o = out_base + (size_t)(job.out_row + lu) * blen;
+      uint64_t* o = reinterpret_cast<uint64_t*>(job.out_addr) + (size_t)lu * blen;
 #pragma unroll
       for (int g = 0; g < K1 - 1; g++)
 #pragma unroll
@@ -276,7 +276,6 @@ __global__ __launch_bounds__(512, 1) void k_blind_rotate_lat(const uint64_t* __r
                                                              const uint64_t* __restrict__ luts,
                                                              const uint16_t* __restrict__ lwe2n,
                                                              const PbsJob* __restrict__ jobs, int nct,
-                                                             uint64_t* __restrict__ out_base,
                                                              const uint64_t* __restrict__ tab, BrArgs a) {
   static_assert(K1 == 2, "one half per GLWE polynomial");
   constexpr int NPF = K1 * L;
@@ -288,6 +287,7 @@ __global__ __launch_bounds__(512, 1) void k_blind_rotate_lat(const uint64_t* __r
   const size_t bsk_stride = (size_t)8 * NPF * K1 * T;
   for (int c = blockIdx.x; c < nct; c += gridDim.x) {
     const PbsJob job = jobs[c];
+    if (job.out_addr == 0) continue;  // an empty slot of a batch list (uniform per workgroup)
     const uint16_t* a2n = lwe2n + (size_t)c * a.lwe_stride;
     const uint64_t* lut = luts + (size_t)job.gid * K1 * N;
     uint64_t acc[8];
@@ -346,7 +346,7 @@ __global__ __launch_bounds__(512, 1) void k_blind_rotate_lat(const uint64_t* __r
     const int lut_nb = 1 << job.log_lut_nb;
     for (int lu = 0; lu < lut_nb; lu++) {
       const uint32_t offs = (uint32_t)(lu * (N >> job.log_lut_nb));
-      uint64_t* o = out_base + (size_t)(job.out_row + lu) * blen;
+      uint64_t* o = reinterpret_cast<uint64_t*>(job.out_addr) + (size_t)lu * blen;
 #pragma unroll
       for (int g = 0; g < K1 - 1; g++)
 #pragma unroll
@@ -421,14 +421,58 @@ __global__ __launch_bounds__(256) void k_copy(uint64_t* __restrict__ base, const
   if (e < blen) base[(size_t)op.dst_row * blen + e] = base[(size_t)op.src_row * blen + e];
 }
 
+// Jobs of a list of batches (luthe_pbs_batch_list): block b writes batch b's jobs at the sum
+// of the earlier batches' sizes; jobs at or beyond `total` are dropped, and slots of [0, total)
+// that no batch fills are marked empty (address 0), which the key switch and the blind
+// rotation skip, so a total that disagrees with the descriptors cannot send them to unset
+// addresses.  Many-LUT counts are clamped to 8 outputs, LUT ids to the LUT memory.
+__global__ __launch_bounds__(256) void k_make_jobs_list(PbsJob* __restrict__ jobs,
+                                                        const PbsBatchDesc* __restrict__ descs, int count, int total,
+                                                        uint32_t lut_slots, int blen) {
+  __shared__ uint64_t part[256];
+  const int b = blockIdx.x, t = threadIdx.x;
+  if (b == count) {  // the extra block: mark [sum of all sizes, total) empty
+    uint64_t s = 0;
+    for (int i = t; i < count; i += 256) s += descs[i].n;
+    part[t] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (t < w) part[t] += part[t + w];
+      __syncthreads();
+    }
+    for (uint64_t c = part[0] + t; c < (uint64_t)total; c += 256) jobs[c] = PbsJob{0, 0, 0, 0};
+    return;
+  }
+  uint64_t s = 0;
+  for (int i = t; i < b; i += 256) s += descs[i].n;
+  part[t] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) part[t] += part[t + w];
+    __syncthreads();
+  }
+  const uint64_t off = part[0];
+  const PbsBatchDesc d = descs[b];
+  const uint32_t lg = d.log_lut_nb < 3 ? d.log_lut_nb : 3;
+  for (uint32_t c = t; c < d.n; c += 256) {
+    if (off + c >= (uint64_t)total) break;
+    uint32_t g = d.gid ? d.gid[c] : 0u;
+    g = g < lut_slots ? g : lut_slots - 1;
+    jobs[off + c] = PbsJob{(uint64_t)(uintptr_t)(d.in + (size_t)c * blen),
+                           (uint64_t)(uintptr_t)(d.out + ((size_t)c << lg) * blen), g, lg};
+  }
+}
+
 // LUT ids come from caller device memory: ids beyond the LUT memory are clamped to its
 // last slot so that a bad id can never read outside the allocation.
-__global__ void k_make_jobs(PbsJob* jobs, const uint32_t* gid, int n, uint32_t log_lut_nb, uint32_t lut_slots) {
+__global__ void k_make_jobs(PbsJob* jobs, const uint64_t* in, uint64_t* out, const uint32_t* gid, int n,
+                            uint32_t log_lut_nb, uint32_t lut_slots, int blen) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c < n) {
     uint32_t g = gid ? gid[c] : 0u;
     g = g < lut_slots ? g : lut_slots - 1;
-    jobs[c] = PbsJob{(uint32_t)c, (uint32_t)c << log_lut_nb, g, log_lut_nb};
+    jobs[c] = PbsJob{(uint64_t)(uintptr_t)(in + (size_t)c * blen),
+                     (uint64_t)(uintptr_t)(out + ((size_t)c << log_lut_nb) * blen), g, log_lut_nb};
   }
 }
 
@@ -524,21 +568,21 @@ static int br_lat_max_grid() {
 
 template <int L, bool HI>
 static void br_launch(const uint64_t* bsk, const uint64_t* luts, const uint16_t* lwe2n, const PbsJob* jobs, int nct,
-                      uint64_t* out_base, const uint64_t* tab, const BrArgs& a, int grid_override, hipStream_t s,
+                      const uint64_t* tab, const BrArgs& a, int grid_override, hipStream_t s,
                       bool latency) {
   if (latency) {
     int g = br_lat_max_grid<2, L>();
     g = g < nct ? g : nct;
-    k_blind_rotate_lat<2, L, HI><<<dim3(g), 512, 0, s>>>(bsk, luts, lwe2n, jobs, nct, out_base, tab, a);
+    k_blind_rotate_lat<2, L, HI><<<dim3(g), 512, 0, s>>>(bsk, luts, lwe2n, jobs, nct, tab, a);
   } else {
     int g = grid_override > 0 ? grid_override : br_max_grid<2, L>();
     g = g < nct ? g : nct;
-    k_blind_rotate<2, L, HI><<<dim3(g), 256, 0, s>>>(bsk, luts, lwe2n, jobs, nct, out_base, tab, a);
+    k_blind_rotate<2, L, HI><<<dim3(g), 256, 0, s>>>(bsk, luts, lwe2n, jobs, nct, tab, a);
   }
 }
 
 hipError_t launch_blind_rotate(const uint64_t* bsk, const uint64_t* luts, const uint16_t* lwe2n, const PbsJob* jobs,
-                               int nct, uint64_t* out_base, const uint64_t* tab, const BrArgs& a,
+                               int nct, const uint64_t* tab, const BrArgs& a,
                                int pbs_level, int grid_override, hipStream_t s, int latency_max) {
   if (nct == 0) return hipSuccess;
   if (pbs_level != 1 && pbs_level != 2) return hipErrorInvalidValue;
@@ -548,14 +592,14 @@ hipError_t launch_blind_rotate(const uint64_t* bsk, const uint64_t* luts, const 
   const bool hi = (uint32_t)pbs_level * a.base_log <= 31 && a.base_log <= 23;
   if (pbs_level == 1) {
     if (hi)
-      br_launch<1, true>(bsk, luts, lwe2n, jobs, nct, out_base, tab, a, grid_override, s, latency);
+      br_launch<1, true>(bsk, luts, lwe2n, jobs, nct, tab, a, grid_override, s, latency);
     else
-      br_launch<1, false>(bsk, luts, lwe2n, jobs, nct, out_base, tab, a, grid_override, s, latency);
+      br_launch<1, false>(bsk, luts, lwe2n, jobs, nct, tab, a, grid_override, s, latency);
   } else {
     if (hi)
-      br_launch<2, true>(bsk, luts, lwe2n, jobs, nct, out_base, tab, a, grid_override, s, latency);
+      br_launch<2, true>(bsk, luts, lwe2n, jobs, nct, tab, a, grid_override, s, latency);
     else
-      br_launch<2, false>(bsk, luts, lwe2n, jobs, nct, out_base, tab, a, grid_override, s, latency);
+      br_launch<2, false>(bsk, luts, lwe2n, jobs, nct, tab, a, grid_override, s, latency);
   }
   return hipGetLastError();
 }
@@ -608,10 +652,17 @@ hipError_t launch_modmul_peak(uint64_t* out, int blocks, int iters, hipStream_t 
   return hipGetLastError();
 }
 
-hipError_t launch_make_jobs(PbsJob* jobs, const uint32_t* gid, int n, uint32_t log_lut_nb, uint32_t lut_slots,
-                            hipStream_t s) {
+hipError_t launch_make_jobs_list(PbsJob* jobs, const PbsBatchDesc* descs, int count, int total, uint32_t lut_slots,
+                                 int blen, hipStream_t s) {
+  if (count <= 0 || total <= 0) return hipSuccess;
+  k_make_jobs_list<<<dim3((unsigned)count + 1), 256, 0, s>>>(jobs, descs, count, total, lut_slots, blen);
+  return hipGetLastError();
+}
+
+hipError_t launch_make_jobs(PbsJob* jobs, const uint64_t* in, uint64_t* out, const uint32_t* gid, int n,
+                            uint32_t log_lut_nb, uint32_t lut_slots, int blen, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  k_make_jobs<<<dim3((unsigned)((n + 255) / 256)), 256, 0, s>>>(jobs, gid, n, log_lut_nb, lut_slots);
+  k_make_jobs<<<dim3((unsigned)((n + 255) / 256)), 256, 0, s>>>(jobs, in, out, gid, n, log_lut_nb, lut_slots, blen);
   return hipGetLastError();
 }
 

@@ -22,7 +22,7 @@ constexpr int KS_CT = 32;
 constexpr int KS_CHUNK = 64;
 
 template <int KSL>
-__global__ __launch_bounds__(256) void k_keyswitch(const uint64_t* __restrict__ base, const PbsJob* __restrict__ jobs,
+__global__ __launch_bounds__(256) void k_keyswitch(const PbsJob* __restrict__ jobs,
                                                    int nct, const uint32_t* __restrict__ ksk,
                                                    uint16_t* __restrict__ lwe2n, KsArgs a) {
   __shared__ int32_t dig[KS_CHUNK][KSL][KS_CT / 4];  // 4 signed 8-bit digits per word
@@ -30,7 +30,6 @@ __global__ __launch_bounds__(256) void k_keyswitch(const uint64_t* __restrict__ 
   const int c0 = blockIdx.y * KS_CT;
   const int j = blockIdx.x * 256 + t;
   const int n1 = a.n_lwe + 1;
-  const size_t blen = (size_t)a.kN + 1;
   uint32_t acc[KS_CT];  // sums mod 2^32 (the result is taken mod 2^w, w <= 24)
 #pragma unroll
   for (int c = 0; c < KS_CT; c++) acc[c] = 0;
@@ -43,7 +42,10 @@ __global__ __launch_bounds__(256) void k_keyswitch(const uint64_t* __restrict__ 
       int idx = e * 256 + t;
       int c = idx / KS_CHUNK, ii = idx % KS_CHUNK;
       uint64_t x = 0;
-      if (c0 + c < nct && i0 + ii < a.kN) x = base[(size_t)jobs[c0 + c].in_row * blen + i0 + ii];
+      if (c0 + c < nct && i0 + ii < a.kN) {
+        const uint64_t in = jobs[c0 + c].in_addr;  // 0: an empty slot of a batch list
+        if (in) x = reinterpret_cast<const uint64_t*>(in)[i0 + ii];
+      }
       int64_t d[KSL];
       pbs::decompose<KSL>(x, a.base_log, d);
 #pragma unroll
@@ -74,13 +76,16 @@ __global__ __launch_bounds__(256) void k_keyswitch(const uint64_t* __restrict__ 
     for (int c = 0; c < KS_CT; c++) {
       if (c0 + c >= nct) break;
       uint32_t body = 0;
-      if (j == a.n_lwe) body = pbs::ks_body(base[(size_t)jobs[c0 + c].in_row * blen + a.kN], a.mod_log);
+      if (j == a.n_lwe) {
+        const uint64_t in = jobs[c0 + c].in_addr;
+        if (in) body = pbs::ks_body(reinterpret_cast<const uint64_t*>(in)[a.kN], a.mod_log);
+      }
       lwe2n[(size_t)(c0 + c) * a.lwe_stride + j] = (uint16_t)pbs::ks_finish((uint32_t)acc[c], body, a.mod_log, a.lc);
     }
   }
 }
 
-hipError_t launch_keyswitch(const uint64_t* base, const PbsJob* jobs, int nct, const uint32_t* ksk, uint16_t* lwe2n,
+hipError_t launch_keyswitch(const PbsJob* jobs, int nct, const uint32_t* ksk, uint16_t* lwe2n,
                             const KsArgs& a, int ks_level, hipStream_t s) {
   if (ks_level < 1 || ks_level > 8) return hipErrorInvalidValue;
   // grid.y is limited to 65535: chunks of at most 65535 * KS_CT ciphertexts
@@ -91,7 +96,7 @@ hipError_t launch_keyswitch(const uint64_t* base, const PbsJob* jobs, int nct, c
     uint16_t* out = lwe2n + (size_t)c0 * a.lwe_stride;
     switch (ks_level) {
 #define KS_CASE(L) \
-  case L: k_keyswitch<L><<<grid, 256, 0, s>>>(base, jb, nc, ksk, out, a); break;
+  case L: k_keyswitch<L><<<grid, 256, 0, s>>>(jb, nc, ksk, out, a); break;
       KS_CASE(1) KS_CASE(2) KS_CASE(3) KS_CASE(4) KS_CASE(5) KS_CASE(6) KS_CASE(7) KS_CASE(8)
 #undef KS_CASE
     }

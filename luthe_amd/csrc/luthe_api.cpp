@@ -185,15 +185,15 @@ int latency_max(luthe_ctx* c) {
 
 // KS + BR over jobs already on the device, with the mod-switched LWEs in `lwe2n`
 // (n rows of lwe_stride values).  Allocates nothing.
-int run_pbs_on(luthe_ctx* c, const uint64_t* in_base, const PbsJob* d_jobs, int n, uint64_t* out_base, hipStream_t s,
+int run_pbs_on(luthe_ctx* c, const PbsJob* d_jobs, int n, hipStream_t s,
                uint16_t* lwe2n) {
   if (!c->bsk_ok || !c->ksk_ok) return fail(c, LUTHE_ERR_STATE, "keys not uploaded");
   if (n == 0) return LUTHE_OK;
   const luthe_params& p = c->prm;
   lk::KsArgs ka{(int)(p.glwe_k * p.poly_size), (int)p.lwe_dim, c->lwe_stride, p.ks_base_log, p.ks_mod_log, 12};
-  HIPCHK(c, lk::launch_keyswitch(in_base, d_jobs, n, c->d_ksk, lwe2n, ka, (int)p.ks_level, s));
+  HIPCHK(c, lk::launch_keyswitch(d_jobs, n, c->d_ksk, lwe2n, ka, (int)p.ks_level, s));
   lk::BrArgs ba{(int)p.lwe_dim, c->lwe_stride, p.pbs_base_log};
-  HIPCHK(c, lk::launch_blind_rotate(c->d_bsk, c->d_luts, lwe2n, d_jobs, n, out_base, c->d_tab, ba,
+  HIPCHK(c, lk::launch_blind_rotate(c->d_bsk, c->d_luts, lwe2n, d_jobs, n, c->d_tab, ba,
                                     (int)p.pbs_level, c->br_grid_override, s, latency_max(c)));
   c->cnt.pbs += (uint64_t)n;
   c->cnt.pbs_batches += 1;
@@ -202,12 +202,12 @@ int run_pbs_on(luthe_ctx* c, const uint64_t* in_base, const PbsJob* d_jobs, int 
 }
 
 // KS + BR over jobs already on the device, on the context's (grown as needed) scratch.
-int run_pbs(luthe_ctx* c, const uint64_t* in_base, const PbsJob* d_jobs, int n, uint64_t* out_base, hipStream_t s) {
+int run_pbs(luthe_ctx* c, const PbsJob* d_jobs, int n, hipStream_t s) {
   if (!c->bsk_ok || !c->ksk_ok) return fail(c, LUTHE_ERR_STATE, "keys not uploaded");
   if (n == 0) return LUTHE_OK;
   int rc = ensure_lwe2n(c, (size_t)n);
   if (rc) return rc;
-  return run_pbs_on(c, in_base, d_jobs, n, out_base, s, c->d_lwe2n);
+  return run_pbs_on(c, d_jobs, n, s, c->d_lwe2n);
 }
 
 void trace_push(luthe_ctx* c, const luthe_trace_event& e) {
@@ -481,10 +481,10 @@ int luthe_keyswitch_batch(luthe_ctx* c, const uint64_t* d_blwe, uint16_t* d_lwe2
   int rc = ensure_jobs(c, n);
   if (rc) return rc;
   HIPCHK(c, hipSetDevice(c->device));
-  HIPCHK(c, lk::launch_make_jobs(c->d_jobs, nullptr, (int)n, 0, c->prm.lut_slots, s));
+  HIPCHK(c, lk::launch_make_jobs(c->d_jobs, d_blwe, nullptr, nullptr, (int)n, 0, c->prm.lut_slots, (int)c->blen, s));
   const luthe_params& p = c->prm;
   lk::KsArgs ka{(int)(p.glwe_k * p.poly_size), (int)p.lwe_dim, (int)p.lwe_dim + 1, p.ks_base_log, p.ks_mod_log, 12};
-  HIPCHK(c, lk::launch_keyswitch(d_blwe, c->d_jobs, (int)n, c->d_ksk, d_lwe2n, ka, (int)p.ks_level, s));
+  HIPCHK(c, lk::launch_keyswitch(c->d_jobs, (int)n, c->d_ksk, d_lwe2n, ka, (int)p.ks_level, s));
   return LUTHE_OK;
 }
 
@@ -498,10 +498,11 @@ int luthe_blind_rotate_batch(luthe_ctx* c, const uint16_t* d_lwe2n, uint64_t* d_
   int rc = ensure_jobs(c, n);
   if (rc) return rc;
   HIPCHK(c, hipSetDevice(c->device));
-  HIPCHK(c, lk::launch_make_jobs(c->d_jobs, d_gid, (int)n, log_lut_nb, c->prm.lut_slots, s));
+  HIPCHK(c, lk::launch_make_jobs(c->d_jobs, nullptr, d_out, d_gid, (int)n, log_lut_nb, c->prm.lut_slots, (int)c->blen,
+                                 s));
   const luthe_params& p = c->prm;
   lk::BrArgs ba{(int)p.lwe_dim, (int)p.lwe_dim + 1, p.pbs_base_log};
-  HIPCHK(c, lk::launch_blind_rotate(c->d_bsk, c->d_luts, d_lwe2n, c->d_jobs, (int)n, d_out, c->d_tab, ba,
+  HIPCHK(c, lk::launch_blind_rotate(c->d_bsk, c->d_luts, d_lwe2n, c->d_jobs, (int)n, c->d_tab, ba,
                                     (int)p.pbs_level, c->br_grid_override, s, latency_max(c)));
   return LUTHE_OK;
 }
@@ -516,8 +517,25 @@ int luthe_pbs_batch(luthe_ctx* c, const uint64_t* d_in, uint64_t* d_out, const u
   HIPCHK(c, hipSetDevice(c->device));
   int rc = ensure_jobs(c, n);
   if (rc) return rc;
-  HIPCHK(c, lk::launch_make_jobs(c->d_jobs, d_gid, (int)n, log_lut_nb, c->prm.lut_slots, s));
-  return run_pbs(c, d_in, c->d_jobs, (int)n, d_out, s);
+  HIPCHK(c, lk::launch_make_jobs(c->d_jobs, d_in, d_out, d_gid, (int)n, log_lut_nb, c->prm.lut_slots, (int)c->blen, s));
+  return run_pbs(c, c->d_jobs, (int)n, s);
+}
+
+int luthe_pbs_batch_list(luthe_ctx* c, const luthe_pbs_batch_desc* d_descs, uint32_t count, size_t total,
+                         void* stream) {
+  static_assert(sizeof(luthe_pbs_batch_desc) == sizeof(lk::PbsBatchDesc), "descriptor layouts agree");
+  if (!c || (count && !d_descs)) return LUTHE_ERR_ARG;
+  if (count == 0 || total == 0) return LUTHE_OK;
+  if (total > (size_t)1 << 28 || count > (1u << 20)) return fail(c, LUTHE_ERR_ARG, "batch list too large");
+  if (!c->bsk_ok || !c->ksk_ok) return fail(c, LUTHE_ERR_STATE, "keys not uploaded");
+  hipStream_t s = pick_stream(c, stream);
+  HIPCHK(c, hipSetDevice(c->device));
+  int rc = ensure_jobs(c, total);
+  if (!rc) rc = ensure_lwe2n(c, total);
+  if (rc) return rc;
+  HIPCHK(c, lk::launch_make_jobs_list(c->d_jobs, (const lk::PbsBatchDesc*)d_descs, (int)count, (int)total,
+                                      c->prm.lut_slots, (int)c->blen, s));
+  return run_pbs(c, c->d_jobs, (int)total, s);
 }
 
 int luthe_alu_batch(luthe_ctx* c, uint32_t opcode, const uint64_t* d_a, const uint64_t* d_b, uint64_t* d_out,
@@ -671,7 +689,8 @@ static int execute_group(luthe_ctx* c, const std::vector<dop::Dop>& prog, const 
         alus[L].push_back(AluOp{d.opcode, o.dst, o.src0, o.src1, d.mul_factor, d.msg_cst});
         break;
       case dop::KIND_PBS:
-        pbss[L].push_back(PbsJob{o.src0, o.dst, d.gid, d.log_lut_nb});
+        pbss[L].push_back(PbsJob{(uint64_t)(uintptr_t)(c->d_mem + (size_t)o.src0 * c->blen),
+                                 (uint64_t)(uintptr_t)(c->d_mem + (size_t)o.dst * c->blen), d.gid, d.log_lut_nb});
         break;
       default:
         break;
@@ -725,7 +744,7 @@ static int execute_group(luthe_ctx* c, const std::vector<dop::Dop>& prog, const 
       HIPCHK(c, lk::launch_alu(c->d_mem, (const AluOp*)(c->d_stage + lp.alu_off), lp.nalu, blen, useful_bits(c),
                                c->stream));
     if (lp.npbs) {
-      rc = run_pbs(c, c->d_mem, (const PbsJob*)(c->d_stage + lp.pbs_off), lp.npbs, c->d_mem, c->stream);
+      rc = run_pbs(c, (const PbsJob*)(c->d_stage + lp.pbs_off), lp.npbs, c->stream);
       if (rc) return rc;
     }
     c->cnt.alu += (uint64_t)lp.nalu;

@@ -23,7 +23,7 @@
 
 namespace {
 
-constexpr size_t JOBS_BYTES = (size_t)LUTHE_NOALLOC_MAX_BATCH * sizeof(lk::PbsJob);  // 128 KiB
+constexpr size_t JOBS_BYTES = (size_t)LUTHE_NOALLOC_MAX_BATCH * sizeof(lk::PbsJob);  // 192 KiB
 constexpr size_t LWE_BYTES = (size_t)LUTHE_NOALLOC_MAX_LWE_WORDS * sizeof(uint16_t);  // 16 MiB
 constexpr size_t SCRATCH_BYTES = JOBS_BYTES + LWE_BYTES;
 static_assert(JOBS_BYTES % 256 == 0, "the LWE region starts 256-byte aligned");
@@ -75,6 +75,6 @@ extern "C" int luthe_pbs_batch_noalloc(luthe_ctx* c, const uint64_t* d_in, uint6
   uint16_t* lwe2n = (uint16_t*)((uint8_t*)base + JOBS_BYTES);
   hipStream_t s = stream ? (hipStream_t)stream : c->stream;
   HIPCHK(c, hipSetDevice(c->device));
-  HIPCHK(c, lk::launch_make_jobs(jobs, d_gid, (int)n, log_lut_nb, c->prm.lut_slots, s));
-  return lx::run_pbs_on(c, d_in, jobs, (int)n, d_out, s, lwe2n);
+  HIPCHK(c, lk::launch_make_jobs(jobs, d_in, d_out, d_gid, (int)n, log_lut_nb, c->prm.lut_slots, (int)c->blen, s));
+  return lx::run_pbs_on(c, jobs, (int)n, s, lwe2n);
 }

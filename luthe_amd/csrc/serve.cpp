@@ -292,7 +292,9 @@ int issue_step(luthe_server* s, std::vector<Request*>& admitted) {
     for (const ReadyPbs& rp : take) {
       const iop::Node& nd = rp.r->prog->nodes[rp.node];
       const uint32_t b = (uint32_t)rp.r->row0;
-      jobs.push_back(PbsJob{b + nd.src0, b + nd.dst, s->cfg.lut_base + nd.lut, nd.log_lut_nb});
+      jobs.push_back(PbsJob{(uint64_t)(uintptr_t)(s->d_arena + (size_t)(b + nd.src0) * s->blen),
+                            (uint64_t)(uintptr_t)(s->d_arena + (size_t)(b + nd.dst) * s->blen), s->cfg.lut_base + nd.lut,
+                            nd.log_lut_nb});
       if (rp.r->last_cohort != s->ncohorts) {
         rp.r->last_cohort = s->ncohorts;
         rp.r->cohorts++;
@@ -344,7 +346,7 @@ int issue_step(luthe_server* s, std::vector<Request*>& admitted) {
     nalu += pre_off[i].second;
   }
   if (!jobs.empty()) {
-    rc = lx::run_pbs(c, s->d_arena, (const PbsJob*)(sl.dev + jobs_off), (int)jobs.size(), s->d_arena, st);
+    rc = lx::run_pbs(c, (const PbsJob*)(sl.dev + jobs_off), (int)jobs.size(), st);
     if (rc) return sfail(s, rc, "cohort: %s", c->err.c_str());
   }
   for (size_t i = 0; i < post.size(); i++) {

@@ -229,3 +229,59 @@ def test_alu_constant_other_block_widths(oracle, msg, carry):
         assert np.array_equal(got[1], oracle.alu(dop.SSUB, a[1], None, msg_cst=cst, useful_bits=useful))
     finally:
         ctx.close()
+
+
+def test_pbs_batch_list(prm_l2, oracle):
+    """luthe_pbs_batch_list: batches in separate allocations with their own LUT ids and Many-LUT
+    counts in one launch equal luthe_pbs_batch on each batch (and the oracle on sampled rows),
+    also replayed from a graph; a total below the descriptors' sum drops the trailing PBS
+    (their outputs stay untouched) and a larger total runs nothing extra."""
+    print("case: PBS batch list, 5 batches (1, 300, 7, 64, 129; Many-LUT 0/1/2), graph replay", flush=True)
+    prm, ctx, keys, rp, bsk_ntt = prm_l2
+    rng = np.random.default_rng(70)
+    tables = rng.integers(0, 16, (40, 16), dtype=np.uint8)
+    luts = np.stack([la.lut_encode(prm, t) for t in tables])
+    ctx.write_lut(0, luts)
+    spec = [(1, 0), (300, 0), (7, 2), (64, 1), (129, 0)]
+    bufs, exp = [], []
+    for k, (n, lg) in enumerate(spec):
+        cts = keys.encrypt_msg(rng.integers(0, 16 >> lg, n), seed=700 + k)
+        gids = rng.integers(0, 40, n).astype(np.uint32)
+        d_in, d_gid = la.DeviceBuffer.from_numpy(cts), la.DeviceBuffer.from_numpy(gids)
+        d_out = la.DeviceBuffer((n << lg, prm.blen), np.uint64)
+        d_out.upload(np.full((n << lg, prm.blen), 0xA5A5A5A5A5A5A5A5, np.uint64))
+        exp.append(ctx.pbs_batch(cts, gids, log_lut_nb=lg))
+        bufs.append((cts, gids, d_in, d_gid, d_out))
+    total = sum(n for n, _ in spec)
+    descs = la.DeviceBuffer.from_numpy(la.batch_descs(
+        [(b[2].ptr.value, b[4].ptr.value, b[3].ptr.value, n, lg) for b, (n, lg) in zip(bufs, spec)]))
+    ctx.pbs_batch_list_dev(descs, len(spec), total)
+    ctx.sync()
+    for k in range(len(spec)):
+        assert np.array_equal(bufs[k][4].numpy(), exp[k]), k
+    cts, gids = bufs[1][0], bufs[1][1]
+    rows = np.array([0, 151, 299])
+    o = oracle.pbs_batch(rp, bsk_ntt, keys.ksk, luts, gids[rows], cts[rows], threads=THREADS)
+    assert np.array_equal(bufs[1][4].numpy()[rows], o)
+    print("case: batch list captured in a graph, replayed twice", flush=True)
+    for b in bufs:
+        b[4].upload(np.zeros(b[4].shape, np.uint64))
+    ctx.reserve(total + 64)
+    ctx.capture_begin()
+    ctx.pbs_batch_list_dev(descs, len(spec), total + 64)  # 64 slots no descriptor fills
+    g = ctx.capture_end()
+    for _ in range(2):
+        g.launch()
+        ctx.sync()
+        for k in range(len(spec)):
+            assert np.array_equal(bufs[k][4].numpy(), exp[k]), k
+        ctx.pbs_batch(cts[:3], gids[:3])  # eager work between the replays
+    g.destroy()
+    print("case: batch list with a total below the sum", flush=True)
+    last = bufs[-1][4]
+    last.upload(np.zeros(last.shape, np.uint64))
+    ctx.pbs_batch_list_dev(descs, len(spec), total - 29)
+    ctx.sync()
+    got = last.numpy()
+    assert np.array_equal(got[:100], exp[-1][:100])
+    assert not got[100:].any()

@@ -5,7 +5,7 @@
   over all rounds, and four of them recomputed alone;
 * uneven batches: the 64-batch program of the benchmark (sizes uniform in [1, 4096] from its
   seed, ADD and MAC before a PBS with per-ciphertext LUTs, PBS 2^15 x 2) captured as one HIP
-  graph, oracle rows from 8 batches;
+  graph, oracle rows from 8 batches, and the same PBS as one batch list equal to it;
 * serving: 32-block ADD, CMP_GT and MUL through the server at n=724, decrypted and compared
   alone against in a mix, and an 8-block ADD compared node by node with the oracle.
 
@@ -102,6 +102,16 @@ def test_uneven_batches_benchmark_shape(oracle):
     out = dO.numpy()
     g.destroy()
     starts = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    print("case: the same 64 batches' PBS as one batch list", flush=True)
+    dO2 = la.DeviceBuffer((total, prm.blen), np.uint64)
+    descs = la.DeviceBuffer.from_numpy(la.batch_descs(
+        [(dU.ptr.value + int(o) * b8, dO2.ptr.value + int(o) * b8, dG.ptr.value + int(o) * 4, s, 0)
+         for o, s in zip(starts, sizes)]))
+    ctx.reserve(total)
+    ctx.pbs_batch_list_dev(descs, len(sizes), total)
+    ctx.sync()
+    assert np.array_equal(dO2.numpy(), out)
+    del dO2
     pick_b = np.random.default_rng(6).choice(64, 8, replace=False)
     rows = []
     for bi in pick_b:  # first and last ciphertext of 8 batches

@@ -14,7 +14,8 @@ THREADS = min(16, os.cpu_count() or 1)
 
 
 def test_bsk_lut_ct_images_vs_engine_layouts(oracle):
-    print("case: HPU images (BSK cut [3,4,4] chunk 32, LUT bodies, 2-channel ct image), n=64", flush=True)
+    print("case: HPU images (BSK cut [3,4,4] chunk 32, 2-channel KSK image, LUT bodies, 2-channel ct image), "
+          "n=64", flush=True)
     prm = la.default_params(lwe_dim=64)
     keys = la.Keys(prm, seed=91)
     rp = oracle.params(lwe_dim=64)
@@ -27,6 +28,8 @@ def test_bsk_lut_ct_images_vs_engine_layouts(oracle):
     msgs = rng.integers(0, 16, 24)
     cts = keys.encrypt_msg(msgs, seed=93)
     ct_img = la.hpu_ct_to_image(prm, lay, cts)
+    klay = la.hpu_ksk_layout(axi_bytes=64, ksk_pc=2, ksk_cut_nb=4, lbx=3, lby=64, lbz=3)
+    ksk_img = la.hpu_ksk_to_image(prm, klay, keys.ksk)
 
     ref_ctx = la.Context(prm, device=0)
     ref_ctx.upload_bsk(keys.bsk, la.BSK_COEF)
@@ -37,7 +40,7 @@ def test_bsk_lut_ct_images_vs_engine_layouts(oracle):
 
     ctx = la.Context(prm, device=0)
     ctx.upload_bsk_hpu_image(lay, image)
-    ctx.upload_ksk(keys.ksk)
+    ctx.upload_ksk_hpu_image(klay, ksk_img)
     ctx.write_lut_hpu_image(5, lut[2048:])
     got = ctx.pbs_batch(la.hpu_ct_from_image(prm, lay, ct_img, 24), np.full(24, 5, np.uint32))
     ctx.close()

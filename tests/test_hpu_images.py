@@ -139,3 +139,83 @@ def test_trace_record_fields():
                     "pe_reserved": int(rng.integers(0, 1024))})
     back = la.trace_unpack(724, la.trace_pack(724, evs))
     assert back == evs
+
+
+# ---------------------------------------------------------------- KSK image
+def _tb_ksk_if_image(prm, lay, ksk):
+    """The ksk_if testbench's image writer restated (tb_ksk_if.sv:40-43,63-98 for the per-channel
+    cut counts, :470-525 for the loop order, the run padding and the AXI packing), with the
+    coefficient of (line, level, column) read from this engine's [kN][L][n+1] layout through
+    the header's reverse-on-N line order.  The testbench pads runs with all-ones dummies; the
+    engine writes zeros, and so does this restatement."""
+    W, lbx, lby, lbz, npc, cut_nb = prm.ks_mod_log, lay.lbx, lay.lby, lay.lbz, lay.ksk_pc, lay.ksk_cut_nb
+    N, kN, L, n1 = prm.poly_size, prm.glwe_k * prm.poly_size, prm.ks_level, prm.lwe_dim + 1
+    cols, lines, lgs = -(-n1 // lbx), -(-kN // lby), -(-L // lbz)
+    acs = 64 if lbz * W > 32 else 32
+    cpw = lay.axi_bytes * 8 // acs
+    fcoef = -(-lby // cut_nb)                                     # KSK_CUT_FCOEF_NB
+    dist, cnt, per_pc = -(-cut_nb // npc), cut_nb, []             # get_cut_per_pc
+    for _ in range(npc):
+        per_pc.append(min(cnt, dist))
+        cnt -= dist
+    ofs = [sum(per_pc[:i]) for i in range(npc)]                    # get_cut_ofs
+    nb = [-(-(lby * lines * lgs * lbz) // (cut_nb // per_pc[i])) for i in range(npc)]  # get_bcol_coef
+    rnd = [-(-nb[i] // (cpw * lbz)) * cpw * lbz for i in range(npc)]                   # round
+    q = [[] for _ in range(npc)]
+    for c in range(cols):
+        for x in range(lbx):
+            for ln in range(lines):
+                for lg in range(lgs):
+                    for y in range(lby):
+                        for z in range(lbz):
+                            line, lev, j = ln * lby + y, lg * lbz + z, c * lbx + x
+                            v = 0
+                            if line < kN and lev < L and j < n1:
+                                g, m = divmod(line, N)
+                                v = int(ksk[g * N + (N - 1 - m), lev, j])
+                            for pc in range(npc):
+                                if ofs[pc] * fcoef <= y < (ofs[pc] + per_pc[pc]) * fcoef:
+                                    q[pc].append(v)
+                                    if len(q[pc]) % rnd[pc] == nb[pc]:
+                                        q[pc] += [0] * (rnd[pc] - nb[pc])
+    out = []
+    for pc in range(npc):
+        words = []
+        for w0 in range(0, len(q[pc]), cpw * lbz):
+            word = 0
+            for i in range(cpw):
+                for z in range(lbz):
+                    word |= q[pc][w0 + i * lbz + z] << (i * acs + z * W)
+            words.append(word.to_bytes(lay.axi_bytes, "little"))
+        out.append(np.frombuffer(b"".join(words), np.uint8))
+    return out
+
+
+@pytest.mark.parametrize("lay_kw", [dict(axi_bytes=64, ksk_pc=1, ksk_cut_nb=1, lbx=3, lby=64, lbz=3),
+                                    dict(axi_bytes=32, ksk_pc=2, ksk_cut_nb=4, lbx=2, lby=16, lbz=1),
+                                    dict(axi_bytes=16, ksk_pc=4, ksk_cut_nb=8, lbx=5, lby=32, lbz=2)])
+def test_ksk_image_matches_ksk_if_testbench_order(lay_kw):
+    prm = la.default_params(lwe_dim=10)
+    lay = la.hpu_ksk_layout(**lay_kw)
+    ksk = rng.integers(0, 1 << prm.ks_mod_log, prm.ksk_shape(), dtype=np.uint32)
+    got = la.hpu_ksk_to_image(prm, lay, ksk)
+    exp = _tb_ksk_if_image(prm, lay, ksk)
+    assert len(got) == lay.ksk_pc
+    for g, e in zip(got, exp):
+        assert g.size == e.size == la.hpu_ksk_image_bytes(prm, lay)
+        assert np.array_equal(g, e)
+    assert np.array_equal(la.hpu_ksk_from_image(prm, lay, got), ksk)
+
+
+def test_ksk_image_round_trip_and_limits():
+    prm = la.default_params()  # n = 724, KS 2^3 x 5, w = 21
+    lay = la.hpu_ksk_layout(axi_bytes=64, ksk_pc=2, ksk_cut_nb=2, lbx=3, lby=64, lbz=3)
+    ksk = rng.integers(0, 1 << 21, prm.ksk_shape(), dtype=np.uint32)
+    imgs = la.hpu_ksk_to_image(prm, lay, ksk)
+    assert np.array_equal(la.hpu_ksk_from_image(prm, lay, imgs), ksk)
+    # every coefficient appears once: the images hold the key's bits and zero padding only
+    assert sum(int(np.unpackbits(i).sum()) for i in imgs) == sum(bin(int(v)).count("1") for v in ksk.ravel())
+    for bad in (dict(lbz=4), dict(ksk_cut_nb=3), dict(axi_bytes=24), dict(lby=63), dict(lbx=0)):
+        kw = dict(axi_bytes=64, ksk_pc=2, ksk_cut_nb=2, lbx=3, lby=64, lbz=3)
+        kw.update(bad)
+        assert la.hpu_ksk_image_bytes(prm, la.hpu_ksk_layout(**kw)) == 0, bad
