@@ -543,7 +543,14 @@ typedef struct luthe_server_config {
                                  0xFFFFFFFF = the top LUTHE_IOP_LUTS slots */
   uint32_t max_inflight;      /* admitted requests at once; 0 = limited by the arena only */
   uint32_t queue_max;         /* submitted-but-not-admitted requests; beyond: LUTHE_ERR_BUSY */
+  uint32_t policy;            /* which ready PBS fill a cohort: LUTHE_COHORT_* (default SHORTEST) */
+  uint32_t age_limit_us;      /* SHORTEST: PBS ready this long go first whatever their height
+                                 (no starvation of long requests); 0 = never (default 500 ms) */
 } luthe_server_config;
+/* Cohort policies.  OLDEST: ready PBS in arrival order (the HPU's PBS pool order).  SHORTEST:
+ * the PBS with the fewest PBS levels left on their request's critical path first (short
+ * requests finish without waiting behind the fan-out of long ones), subject to age_limit_us. */
+enum { LUTHE_COHORT_OLDEST = 0, LUTHE_COHORT_SHORTEST = 1 };
 
 /* Counters of the mockup's per-IOp report (docs/debug.md:149-170: PBS issued, batches,
  * by_timeout, usage) plus admission and latency figures since creation or the last reset. */

@@ -20,6 +20,7 @@ P = 0xFFFFFFFF00000001
 ERRORS = {0: "OK", -1: "ARG", -2: "PARAMS", -3: "HIP", -4: "NOMEM", -5: "DOP", -6: "RANGE",
           -7: "STATE", -8: "TIMEOUT", -9: "NODEV", -10: "BUSY"}
 IOP_ADD, IOP_CMP_GT, IOP_MUL, IOP_ADD_RIPPLE = 0, 1, 2, 3
+COHORT_OLDEST, COHORT_SHORTEST = 0, 1  # luthe_server_config.policy
 IOP_NAMES = {IOP_ADD: "ADD", IOP_CMP_GT: "CMP_GT", IOP_MUL: "MUL", IOP_ADD_RIPPLE: "ADD_RIPPLE"}
 IOP_LUTS = 11
 BSK_COEF, BSK_NTT = 0, 1
@@ -123,7 +124,8 @@ class IopDopStats(C.Structure):
 class ServerConfig(C.Structure):
     _fields_ = [("arena_bytes", C.c_uint64), ("pinned_bytes", C.c_uint64), ("cohort_max", C.c_uint32),
                 ("cohort_min", C.c_uint32), ("cohort_timeout_us", C.c_uint32), ("pipeline_depth", C.c_uint32),
-                ("lut_base", C.c_uint32), ("max_inflight", C.c_uint32), ("queue_max", C.c_uint32)]
+                ("lut_base", C.c_uint32), ("max_inflight", C.c_uint32), ("queue_max", C.c_uint32),
+                ("policy", C.c_uint32), ("age_limit_us", C.c_uint32)]
 
 
 class ServerStats(C.Structure):

@@ -55,6 +55,7 @@ struct Program {
   std::vector<uint32_t> out_rows;
   std::vector<Node> nodes;
   uint32_t npbs = 0, depth = 0;  // PBS count, PBS levels on the critical path
+  std::vector<uint32_t> height;  // per node: PBS levels from it to the end (filled by the server)
 };
 
 // Builds the DAG of an IOp on `nblocks`-block operands (1..64).  Returns false if the

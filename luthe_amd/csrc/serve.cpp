@@ -13,8 +13,11 @@
 // thread issues "steps" on the context stream.  One step is:
 //   inputs of newly admitted requests (host -> HBM)
 //   ALU nodes that became ready, in dependency rounds (one k_alu launch per round)
-//   one PBS cohort: the ready PBS nodes of all in-flight requests, oldest first, up to
-//     cohort_max (one key-switch + one blind-rotation launch)
+//   one PBS cohort: up to cohort_max ready PBS nodes of all in-flight requests (one
+//     key-switch + one blind-rotation launch), chosen by the cohort policy: oldest first, or
+//     (default) the nodes with the fewest PBS levels left on their request's critical path
+//     first, so that short requests are not held behind the fan-out of long ones, with
+//     nodes that have waited age_limit_us taken first whatever their height
 //   ALU nodes that the cohort made ready, in rounds
 //   outputs of requests whose nodes are all issued (gather + HBM -> host)
 //   an event.
@@ -36,6 +39,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <set>
 #include <thread>
 #include <vector>
 
@@ -130,6 +134,42 @@ struct ReadyPbs {
   Clock::time_point since;
 };
 
+// Ready PBS nodes, indexed by arrival (seq) and by cohort priority (height, seq).
+struct ReadyQueue {
+  std::map<uint64_t, ReadyPbs> by_age;
+  std::set<std::pair<uint32_t, uint64_t>> by_prio;
+  std::map<uint64_t, uint32_t> prio_of;  // seq -> priority key
+  uint64_t next_seq = 0;
+
+  bool empty() const { return by_age.empty(); }
+  size_t size() const { return by_age.size(); }
+  Clock::time_point oldest() const { return by_age.begin()->second.since; }
+  void push(const ReadyPbs& rp, uint32_t prio) {
+    const uint64_t q = next_seq++;
+    by_age.emplace(q, rp);
+    by_prio.emplace(prio, q);
+    prio_of.emplace(q, prio);
+  }
+  ReadyPbs take(uint64_t q) {
+    auto it = by_age.find(q);
+    ReadyPbs rp = it->second;
+    by_age.erase(it);
+    auto p = prio_of.find(q);
+    by_prio.erase(std::make_pair(p->second, q));
+    prio_of.erase(p);
+    return rp;
+  }
+  // Up to n nodes: those waited at least age_us first (oldest first), then by priority.
+  std::vector<ReadyPbs> pop(size_t n, Clock::time_point now, uint32_t age_us) {
+    std::vector<ReadyPbs> out;
+    out.reserve(n);
+    while (age_us && out.size() < n && !by_age.empty() && us_between(oldest(), now) >= (double)age_us)
+      out.push_back(take(by_age.begin()->first));
+    while (out.size() < n && !by_prio.empty()) out.push_back(take(by_prio.begin()->second));
+    return out;
+  }
+};
+
 struct StepSlot {  // descriptor staging of one in-flight step
   uint8_t* host = nullptr;
   uint8_t* dev = nullptr;
@@ -166,7 +206,7 @@ struct luthe_server {
 
   // server thread only
   std::thread th;
-  std::deque<ReadyPbs> ready_pbs;
+  ReadyQueue ready_pbs;
   std::deque<Step> inflight;
   std::vector<StepSlot> slots;
   std::vector<hipEvent_t> free_events;
@@ -199,6 +239,13 @@ const iop::Program* program_for(luthe_server* s, uint32_t op, uint32_t nb) {
   if (it != s->programs.end()) return &it->second;
   iop::Program p;
   if (!iop::build(op, nb, p)) return nullptr;
+  // PBS levels from each node to the end of the program (nodes are in topological order)
+  p.height.assign(p.nodes.size(), 0);
+  for (size_t i = p.nodes.size(); i-- > 0;) {
+    uint32_t h = 0;
+    for (uint32_t succ : p.nodes[i].succ) h = std::max(h, p.height[succ]);
+    p.height[i] = h + p.nodes[i].pbs;
+  }
   return &s->programs.emplace(key, std::move(p)).first->second;
 }
 
@@ -213,7 +260,7 @@ struct Issue {
   void mark_ready(Request* r, uint32_t node) {
     const iop::Node& n = r->prog->nodes[node];
     if (n.pbs)
-      s->ready_pbs.push_back(ReadyPbs{r, node, now});
+      s->ready_pbs.push(ReadyPbs{r, node, now}, s->cfg.policy == LUTHE_COHORT_OLDEST ? 0u : r->prog->height[node]);
     else
       alu.emplace_back(r, node);
   }
@@ -258,7 +305,7 @@ int slot_reserve(luthe_server* s, StepSlot& sl, size_t bytes) {
 bool cohort_due(luthe_server* s, Clock::time_point now, bool force) {
   if (s->ready_pbs.empty()) return false;
   if (force || s->ready_pbs.size() >= s->cfg.cohort_min) return true;
-  return us_between(s->ready_pbs.front().since, now) >= (double)s->cfg.cohort_timeout_us;
+  return us_between(s->ready_pbs.oldest(), now) >= (double)s->cfg.cohort_timeout_us;
 }
 
 // Builds and issues one step.  Returns LUTHE_OK, 1 if there was nothing to issue, or an
@@ -287,8 +334,7 @@ int issue_step(luthe_server* s, std::vector<Request*>& admitted) {
     by_timeout = !force && s->ready_pbs.size() < s->cfg.cohort_min;
     const size_t n = std::min<size_t>(s->ready_pbs.size(), s->cfg.cohort_max);
     jobs.reserve(n);
-    std::vector<ReadyPbs> take(s->ready_pbs.begin(), s->ready_pbs.begin() + (ptrdiff_t)n);
-    s->ready_pbs.erase(s->ready_pbs.begin(), s->ready_pbs.begin() + (ptrdiff_t)n);
+    std::vector<ReadyPbs> take = s->ready_pbs.pop(n, now, s->cfg.policy == LUTHE_COHORT_OLDEST ? 0 : s->cfg.age_limit_us);
     for (const ReadyPbs& rp : take) {
       const iop::Node& nd = rp.r->prog->nodes[rp.node];
       const uint32_t b = (uint32_t)rp.r->row0;
@@ -523,6 +569,8 @@ void luthe_server_config_default(luthe_server_config* cfg) {
   cfg->lut_base = 0xFFFFFFFFu;  // top LUTHE_IOP_LUTS slots
   cfg->max_inflight = 0;
   cfg->queue_max = 65536;
+  cfg->policy = LUTHE_COHORT_SHORTEST;
+  cfg->age_limit_us = 500000;
 }
 
 int luthe_server_create(luthe_ctx* c, const luthe_server_config* cfg_in, luthe_server** out) {
@@ -553,6 +601,7 @@ int luthe_server_create(luthe_ctx* c, const luthe_server_config* cfg_in, luthe_s
   HIPCHK(c, hipSetDevice(c->device));
   if (cfg.cohort_max == 0) cfg.cohort_max = c->cohort_max > 0 ? (uint32_t)c->cohort_max : (uint32_t)luthe_get_br_grid(c);
   if (cfg.cohort_max > (1u << 20)) return lx::fail(c, LUTHE_ERR_ARG, "cohort_max must be <= 2^20");
+  if (cfg.policy > LUTHE_COHORT_SHORTEST) return lx::fail(c, LUTHE_ERR_ARG, "unknown cohort policy %u", cfg.policy);
 
   auto s = std::make_unique<luthe_server>();
   s->c = c;

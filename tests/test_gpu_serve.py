@@ -218,3 +218,43 @@ def test_server_errors(full):
     # the context works normally once the server is gone
     out = ctx.pbs_batch(ca[:1], np.zeros(1, np.uint32))
     assert out.shape == (1, prm.blen)
+
+
+def test_server_short_request_overtakes_long_ones(full):
+    """Cohort policy SHORTEST (default): ready PBS with the fewest PBS levels left on their
+    request's critical path go first, so an 8-block ADD submitted after four 32-block MULs
+    (whose first level alone is ~4,200 PBS) completes before all of them; its bits equal the
+    same request run alone, and the oldest-first policy gives the same bits."""
+    prm, ctx, keys = full
+    rng = np.random.default_rng(13)
+    muls = []
+    for k in range(4):
+        a, b = (int(x) for x in rng.integers(0, 1 << 62, 2))
+        muls.append((a, b, keys.encrypt_msg(la.to_blocks(a, 32), seed=500 + k),
+                     keys.encrypt_msg(la.to_blocks(b, 32), seed=600 + k)))
+    x, y = 12345, 54321
+    cx, cy = keys.encrypt_msg(la.to_blocks(x, 8), seed=700), keys.encrypt_msg(la.to_blocks(y, 8), seed=701)
+    results = {}
+    for policy in (la.COHORT_SHORTEST, la.COHORT_OLDEST):
+        with la.Server(ctx, policy=policy) as srv:
+            print(f"case: policy {policy}: ADD8 alone", flush=True)
+            alone, _ = srv.wait(srv.submit(la.IOP_ADD, cx, cy), timeout_us=60_000_000)
+            print(f"case: policy {policy}: four MUL32 then ADD8", flush=True)
+            ids = [srv.submit(la.IOP_MUL, ca, cb) for _, _, ca, cb in muls]
+            add_id = srv.submit(la.IOP_ADD, cx, cy)
+            add_out, add_info = srv.wait(add_id, timeout_us=120_000_000)
+            mul_res = [srv.wait(i, timeout_us=120_000_000) for i in ids]
+        assert np.array_equal(add_out, alone)
+        assert la.from_blocks(keys.decrypt_msg(add_out)) == (x + y) & 0xFFFF
+        for (a, b, _, _), (out, _) in zip(muls, mul_res):
+            assert la.from_blocks(keys.decrypt_msg(out)) == (a * b) & ((1 << 64) - 1)
+        results[policy] = (add_out, [m[0] for m in mul_res], add_info["latency_us"],
+                           min(m[1]["latency_us"] for m in mul_res))
+        print(f"    ADD8 {add_info['latency_us'] / 1e3:.1f} ms, first MUL32 "
+              f"{results[policy][3] / 1e3:.1f} ms", flush=True)
+    add_s, muls_s, lat_add, lat_mul = results[la.COHORT_SHORTEST]
+    assert lat_add < lat_mul  # the short request finished first
+    add_o, muls_o, _, _ = results[la.COHORT_OLDEST]
+    assert np.array_equal(add_s, add_o)
+    for p, q in zip(muls_s, muls_o):
+        assert np.array_equal(p, q)
