@@ -328,6 +328,9 @@ typedef struct luthe_tuning {
   int32_t br_latency_max;   /* largest cohort for the latency variant, -1 = its capacity */
   int32_t ntt_polys_per_wg; /* 1, 2 or 4 */
   int32_t cohort_max;       /* serving cohort size, 0 = blind-rotation width */
+  int32_t br_pair;          /* throughput kernel: 0 = 256 threads per ciphertext (up to 4 per CU),
+                               1 = 512 threads per ciphertext (2 per CU), -1 = default (1 at PBS
+                               level 2, 0 at level 1) */
 } luthe_tuning;
 int luthe_autotune(luthe_ctx* ctx, uint32_t batch, const char* json_path, luthe_tuning* out);
 int luthe_load_tuning(luthe_ctx* ctx, const char* json_path);

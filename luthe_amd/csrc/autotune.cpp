@@ -65,12 +65,14 @@ bool json_int(const std::string& s, const char* key, long& v) {
 }
 
 int apply(luthe_ctx* c, const luthe_tuning& t) {
-  if (t.br_grid < 0 || t.br_latency_max < -1 || t.cohort_max < 0) return LUTHE_ERR_ARG;
+  if (t.br_grid < 0 || t.br_latency_max < -1 || t.cohort_max < 0 || t.br_pair < -1 || t.br_pair > 1)
+    return LUTHE_ERR_ARG;
   if (t.ntt_polys_per_wg != 1 && t.ntt_polys_per_wg != 2 && t.ntt_polys_per_wg != 4) return LUTHE_ERR_ARG;
   c->br_grid_override = t.br_grid;
   c->br_latency_max = t.br_latency_max;
   c->ntt_polys_per_wg = t.ntt_polys_per_wg;
   c->cohort_max = t.cohort_max;
+  c->br_pair = t.br_pair;
   return LUTHE_OK;
 }
 
@@ -84,6 +86,7 @@ int luthe_get_tuning(luthe_ctx* c, luthe_tuning* out) {
   out->br_latency_max = lx::latency_max(c);
   out->ntt_polys_per_wg = c->ntt_polys_per_wg;
   out->cohort_max = c->cohort_max;
+  out->br_pair = c->br_pair;
   return LUTHE_OK;
 }
 
@@ -113,6 +116,7 @@ int luthe_load_tuning(luthe_ctx* c, const char* path) {
   t.br_latency_max = json_int(s, "br_latency_max", v) ? (int32_t)v : -1;
   t.ntt_polys_per_wg = json_int(s, "ntt_polys_per_wg", v) ? (int32_t)v : 2;
   t.cohort_max = json_int(s, "cohort_max", v) ? (int32_t)v : 0;
+  t.br_pair = json_int(s, "br_pair", v) ? (int32_t)v : -1;
   int rc = apply(c, t);
   if (rc) return lx::fail(c, rc, "%s: invalid tuning", path);
   return LUTHE_OK;
@@ -126,7 +130,8 @@ int luthe_autotune(luthe_ctx* c, uint32_t batch, const char* json_path, luthe_tu
   int cus = 0;
   HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
   const int L = (int)c->prm.pbs_level;
-  const int occ = lk::blind_rotate_occupancy(L), per_cu = occ / (cus > 0 ? cus : 1);
+  const int occ = lk::blind_rotate_occupancy(L, false), per_cu = occ / (cus > 0 ? cus : 1);
+  const int occ_pair = lk::blind_rotate_occupancy(L, true);
   const int lat_slots = lk::blind_rotate_latency_slots(L);
   int rc = lx::ensure_jobs(c, batch);
   if (!rc) rc = lx::ensure_lwe2n(c, batch);
@@ -149,7 +154,8 @@ int luthe_autotune(luthe_ctx* c, uint32_t batch, const char* json_path, luthe_tu
   (void)hipMemsetAsync(d_in, 0x5A, rows * c->blen * 8, c->stream);
   (void)hipMemsetAsync(d_a, 0x31, npoly * 2048 * 8, c->stream);
   (void)hipMemsetAsync(d_b, 0x17, npoly * 2048 * 8, c->stream);
-  const luthe_tuning saved{c->br_grid_override, c->br_latency_max, c->ntt_polys_per_wg, c->cohort_max};
+  const luthe_tuning saved{c->br_grid_override, c->br_latency_max, c->ntt_polys_per_wg, c->cohort_max, c->br_pair};
+  c->br_pair = 0;  // (1) sweeps the 256-thread kernel
   std::string meas;
   char line[256];
   // (1) resident ciphertexts per CU
@@ -165,14 +171,28 @@ int luthe_autotune(luthe_ctx* c, uint32_t batch, const char* json_path, luthe_tu
     meas += line;
     if (rate > best_rate * 1.005f) best_rate = rate, best_grid = k * cus;
   }
+  // (1b) the 512-thread pair kernel at its full width
+  int best_pair = 0;
+  if (!rc) {
+    float ms = 0;
+    c->br_pair = 1;
+    rc = time_pbs(c, d_in, d_out, (int)batch, 0, 0, ms);
+    const float rate = batch / (ms * 1e-3f);
+    snprintf(line, sizeof line, ", {\"pair\": 1, \"grid\": %d, \"ms\": %.3f, \"pbs_per_s\": %.1f}", occ_pair, ms,
+             rate);
+    meas += line;
+    if (!rc && rate > best_rate * 1.005f) best_rate = rate, best_pair = 1;
+    c->br_pair = best_pair;
+  }
+  const int width = best_pair ? occ_pair : best_grid;
   // (2) latency variant: largest cohort where it beats the throughput kernel
   std::string lat;
   int lat_max = 0;
   for (int n : {cus / 4, cus / 2, cus, (cus * 3) / 2, 2 * cus, 3 * cus}) {
     if (rc || n < 1 || n > lat_slots || n > (int)batch) continue;
     float t_lat = 0, t_thr = 0;
-    rc = time_pbs(c, d_in, d_out, n, best_grid, n, t_lat);
-    if (!rc) rc = time_pbs(c, d_in, d_out, n, best_grid, 0, t_thr);
+    rc = time_pbs(c, d_in, d_out, n, best_pair ? 0 : best_grid, n, t_lat);
+    if (!rc) rc = time_pbs(c, d_in, d_out, n, best_pair ? 0 : best_grid, 0, t_thr);
     if (rc) break;
     snprintf(line, sizeof line, "%s{\"n\": %d, \"latency_ms\": %.3f, \"throughput_ms\": %.3f}", lat.empty() ? "" : ", ",
              n, t_lat, t_thr);
@@ -204,7 +224,7 @@ int luthe_autotune(luthe_ctx* c, uint32_t batch, const char* json_path, luthe_tu
     (void)apply(c, saved);
     return rc;
   }
-  luthe_tuning t{best_grid == occ ? 0 : best_grid, lat_max, best_np, best_grid};
+  luthe_tuning t{best_pair || best_grid == occ ? 0 : best_grid, lat_max, best_np, width, best_pair};
   (void)apply(c, t);
   if (out) *out = t;
   if (json_path) {
@@ -214,10 +234,10 @@ int luthe_autotune(luthe_ctx* c, uint32_t batch, const char* json_path, luthe_tu
     (void)hipGetDeviceProperties(&prop, c->device);
     fprintf(f,
             "{\n  \"device\": \"%s\", \"cus\": %d, \"lwe_dim\": %u, \"pbs_level\": %d, \"batch\": %u,\n"
-            "  \"br_grid\": %d, \"br_latency_max\": %d, \"ntt_polys_per_wg\": %d, \"cohort_max\": %d,\n"
+            "  \"br_grid\": %d, \"br_pair\": %d, \"br_latency_max\": %d, \"ntt_polys_per_wg\": %d, \"cohort_max\": %d,\n"
             "  \"measured\": {\n    \"blind_rotation_width\": [%s],\n    \"latency_variant\": [%s],\n"
             "    \"ntt_polymul_4096\": [%s]\n  }\n}\n",
-            prop.gcnArchName, cus, c->prm.lwe_dim, L, batch, t.br_grid, t.br_latency_max, t.ntt_polys_per_wg,
+            prop.gcnArchName, cus, c->prm.lwe_dim, L, batch, t.br_grid, t.br_pair, t.br_latency_max, t.ntt_polys_per_wg,
             t.cohort_max, meas.c_str(), lat.c_str(), ntt.c_str());
     fclose(f);
   }

@@ -78,6 +78,7 @@ struct luthe_ctx {
   luthe_dop_batching batching{0, 0, 1};
   std::map<uint32_t, std::vector<uint32_t>> iop_table;  // key = max_align << 8 | opcode
   int br_grid_override = 0;
+  int br_pair = -1;          // throughput kernel: -1 default, 0 256-thread, 1 512-thread pair
   int br_latency_max = -1;  // cohorts up to this size use the latency kernel; -1 = its resident capacity
   int ntt_polys_per_wg = 2;  // polynomials per NTT workgroup (1, 2, 4)
   int cohort_max = 0;        // serving cohort size chosen by the autotuner (0 = blind-rotation width)
@@ -111,6 +112,8 @@ int dop_seal(luthe_ctx* c);
 int dop_progress(luthe_ctx* c, bool force);
 // Width of the ALU scalar constant in the body: msg + carry + 1 padding bit.
 uint32_t useful_bits(const luthe_ctx* c);
+// Whether batches above the latency threshold run the 512-thread pair kernel.
+bool use_pair(const luthe_ctx* c);
 // Largest cohort that runs the 512-thread latency kernel.
 int latency_max(luthe_ctx* c);
 

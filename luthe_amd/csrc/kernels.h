@@ -59,11 +59,13 @@ hipError_t launch_bsk_prepare(const uint64_t* src, uint64_t* dst, int n_lwe, int
                               const uint64_t* tab, hipStream_t s);
 hipError_t launch_keyswitch(const PbsJob* jobs, int nct, const uint32_t* ksk, uint16_t* lwe2n,
                             const KsArgs& a, int ks_level, hipStream_t s);
-// Cohorts of at most latency_max ciphertexts run the 512-thread latency variant.
+// Cohorts of at most latency_max ciphertexts run the 512-thread latency variant; larger ones
+// the 256-thread persistent kernel, or with `pair` the 512-thread two-per-CU variant.
 hipError_t launch_blind_rotate(const uint64_t* bsk, const uint64_t* luts, const uint16_t* lwe2n, const PbsJob* jobs,
                                int nct, const uint64_t* tab, const BrArgs& a,
-                               int pbs_level, int grid_override, hipStream_t s, int latency_max = 0);
-int blind_rotate_occupancy(int pbs_level);
+                               int pbs_level, int grid_override, hipStream_t s, int latency_max, bool pair);
+// Resident workgroups of the throughput kernel (CUs x workgroups per CU).
+int blind_rotate_occupancy(int pbs_level, bool pair);
 // Ciphertexts the latency variant holds resident at once (its workgroups per CU x CUs).
 int blind_rotate_latency_slots(int pbs_level);
 hipError_t launch_modmul_peak(uint64_t* out, int blocks, int iters, hipStream_t s);

@@ -270,9 +270,14 @@ __global__ __launch_bounds__(256, (L == 1 ? 4 : 2)) void k_blind_rotate(const ui
 // polynomials, output column j = h of the external product and its inverse NTT.  The
 // halves exchange NTT-domain digits through LDS before the product.  Arithmetic is the
 // same canonical Goldilocks arithmetic, so results equal k_blind_rotate's bit for bit.
+// TP (the throughput pair variant): two workgroups per CU (at most 128 VGPRs, so four
+// waves per SIMD), the key words read in groups of four outputs through a buffer resource
+// and the products summed as 128-bit values with one reduction per output, as in
+// k_blind_rotate; without TP one workgroup per CU holds its whole key slice in registers
+// (the latency variant for small cohorts).
 // ------------------------------------------------------------------------------------
-template <int K1, int L, bool HI>
-__global__ __launch_bounds__(512, 1) void k_blind_rotate_lat(const uint64_t* __restrict__ bsk,
+template <int K1, int L, bool HI, bool TP>
+__global__ __launch_bounds__(512, TP ? 2 : 1) void k_blind_rotate_lat(const uint64_t* __restrict__ bsk,
                                                              const uint64_t* __restrict__ luts,
                                                              const uint16_t* __restrict__ lwe2n,
                                                              const PbsJob* __restrict__ jobs, int nct,
@@ -285,6 +290,8 @@ __global__ __launch_bounds__(512, 1) void k_blind_rotate_lat(const uint64_t* __r
   const int r = t & 63, m0 = t >> 6;
   const size_t blen = (size_t)(K1 - 1) * N + 1;
   const size_t bsk_stride = (size_t)8 * NPF * K1 * T;
+  const __amdgpu_buffer_rsrc_t bsk_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)bsk, (short)0, (int)((size_t)a.n_lwe * bsk_stride * 8), 0x00020000);
   for (int c = blockIdx.x; c < nct; c += gridDim.x) {
     const PbsJob job = jobs[c];
     if (job.out_addr == 0) continue;  // an empty slot of a batch list (uniform per workgroup)
@@ -301,12 +308,14 @@ __global__ __launch_bounds__(512, 1) void k_blind_rotate_lat(const uint64_t* __r
       int tt = t;
       asm volatile("" : "+v"(tt));
       const int rr = tt & 63, mm0 = tt >> 6;
-      // key slice of this half's output column, issued early
-      uint64_t kb[8][NPF];
+      // key slice of this half's output column, issued early (latency variant)
+      uint64_t kb[TP ? 1 : 8][TP ? 1 : NPF];
+      if constexpr (!TP) {
 #pragma unroll
-      for (int w0 = 0; w0 < 8; w0++)
+        for (int w0 = 0; w0 < 8; w0++)
 #pragma unroll
-        for (int gl_ = 0; gl_ < NPF; gl_++) kb[w0][gl_] = bsk_i[((w0 * NPF + gl_) * K1 + h) * T + tt];
+          for (int gl_ = 0; gl_ < NPF; gl_++) kb[w0][gl_] = bsk_i[((w0 * NPF + gl_) * K1 + h) * T + tt];
+      }
       __syncthreads();
 #pragma unroll
       for (int m1 = 0; m1 < 8; m1++) lds[h * N + 64 * (mm0 + 4 * m1) + rr] = acc[m1];
@@ -323,15 +332,52 @@ __global__ __launch_bounds__(512, 1) void k_blind_rotate_lat(const uint64_t* __r
         for (int w0 = 0; w0 < 8; w0++) lds[((h * L + l) * 8 + w0) * T + tt] = v[l][w0];
       __syncthreads();
       uint64_t z[1][8];
+      if constexpr (TP) {
+        // outputs w0 in groups of four: the group's key words first, then the 128-bit sums
+        const int hu = wave_uniform(h);
+        const int slice_off = (int)((size_t)i * bsk_stride * 8);
+        const int lane_off = tt * 8;
 #pragma unroll
-      for (int w0 = 0; w0 < 8; w0++) {
-        uint64_t s = 0;
+        for (int q = 0; q < 8; q += 4) {
+          uint64_t kw[4][NPF];
 #pragma unroll
-        for (int gl_ = 0; gl_ < NPF; gl_++) {
-          const uint64_t d = lds[(gl_ * 8 + w0) * T + tt];
-          s = gl::add(s, gl::mul(d, kb[w0][gl_]));
+          for (int u = 0; u < 4; u++)
+#pragma unroll
+            for (int gl_ = 0; gl_ < NPF; gl_++) {
+              const int e = ((q + u) * NPF + gl_) * K1 + hu;
+              const auto w = __builtin_amdgcn_raw_buffer_load_b64(bsk_rsrc, lane_off, slice_off + e * T * 8, 0);
+              kw[u][gl_] = ((uint64_t)w[1] << 32) | w[0];
+            }
+          uint64_t lo[4], hi[4], o[4];
+          uint32_t cc[4];
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            cc[u] = 0;
+#pragma unroll
+            for (int gl_ = 0; gl_ < NPF; gl_++) {
+              uint64_t plo, phi;
+              gl::mul_full(lds[(gl_ * 8 + q + u) * T + tt], kw[u][gl_], plo, phi);
+              if (gl_ == 0)
+                lo[u] = plo, hi[u] = phi;
+              else
+                gl::acc128(lo[u], hi[u], cc[u], plo, phi);
+            }
+          }
+          gl::reduce160x4(o, lo, hi, cc);
+#pragma unroll
+          for (int u = 0; u < 4; u++) z[0][q + u] = o[u];
         }
-        z[0][w0] = s;
+      } else {
+#pragma unroll
+        for (int w0 = 0; w0 < 8; w0++) {
+          uint64_t s = 0;
+#pragma unroll
+          for (int gl_ = 0; gl_ < NPF; gl_++) {
+            const uint64_t d = lds[(gl_ * 8 + w0) * T + tt];
+            s = gl::add(s, gl::mul(d, kb[w0][gl_]));
+          }
+          z[0][w0] = s;
+        }
       }
       inv_passes<1>(z, lds + h * N, tab, tt);
 #pragma unroll
@@ -553,14 +599,14 @@ static int br_max_grid() {
   return cached;
 }
 
-template <int K1, int L>
+template <int K1, int L, bool TP>
 static int br_lat_max_grid() {
   static int cached = 0;
   if (!cached) {
     int dev = 0, cus = 0, per = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_blind_rotate_lat<K1, L, true>, 512, 0);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_blind_rotate_lat<K1, L, true, TP>, 512, 0);
     cached = (cus > 0 ? cus : 256) * (per > 0 ? per : 1);
   }
   return cached;
@@ -569,11 +615,15 @@ static int br_lat_max_grid() {
 template <int L, bool HI>
 static void br_launch(const uint64_t* bsk, const uint64_t* luts, const uint16_t* lwe2n, const PbsJob* jobs, int nct,
                       const uint64_t* tab, const BrArgs& a, int grid_override, hipStream_t s,
-                      bool latency) {
+                      bool latency, bool pair) {
   if (latency) {
-    int g = br_lat_max_grid<2, L>();
+    int g = br_lat_max_grid<2, L, false>();
     g = g < nct ? g : nct;
-    k_blind_rotate_lat<2, L, HI><<<dim3(g), 512, 0, s>>>(bsk, luts, lwe2n, jobs, nct, tab, a);
+    k_blind_rotate_lat<2, L, HI, false><<<dim3(g), 512, 0, s>>>(bsk, luts, lwe2n, jobs, nct, tab, a);
+  } else if (pair) {
+    int g = grid_override > 0 ? grid_override : br_lat_max_grid<2, L, true>();
+    g = g < nct ? g : nct;
+    k_blind_rotate_lat<2, L, HI, true><<<dim3(g), 512, 0, s>>>(bsk, luts, lwe2n, jobs, nct, tab, a);
   } else {
     int g = grid_override > 0 ? grid_override : br_max_grid<2, L>();
     g = g < nct ? g : nct;
@@ -583,7 +633,7 @@ static void br_launch(const uint64_t* bsk, const uint64_t* luts, const uint16_t*
 
 hipError_t launch_blind_rotate(const uint64_t* bsk, const uint64_t* luts, const uint16_t* lwe2n, const PbsJob* jobs,
                                int nct, const uint64_t* tab, const BrArgs& a,
-                               int pbs_level, int grid_override, hipStream_t s, int latency_max) {
+                               int pbs_level, int grid_override, hipStream_t s, int latency_max, bool pair) {
   if (nct == 0) return hipSuccess;
   if (pbs_level != 1 && pbs_level != 2) return hipErrorInvalidValue;
   // the batched kernel addresses the key through a buffer resource of at most 2^31 - 1 bytes
@@ -592,24 +642,25 @@ hipError_t launch_blind_rotate(const uint64_t* bsk, const uint64_t* luts, const 
   const bool hi = (uint32_t)pbs_level * a.base_log <= 31 && a.base_log <= 23;
   if (pbs_level == 1) {
     if (hi)
-      br_launch<1, true>(bsk, luts, lwe2n, jobs, nct, tab, a, grid_override, s, latency);
+      br_launch<1, true>(bsk, luts, lwe2n, jobs, nct, tab, a, grid_override, s, latency, pair);
     else
-      br_launch<1, false>(bsk, luts, lwe2n, jobs, nct, tab, a, grid_override, s, latency);
+      br_launch<1, false>(bsk, luts, lwe2n, jobs, nct, tab, a, grid_override, s, latency, pair);
   } else {
     if (hi)
-      br_launch<2, true>(bsk, luts, lwe2n, jobs, nct, tab, a, grid_override, s, latency);
+      br_launch<2, true>(bsk, luts, lwe2n, jobs, nct, tab, a, grid_override, s, latency, pair);
     else
-      br_launch<2, false>(bsk, luts, lwe2n, jobs, nct, tab, a, grid_override, s, latency);
+      br_launch<2, false>(bsk, luts, lwe2n, jobs, nct, tab, a, grid_override, s, latency, pair);
   }
   return hipGetLastError();
 }
 
-int blind_rotate_occupancy(int pbs_level) {
+int blind_rotate_occupancy(int pbs_level, bool pair) {
+  if (pair) return pbs_level == 1 ? br_lat_max_grid<2, 1, true>() : br_lat_max_grid<2, 2, true>();
   return pbs_level == 1 ? br_max_grid<2, 1>() : br_max_grid<2, 2>();
 }
 
 int blind_rotate_latency_slots(int pbs_level) {
-  return pbs_level == 1 ? br_lat_max_grid<2, 1>() : br_lat_max_grid<2, 2>();
+  return pbs_level == 1 ? br_lat_max_grid<2, 1, false>() : br_lat_max_grid<2, 2, false>();
 }
 
 hipError_t launch_alu(uint64_t* base, const AluOp* ops, int nops, int blen, uint32_t useful_bits, hipStream_t s) {

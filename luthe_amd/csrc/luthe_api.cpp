@@ -175,6 +175,11 @@ hipStream_t pick_stream(luthe_ctx* c, void* s) { return s ? (hipStream_t)s : c->
 
 uint32_t useful_bits(const luthe_ctx* c) { return c->prm.msg_bits + c->prm.carry_bits + 1; }
 
+// Throughput kernel: the 512-thread two-per-CU pair variant by default at PBS level 2 (its
+// four waves per SIMD against the 256-thread kernel's two there), the 256-thread kernel at
+// level 1; luthe_set_tuning / the autotuner override.
+bool use_pair(const luthe_ctx* c) { return c->br_pair < 0 ? c->prm.pbs_level == 2 : c->br_pair == 1; }
+
 int latency_max(luthe_ctx* c) {
   if (c->br_latency_max < 0) {
     (void)hipSetDevice(c->device);
@@ -194,7 +199,7 @@ int run_pbs_on(luthe_ctx* c, const PbsJob* d_jobs, int n, hipStream_t s,
   HIPCHK(c, lk::launch_keyswitch(d_jobs, n, c->d_ksk, lwe2n, ka, (int)p.ks_level, s));
   lk::BrArgs ba{(int)p.lwe_dim, c->lwe_stride, p.pbs_base_log};
   HIPCHK(c, lk::launch_blind_rotate(c->d_bsk, c->d_luts, lwe2n, d_jobs, n, c->d_tab, ba,
-                                    (int)p.pbs_level, c->br_grid_override, s, latency_max(c)));
+                                    (int)p.pbs_level, c->br_grid_override, s, latency_max(c), use_pair(c)));
   c->cnt.pbs += (uint64_t)n;
   c->cnt.pbs_batches += 1;
   c->cnt.pbs_max_batch = std::max<uint64_t>(c->cnt.pbs_max_batch, (uint64_t)n);
@@ -356,7 +361,7 @@ int luthe_get_br_grid(luthe_ctx* c) {
   if (!c) return LUTHE_ERR_ARG;
   if (c->br_grid_override) return c->br_grid_override;
   (void)hipSetDevice(c->device);
-  return lk::blind_rotate_occupancy((int)c->prm.pbs_level);
+  return lk::blind_rotate_occupancy((int)c->prm.pbs_level, use_pair(c));
 }
 
 int luthe_microbench_modmul(luthe_ctx* c, double* gmodmul_per_s) {
@@ -503,7 +508,7 @@ int luthe_blind_rotate_batch(luthe_ctx* c, const uint16_t* d_lwe2n, uint64_t* d_
   const luthe_params& p = c->prm;
   lk::BrArgs ba{(int)p.lwe_dim, (int)p.lwe_dim + 1, p.pbs_base_log};
   HIPCHK(c, lk::launch_blind_rotate(c->d_bsk, c->d_luts, d_lwe2n, c->d_jobs, (int)n, c->d_tab, ba,
-                                    (int)p.pbs_level, c->br_grid_override, s, latency_max(c)));
+                                    (int)p.pbs_level, c->br_grid_override, s, latency_max(c), use_pair(c)));
   return LUTHE_OK;
 }
 
