@@ -267,6 +267,7 @@ def main():
                     help="extra workloads to run beside the headline (ntt, uneven, serving)")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (library defaults)")
     ap.add_argument("--tuning-json", default="", help="where the autotuner writes its choices")
+    ap.add_argument("--tuning-file", default="", help="apply a saved tuning instead of running the autotuner")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="only run a device-to-device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -300,7 +301,11 @@ def main():
     lut = la.lut_encode(prm, np.arange(16, dtype=np.uint8))
     ctx.write_lut(0, lut)
     tuning = None
-    if not args.no_tune and not args.br_grid:
+    if args.tuning_file:
+        ctx.load_tuning(args.tuning_file)
+        tuning = ctx.tuning()
+        log(f"bench: tuning from {args.tuning_file}: {tuning}")
+    elif not args.no_tune and not args.br_grid:
         t0 = time.perf_counter()
         tuning = ctx.autotune(args.batch, args.tuning_json or None)
         log(f"bench: autotuned in {time.perf_counter() - t0:.1f} s: {tuning}")

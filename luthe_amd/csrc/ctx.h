@@ -51,6 +51,9 @@ struct luthe_ctx {
   int lwe_stride = 0;
   lk::PbsJob* d_jobs = nullptr;
   size_t jobs_cap = 0;
+  uint32_t* d_ks_part = nullptr;  // partial sums of the split key switch (small batches)
+  size_t ks_part_cap = 0;         // words
+  size_t ks_part_for = 0;         // batch sizes up to this are covered
   uint8_t* d_stage = nullptr;  // per-submission descriptors
   size_t stage_cap = 0;
   uint8_t* h_stage = nullptr;  // pinned mirror
@@ -99,12 +102,15 @@ int dev_alloc(luthe_ctx* c, void** p, size_t bytes);
 void retire_scratch(luthe_ctx* c, void* p, size_t bytes);
 int ensure_lwe2n(luthe_ctx* c, size_t n);
 int ensure_jobs(luthe_ctx* c, size_t n);
+// Scratch for the split key switch of batches of up to n ciphertexts.
+int ensure_ks_part(luthe_ctx* c, size_t n);
 int ensure_rename_rows(luthe_ctx* c, size_t n);
 // Key switch + blind rotation + sample extract of n jobs whose descriptors are on the
 // device (their BLWE addresses inside).
 int run_pbs(luthe_ctx* c, const lk::PbsJob* d_jobs, int n, hipStream_t s);
 // The same with the mod-switched LWEs in caller-provided device memory; allocates nothing.
-int run_pbs_on(luthe_ctx* c, const lk::PbsJob* d_jobs, int n, hipStream_t s, uint16_t* lwe2n);
+// ks_part: the split key switch's scratch (NULL: no split).
+int run_pbs_on(luthe_ctx* c, const lk::PbsJob* d_jobs, int n, hipStream_t s, uint16_t* lwe2n, uint32_t* ks_part);
 // Looks up the no-allocation entry point's device scratch on `device` (current).
 int noalloc_init(luthe_ctx* c, int device);
 // DOp groups: seal (launch) the open group now; check the sealing rules (force: seal).

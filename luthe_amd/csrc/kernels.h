@@ -57,8 +57,12 @@ hipError_t launch_ntt(int mode, const uint64_t* a, const uint64_t* bn, uint64_t*
                       const uint64_t* tab, hipStream_t s, int polys_per_wg = 2);
 hipError_t launch_bsk_prepare(const uint64_t* src, uint64_t* dst, int n_lwe, int rows, int K1, int is_ntt,
                               const uint64_t* tab, hipStream_t s);
+// nsplit > 1 (small batches): the input coefficients split over nsplit workgroups whose partial
+// sums go to part (nsplit * nct * (n_lwe + 1) words), then added by a finishing kernel.
 hipError_t launch_keyswitch(const PbsJob* jobs, int nct, const uint32_t* ksk, uint16_t* lwe2n,
-                            const KsArgs& a, int ks_level, hipStream_t s);
+                            const KsArgs& a, int ks_level, hipStream_t s, uint32_t* part = nullptr, int nsplit = 1);
+// The split launch_keyswitch uses for a batch of nct ciphertexts (1 = none).
+int keyswitch_split(int nct, int kN, int n_lwe);
 // Cohorts of at most latency_max ciphertexts run the 512-thread latency variant; larger ones
 // the 256-thread persistent kernel, or with `pair` the 512-thread two-per-CU variant.
 hipError_t launch_blind_rotate(const uint64_t* bsk, const uint64_t* luts, const uint16_t* lwe2n, const PbsJob* jobs,

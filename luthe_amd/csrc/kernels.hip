@@ -270,11 +270,10 @@ __global__ __launch_bounds__(256, (L == 1 ? 4 : 2)) void k_blind_rotate(const ui
 // polynomials, output column j = h of the external product and its inverse NTT.  The
 // halves exchange NTT-domain digits through LDS before the product.  Arithmetic is the
 // same canonical Goldilocks arithmetic, so results equal k_blind_rotate's bit for bit.
+// The key words are read in groups of four outputs through a buffer resource and the
+// products summed as 128-bit values with one reduction per output, as in k_blind_rotate.
 // TP (the throughput pair variant): two workgroups per CU (at most 128 VGPRs, so four
-// waves per SIMD), the key words read in groups of four outputs through a buffer resource
-// and the products summed as 128-bit values with one reduction per output, as in
-// k_blind_rotate; without TP one workgroup per CU holds its whole key slice in registers
-// (the latency variant for small cohorts).
+// waves per SIMD); without TP one workgroup per CU (the latency variant for small cohorts).
 // ------------------------------------------------------------------------------------
 template <int K1, int L, bool HI, bool TP>
 __global__ __launch_bounds__(512, TP ? 2 : 1) void k_blind_rotate_lat(const uint64_t* __restrict__ bsk,
@@ -304,18 +303,9 @@ __global__ __launch_bounds__(512, TP ? 2 : 1) void k_blind_rotate_lat(const uint
     for (int i = 0; i < a.n_lwe; i++) {
       const uint32_t rot = rot_next;
       rot_next = a2n[i + 1];  // i + 1 <= n_lwe: the body slot, always present
-      const uint64_t* bsk_i = bsk + (size_t)i * bsk_stride;
       int tt = t;
       asm volatile("" : "+v"(tt));
       const int rr = tt & 63, mm0 = tt >> 6;
-      // key slice of this half's output column, issued early (latency variant)
-      uint64_t kb[TP ? 1 : 8][TP ? 1 : NPF];
-      if constexpr (!TP) {
-#pragma unroll
-        for (int w0 = 0; w0 < 8; w0++)
-#pragma unroll
-          for (int gl_ = 0; gl_ < NPF; gl_++) kb[w0][gl_] = bsk_i[((w0 * NPF + gl_) * K1 + h) * T + tt];
-      }
       __syncthreads();
 #pragma unroll
       for (int m1 = 0; m1 < 8; m1++) lds[h * N + 64 * (mm0 + 4 * m1) + rr] = acc[m1];
@@ -332,52 +322,39 @@ __global__ __launch_bounds__(512, TP ? 2 : 1) void k_blind_rotate_lat(const uint
         for (int w0 = 0; w0 < 8; w0++) lds[((h * L + l) * 8 + w0) * T + tt] = v[l][w0];
       __syncthreads();
       uint64_t z[1][8];
-      if constexpr (TP) {
-        // outputs w0 in groups of four: the group's key words first, then the 128-bit sums
-        const int hu = wave_uniform(h);
-        const int slice_off = (int)((size_t)i * bsk_stride * 8);
-        const int lane_off = tt * 8;
+      // outputs w0 in groups of four: the group's key words first, then the 128-bit sums
+      const int hu = wave_uniform(h);
+      const int slice_off = (int)((size_t)i * bsk_stride * 8);
+      const int lane_off = tt * 8;
 #pragma unroll
-        for (int q = 0; q < 8; q += 4) {
-          uint64_t kw[4][NPF];
+      for (int q = 0; q < 8; q += 4) {
+        uint64_t kw[4][NPF];
 #pragma unroll
-          for (int u = 0; u < 4; u++)
-#pragma unroll
-            for (int gl_ = 0; gl_ < NPF; gl_++) {
-              const int e = ((q + u) * NPF + gl_) * K1 + hu;
-              const auto w = __builtin_amdgcn_raw_buffer_load_b64(bsk_rsrc, lane_off, slice_off + e * T * 8, 0);
-              kw[u][gl_] = ((uint64_t)w[1] << 32) | w[0];
-            }
-          uint64_t lo[4], hi[4], o[4];
-          uint32_t cc[4];
-#pragma unroll
-          for (int u = 0; u < 4; u++) {
-            cc[u] = 0;
-#pragma unroll
-            for (int gl_ = 0; gl_ < NPF; gl_++) {
-              uint64_t plo, phi;
-              gl::mul_full(lds[(gl_ * 8 + q + u) * T + tt], kw[u][gl_], plo, phi);
-              if (gl_ == 0)
-                lo[u] = plo, hi[u] = phi;
-              else
-                gl::acc128(lo[u], hi[u], cc[u], plo, phi);
-            }
-          }
-          gl::reduce160x4(o, lo, hi, cc);
-#pragma unroll
-          for (int u = 0; u < 4; u++) z[0][q + u] = o[u];
-        }
-      } else {
-#pragma unroll
-        for (int w0 = 0; w0 < 8; w0++) {
-          uint64_t s = 0;
+        for (int u = 0; u < 4; u++)
 #pragma unroll
           for (int gl_ = 0; gl_ < NPF; gl_++) {
-            const uint64_t d = lds[(gl_ * 8 + w0) * T + tt];
-            s = gl::add(s, gl::mul(d, kb[w0][gl_]));
+            const int e = ((q + u) * NPF + gl_) * K1 + hu;
+            const auto w = __builtin_amdgcn_raw_buffer_load_b64(bsk_rsrc, lane_off, slice_off + e * T * 8, 0);
+            kw[u][gl_] = ((uint64_t)w[1] << 32) | w[0];
           }
-          z[0][w0] = s;
+        uint64_t lo[4], hi[4], o[4];
+        uint32_t cc[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          cc[u] = 0;
+#pragma unroll
+          for (int gl_ = 0; gl_ < NPF; gl_++) {
+            uint64_t plo, phi;
+            gl::mul_full(lds[(gl_ * 8 + q + u) * T + tt], kw[u][gl_], plo, phi);
+            if (gl_ == 0)
+              lo[u] = plo, hi[u] = phi;
+            else
+              gl::acc128(lo[u], hi[u], cc[u], plo, phi);
+          }
         }
+        gl::reduce160x4(o, lo, hi, cc);
+#pragma unroll
+        for (int u = 0; u < 4; u++) z[0][q + u] = o[u];
       }
       inv_passes<1>(z, lds + h * N, tab, tt);
 #pragma unroll

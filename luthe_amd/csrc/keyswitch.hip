@@ -17,14 +17,19 @@ constexpr int KS_MAX_GRID_Y = 65535;  // grid.y limit: launches split their ciph
 // Key switch.  Workgroup = 32 ciphertexts x 256 output coefficients; K dimension =
 // (k*N mask coefficients) x KSL levels, streamed in chunks of 64 coefficients whose
 // digits are decomposed once into LDS and broadcast to all threads.
+// PART (small batches): workgroup z of gridDim.z takes only the mask coefficients
+// [z * kN / gridDim.z, (z + 1) * kN / gridDim.z) and writes its sums to part[z][c][j];
+// k_ks_finish adds the gridDim.z partial sums (mod 2^32, in a fixed order) and finishes.
+// The sums are exact mod 2^32, so the split changes no result.
 // ------------------------------------------------------------------------------------
 constexpr int KS_CT = 32;
 constexpr int KS_CHUNK = 64;
 
-template <int KSL>
+template <int KSL, bool PART>
 __global__ __launch_bounds__(256) void k_keyswitch(const PbsJob* __restrict__ jobs,
                                                    int nct, const uint32_t* __restrict__ ksk,
-                                                   uint16_t* __restrict__ lwe2n, KsArgs a) {
+                                                   uint16_t* __restrict__ lwe2n, uint32_t* __restrict__ part,
+                                                   KsArgs a) {
   __shared__ int32_t dig[KS_CHUNK][KSL][KS_CT / 4];  // 4 signed 8-bit digits per word
   const int t = threadIdx.x;
   const int c0 = blockIdx.y * KS_CT;
@@ -34,7 +39,9 @@ __global__ __launch_bounds__(256) void k_keyswitch(const PbsJob* __restrict__ jo
 #pragma unroll
   for (int c = 0; c < KS_CT; c++) acc[c] = 0;
   int8_t* dig8 = reinterpret_cast<int8_t*>(&dig[0][0][0]);
-  for (int i0 = 0; i0 < a.kN; i0 += KS_CHUNK) {
+  const int span = PART ? a.kN / (int)gridDim.z : a.kN;  // a multiple of KS_CHUNK (host-checked)
+  const int i_begin = PART ? (int)blockIdx.z * span : 0, i_end = i_begin + span;
+  for (int i0 = i_begin; i0 < i_end; i0 += KS_CHUNK) {
     __syncthreads();
     // 32 x 64 coefficients, 8 per thread; a row's coefficients are contiguous.
 #pragma unroll
@@ -71,6 +78,16 @@ __global__ __launch_bounds__(256) void k_keyswitch(const PbsJob* __restrict__ jo
       }
     }
   }
+  if (PART) {
+    if (j < n1) {
+#pragma unroll
+      for (int c = 0; c < KS_CT; c++) {
+        if (c0 + c >= nct) break;
+        part[((size_t)blockIdx.z * nct + c0 + c) * n1 + j] = acc[c];
+      }
+    }
+    return;
+  }
   if (j < n1) {
 #pragma unroll
     for (int c = 0; c < KS_CT; c++) {
@@ -85,9 +102,50 @@ __global__ __launch_bounds__(256) void k_keyswitch(const PbsJob* __restrict__ jo
   }
 }
 
+// Sum of the split key switch's partial sums for output j of ciphertext c, then the body and
+// the switch to 2N as in k_keyswitch.
+__global__ __launch_bounds__(256) void k_ks_finish(const PbsJob* __restrict__ jobs, int nct, int nsplit,
+                                                   const uint32_t* __restrict__ part, uint16_t* __restrict__ lwe2n,
+                                                   KsArgs a) {
+  const int n1 = a.n_lwe + 1;
+  const int j = blockIdx.x * 256 + threadIdx.x, c = blockIdx.y;
+  if (j >= n1 || c >= nct) return;
+  uint32_t acc = 0;
+  for (int z = 0; z < nsplit; z++) acc += part[((size_t)z * nct + c) * n1 + j];
+  uint32_t body = 0;
+  if (j == a.n_lwe) {
+    const uint64_t in = jobs[c].in_addr;
+    if (in) body = pbs::ks_body(reinterpret_cast<const uint64_t*>(in)[a.kN], a.mod_log);
+  }
+  lwe2n[(size_t)c * a.lwe_stride + j] = (uint16_t)pbs::ks_finish(acc, body, a.mod_log, a.lc);
+}
+
+int keyswitch_split(int nct, int kN, int n_lwe) {
+  // enough workgroups to spread a small batch over the GPU; each keeps >= one 64-coefficient chunk
+  const long wgs = (long)((n_lwe + 1 + 255) / 256) * ((nct + KS_CT - 1) / KS_CT);
+  int s = 1;
+  while (s < 32 && wgs * s * 2 <= 512 && (kN % (s * 2 * KS_CHUNK)) == 0) s *= 2;
+  return s;
+}
+
 hipError_t launch_keyswitch(const PbsJob* jobs, int nct, const uint32_t* ksk, uint16_t* lwe2n,
-                            const KsArgs& a, int ks_level, hipStream_t s) {
+                            const KsArgs& a, int ks_level, hipStream_t s, uint32_t* part, int nsplit) {
   if (ks_level < 1 || ks_level > 8) return hipErrorInvalidValue;
+  if (nsplit > 1) {
+    if (!part || nct > KS_MAX_GRID_Y || a.kN % (nsplit * KS_CHUNK)) return hipErrorInvalidValue;
+    dim3 grid((unsigned)((a.n_lwe + 1 + 255) / 256), (unsigned)((nct + KS_CT - 1) / KS_CT), (unsigned)nsplit);
+    switch (ks_level) {
+#define KS_PCASE(L) \
+  case L: k_keyswitch<L, true><<<grid, 256, 0, s>>>(jobs, nct, ksk, nullptr, part, a); break;
+      KS_PCASE(1) KS_PCASE(2) KS_PCASE(3) KS_PCASE(4) KS_PCASE(5) KS_PCASE(6) KS_PCASE(7) KS_PCASE(8)
+#undef KS_PCASE
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_ks_finish<<<dim3((unsigned)((a.n_lwe + 1 + 255) / 256), (unsigned)nct), 256, 0, s>>>(jobs, nct, nsplit, part,
+                                                                                          lwe2n, a);
+    return hipGetLastError();
+  }
   // grid.y is limited to 65535: chunks of at most 65535 * KS_CT ciphertexts
   for (int c0 = 0; c0 < nct; c0 += KS_MAX_GRID_Y * KS_CT) {
     const int nc = (nct - c0) < KS_MAX_GRID_Y * KS_CT ? (nct - c0) : KS_MAX_GRID_Y * KS_CT;
@@ -96,7 +154,7 @@ hipError_t launch_keyswitch(const PbsJob* jobs, int nct, const uint32_t* ksk, ui
     uint16_t* out = lwe2n + (size_t)c0 * a.lwe_stride;
     switch (ks_level) {
 #define KS_CASE(L) \
-  case L: k_keyswitch<L><<<grid, 256, 0, s>>>(jb, nc, ksk, out, a); break;
+  case L: k_keyswitch<L, false><<<grid, 256, 0, s>>>(jb, nc, ksk, out, nullptr, a); break;
       KS_CASE(1) KS_CASE(2) KS_CASE(3) KS_CASE(4) KS_CASE(5) KS_CASE(6) KS_CASE(7) KS_CASE(8)
 #undef KS_CASE
     }

@@ -138,6 +138,36 @@ int ensure_jobs(luthe_ctx* c, size_t n) {
   return LUTHE_OK;
 }
 
+// Words of the split key switch's partial sums for a batch of n ciphertexts.
+static size_t ks_part_words(const luthe_ctx* c, size_t n) {
+  const int kN = (int)(c->prm.glwe_k * c->prm.poly_size), nl = (int)c->prm.lwe_dim;
+  const int s = lk::keyswitch_split((int)std::min<size_t>(n, 1u << 20), kN, nl);
+  return s > 1 ? (size_t)s * n * (nl + 1) : 0;
+}
+
+int ensure_ks_part(luthe_ctx* c, size_t n) {
+  n = std::min<size_t>(n, 4096);  // batches beyond a few thousand are not split
+  if (n <= c->ks_part_for) return LUTHE_OK;
+  size_t need = 0;  // the largest split scratch of any batch size up to n
+  for (size_t m = 1; m <= n; m++) need = std::max(need, ks_part_words(c, m));
+  if (need <= c->ks_part_cap) {
+    c->ks_part_for = n;
+    return LUTHE_OK;
+  }
+  if (need <= c->ks_part_cap) return LUTHE_OK;
+  uint32_t* nb = nullptr;
+  int rc = dev_alloc(c, (void**)&nb, need * sizeof(uint32_t));
+  if (rc) return rc;
+  if (c->d_ks_part) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    retire_scratch(c, c->d_ks_part, c->ks_part_cap * sizeof(uint32_t));
+  }
+  c->d_ks_part = nb;
+  c->ks_part_cap = need;
+  c->ks_part_for = n;
+  return LUTHE_OK;
+}
+
 // Staging of per-submission descriptors: host pinned buffer -> device buffer.
 int stage_upload(luthe_ctx* c, const std::vector<uint8_t>& bytes) {
   size_t n = bytes.size();
@@ -191,12 +221,13 @@ int latency_max(luthe_ctx* c) {
 // KS + BR over jobs already on the device, with the mod-switched LWEs in `lwe2n`
 // (n rows of lwe_stride values).  Allocates nothing.
 int run_pbs_on(luthe_ctx* c, const PbsJob* d_jobs, int n, hipStream_t s,
-               uint16_t* lwe2n) {
+               uint16_t* lwe2n, uint32_t* ks_part) {
   if (!c->bsk_ok || !c->ksk_ok) return fail(c, LUTHE_ERR_STATE, "keys not uploaded");
   if (n == 0) return LUTHE_OK;
   const luthe_params& p = c->prm;
   lk::KsArgs ka{(int)(p.glwe_k * p.poly_size), (int)p.lwe_dim, c->lwe_stride, p.ks_base_log, p.ks_mod_log, 12};
-  HIPCHK(c, lk::launch_keyswitch(d_jobs, n, c->d_ksk, lwe2n, ka, (int)p.ks_level, s));
+  const int nsplit = ks_part ? lk::keyswitch_split(n, ka.kN, ka.n_lwe) : 1;
+  HIPCHK(c, lk::launch_keyswitch(d_jobs, n, c->d_ksk, lwe2n, ka, (int)p.ks_level, s, ks_part, nsplit));
   lk::BrArgs ba{(int)p.lwe_dim, c->lwe_stride, p.pbs_base_log};
   HIPCHK(c, lk::launch_blind_rotate(c->d_bsk, c->d_luts, lwe2n, d_jobs, n, c->d_tab, ba,
                                     (int)p.pbs_level, c->br_grid_override, s, latency_max(c), use_pair(c)));
@@ -211,8 +242,9 @@ int run_pbs(luthe_ctx* c, const PbsJob* d_jobs, int n, hipStream_t s) {
   if (!c->bsk_ok || !c->ksk_ok) return fail(c, LUTHE_ERR_STATE, "keys not uploaded");
   if (n == 0) return LUTHE_OK;
   int rc = ensure_lwe2n(c, (size_t)n);
+  if (!rc) rc = ensure_ks_part(c, (size_t)n);
   if (rc) return rc;
-  return run_pbs_on(c, d_jobs, n, s, c->d_lwe2n);
+  return run_pbs_on(c, d_jobs, n, s, c->d_lwe2n, c->d_ks_part);
 }
 
 void trace_push(luthe_ctx* c, const luthe_trace_event& e) {
@@ -312,7 +344,7 @@ int luthe_destroy(luthe_ctx* c) {
     if (kv.second.done) (void)hipEventDestroy(kv.second.done);
   for (luthe_graph* g : c->graphs) g->ctx = nullptr;  // their scratch is freed below
   for (auto& r : c->retired) (void)hipFree(r.first);
-  void* bufs[] = {c->d_mem, c->d_luts, c->d_bsk, c->d_ksk, c->d_tab, c->d_lwe2n, c->d_jobs, c->d_stage};
+  void* bufs[] = {c->d_mem, c->d_luts, c->d_bsk, c->d_ksk, c->d_tab, c->d_lwe2n, c->d_jobs, c->d_stage, c->d_ks_part};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (c->h_stage) (void)hipHostFree(c->h_stage);
@@ -489,7 +521,10 @@ int luthe_keyswitch_batch(luthe_ctx* c, const uint64_t* d_blwe, uint16_t* d_lwe2
   HIPCHK(c, lk::launch_make_jobs(c->d_jobs, d_blwe, nullptr, nullptr, (int)n, 0, c->prm.lut_slots, (int)c->blen, s));
   const luthe_params& p = c->prm;
   lk::KsArgs ka{(int)(p.glwe_k * p.poly_size), (int)p.lwe_dim, (int)p.lwe_dim + 1, p.ks_base_log, p.ks_mod_log, 12};
-  HIPCHK(c, lk::launch_keyswitch(c->d_jobs, (int)n, c->d_ksk, d_lwe2n, ka, (int)p.ks_level, s));
+  rc = ensure_ks_part(c, n);
+  if (rc) return rc;
+  HIPCHK(c, lk::launch_keyswitch(c->d_jobs, (int)n, c->d_ksk, d_lwe2n, ka, (int)p.ks_level, s, c->d_ks_part,
+                                 lk::keyswitch_split((int)n, ka.kN, ka.n_lwe)));
   return LUTHE_OK;
 }
 
@@ -561,6 +596,7 @@ int luthe_reserve(luthe_ctx* c, size_t max_batch) {
   if (!c) return LUTHE_ERR_ARG;
   HIPCHK(c, hipSetDevice(c->device));
   int rc = ensure_jobs(c, max_batch);
+  if (!rc) rc = ensure_ks_part(c, max_batch);
   if (rc) return rc;
   return ensure_lwe2n(c, max_batch);
 }

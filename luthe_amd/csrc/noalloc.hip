@@ -76,5 +76,5 @@ extern "C" int luthe_pbs_batch_noalloc(luthe_ctx* c, const uint64_t* d_in, uint6
   hipStream_t s = stream ? (hipStream_t)stream : c->stream;
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, lk::launch_make_jobs(jobs, d_in, d_out, d_gid, (int)n, log_lut_nb, c->prm.lut_slots, (int)c->blen, s));
-  return lx::run_pbs_on(c, jobs, (int)n, s, lwe2n);
+  return lx::run_pbs_on(c, jobs, (int)n, s, lwe2n, nullptr);  // the unsplit key switch: no partial-sum scratch
 }
