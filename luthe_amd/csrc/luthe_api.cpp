@@ -146,10 +146,15 @@ static size_t ks_part_words(const luthe_ctx* c, size_t n) {
 }
 
 int ensure_ks_part(luthe_ctx* c, size_t n) {
-  n = std::min<size_t>(n, 4096);  // batches beyond a few thousand are not split
   if (n <= c->ks_part_for) return LUTHE_OK;
-  size_t need = 0;  // the largest split scratch of any batch size up to n
-  for (size_t m = 1; m <= n; m++) need = std::max(need, ks_part_words(c, m));
+  // the largest split scratch of any batch size up to n: the split depends on the number of
+  // 32-ciphertext tiles only, so the maximum sits at the last size of some tile count
+  size_t need = 0;
+  for (size_t tiles = 1; tiles <= (n + 31) / 32; tiles++) {
+    const size_t w = ks_part_words(c, std::min(n, tiles * 32));
+    if (w == 0) break;  // more tiles: no split
+    need = std::max(need, w);
+  }
   if (need <= c->ks_part_cap) {
     c->ks_part_for = n;
     return LUTHE_OK;
