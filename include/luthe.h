@@ -528,9 +528,11 @@ int luthe_register_iop_program(luthe_ctx* ctx, uint32_t op, uint32_t nblocks, ui
  * (ucore_fw.c:54-232), the instruction scheduler's out-of-order issue
  * (hw/module/instruction_scheduler/rtl/instruction_scheduler.sv) and the BPIP batch
  * former with fill threshold and timeout (pep_sequencer.sv:584-620, bpip registers at
- * hpu_regif_core_cfg_1in3.toml:240-260).  A server thread owns the context stream:
- * while a server is attached, use only luthe_server_* calls on that context (DOp
- * submissions return LUTHE_ERR_BUSY).  Each request runs in an HBM workspace admitted
+ * hpu_regif_core_cfg_1in3.toml:240-260).  A server thread issues its steps on the context
+ * stream.  While a server is attached, DOp/IOp ticket submissions keep working (their groups
+ * are launched on the same stream with scratch of their own, between the server's steps; they
+ * do not join its cohorts); batch calls, graph capture, the autotuner, key uploads and launch
+ * settings (tuning, grid widths) return LUTHE_ERR_BUSY, and luthe_destroy refuses.  Each request runs in an HBM workspace admitted
  * from an arena; each step launches the ready PBS of all admitted requests as one cohort.
  * Results do not depend on which requests share a cohort. */
 typedef struct luthe_server luthe_server;

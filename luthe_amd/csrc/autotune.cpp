@@ -44,11 +44,11 @@ int time_pbs(luthe_ctx* c, const uint64_t* d_in, uint64_t* d_out, int n, int gri
   c->br_grid_override = grid;
   c->br_latency_max = lat_max;
   HIPCHK(c, lk::launch_make_jobs(c->d_jobs, d_in, d_out, nullptr, n, 0, c->prm.lut_slots, (int)c->blen, c->stream));
-  int rc = lx::run_pbs(c, c->d_jobs, n, c->stream);
+  int rc = lx::run_pbs(c, c->bs, c->d_jobs, n, c->stream);
   if (rc) return rc;
   Timer t(c->stream);
   t.start();
-  rc = lx::run_pbs(c, c->d_jobs, n, c->stream);
+  rc = lx::run_pbs(c, c->bs, c->d_jobs, n, c->stream);
   ms = t.stop();
   return rc;
 }
@@ -92,6 +92,7 @@ int luthe_get_tuning(luthe_ctx* c, luthe_tuning* out) {
 
 int luthe_set_tuning(luthe_ctx* c, const luthe_tuning* t) {
   if (!c || !t) return LUTHE_ERR_ARG;
+  if (c->server) return lx::fail(c, LUTHE_ERR_BUSY, "keys and launch settings are fixed while a server is attached");
   int rc = apply(c, *t);
   if (rc) return lx::fail(c, rc, "invalid tuning");
   return LUTHE_OK;
@@ -99,6 +100,7 @@ int luthe_set_tuning(luthe_ctx* c, const luthe_tuning* t) {
 
 int luthe_load_tuning(luthe_ctx* c, const char* path) {
   if (!c || !path) return LUTHE_ERR_ARG;
+  if (c->server) return lx::fail(c, LUTHE_ERR_BUSY, "keys and launch settings are fixed while a server is attached");
   FILE* f = fopen(path, "rb");
   if (!f) return lx::fail(c, LUTHE_ERR_ARG, "cannot open %s", path);
   std::string s;
@@ -134,7 +136,7 @@ int luthe_autotune(luthe_ctx* c, uint32_t batch, const char* json_path, luthe_tu
   const int occ_pair = lk::blind_rotate_occupancy(L, true);
   const int lat_slots = lk::blind_rotate_latency_slots(L);
   int rc = lx::ensure_jobs(c, batch);
-  if (!rc) rc = lx::ensure_lwe2n(c, batch);
+  if (!rc) rc = lx::ensure_lwe2n(c, c->bs, batch);
   if (rc) return rc;
   // synthetic inputs: any u64 values (the kernels' cost does not depend on them)
   uint64_t *d_in = nullptr, *d_out = nullptr, *d_a = nullptr, *d_b = nullptr, *d_o = nullptr;

@@ -5,6 +5,7 @@
 #include <chrono>
 #include <deque>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -30,6 +31,18 @@ struct luthe_graph {
   luthe_ctx* ctx = nullptr;  // cleared when the context is destroyed first
 };
 
+// Per-launch scratch of the PBS path: the key switch's mod-switched LWEs and the partial sums
+// of its split form.  A context keeps two sets: one for batch calls and an attached server's
+// cohorts, one for DOp/IOp ticket groups, so that groups launched by the submitting thread
+// never share scratch with a server thread's cohorts on the same stream.
+struct PbsScratch {
+  uint16_t* lwe2n = nullptr;
+  size_t lwe2n_cap = 0;    // ciphertexts
+  uint32_t* ks_part = nullptr;
+  size_t ks_part_cap = 0;  // words
+  size_t ks_part_for = 0;  // batch sizes up to this are covered
+};
+
 struct luthe_ctx {
   luthe_params prm{};
   int device = 0;
@@ -46,14 +59,11 @@ struct luthe_ctx {
   uint64_t* d_tab = nullptr;  // NTT twiddle tables (ntt2048.h layout)
   bool bsk_ok = false, ksk_ok = false;
   // scratch (grow-only)
-  uint16_t* d_lwe2n = nullptr;
-  size_t lwe2n_cap = 0;
+  PbsScratch bs;  // batch calls and the server's cohorts
+  PbsScratch ds;  // DOp/IOp ticket groups
   int lwe_stride = 0;
   lk::PbsJob* d_jobs = nullptr;
   size_t jobs_cap = 0;
-  uint32_t* d_ks_part = nullptr;  // partial sums of the split key switch (small batches)
-  size_t ks_part_cap = 0;         // words
-  size_t ks_part_for = 0;         // batch sizes up to this are covered
   uint8_t* d_stage = nullptr;  // per-submission descriptors
   size_t stage_cap = 0;
   uint8_t* h_stage = nullptr;  // pinned mirror
@@ -65,6 +75,8 @@ struct luthe_ctx {
   std::vector<std::pair<void*, size_t>> retired;  // (pointer, bytes)
   uint64_t dev_bytes = 0;
   luthe_counters cnt{};
+  mutable std::mutex cnt_mu;  // counters and the error message are also written by a server thread
+  std::mutex err_mu;
   uint64_t next_ticket = 1;
   std::map<uint64_t, Ticket> tickets;
   // DOp/IOp submissions: the open group (streams concatenated in submission order) and the
@@ -100,14 +112,14 @@ int fail(luthe_ctx* c, int code, const char* fmt, ...);
 int dev_alloc(luthe_ctx* c, void** p, size_t bytes);
 // Frees a scratch buffer that is being replaced, or retires it while captured graphs exist.
 void retire_scratch(luthe_ctx* c, void* p, size_t bytes);
-int ensure_lwe2n(luthe_ctx* c, size_t n);
+int ensure_lwe2n(luthe_ctx* c, PbsScratch& sc, size_t n);
 int ensure_jobs(luthe_ctx* c, size_t n);
 // Scratch for the split key switch of batches of up to n ciphertexts.
-int ensure_ks_part(luthe_ctx* c, size_t n);
+int ensure_ks_part(luthe_ctx* c, PbsScratch& sc, size_t n);
 int ensure_rename_rows(luthe_ctx* c, size_t n);
 // Key switch + blind rotation + sample extract of n jobs whose descriptors are on the
 // device (their BLWE addresses inside).
-int run_pbs(luthe_ctx* c, const lk::PbsJob* d_jobs, int n, hipStream_t s);
+int run_pbs(luthe_ctx* c, PbsScratch& sc, const lk::PbsJob* d_jobs, int n, hipStream_t s);
 // The same with the mod-switched LWEs in caller-provided device memory; allocates nothing.
 // ks_part: the split key switch's scratch (NULL: no split).
 int run_pbs_on(luthe_ctx* c, const lk::PbsJob* d_jobs, int n, hipStream_t s, uint16_t* lwe2n, uint32_t* ks_part);

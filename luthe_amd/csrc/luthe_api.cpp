@@ -46,6 +46,7 @@ int fail(luthe_ctx* c, int code, const char* fmt, ...) {
     va_start(ap, fmt);
     vsnprintf(buf, sizeof buf, fmt, ap);
     va_end(ap);
+    std::lock_guard<std::mutex> g(c->err_mu);  // an attached server's thread reports here too
     c->err = buf;
   }
   return code;
@@ -90,18 +91,18 @@ void retire_scratch(luthe_ctx* c, void* p, size_t bytes) {
 
 // Scratch grows by allocating the new buffer first: on failure the context keeps the old
 // buffer and capacity.
-int ensure_lwe2n(luthe_ctx* c, size_t n) {
-  if (n <= c->lwe2n_cap) return LUTHE_OK;
-  const size_t cap = std::max(n, c->lwe2n_cap * 2);
+int ensure_lwe2n(luthe_ctx* c, PbsScratch& sc, size_t n) {
+  if (n <= sc.lwe2n_cap) return LUTHE_OK;
+  const size_t cap = std::max(n, sc.lwe2n_cap * 2);
   uint16_t* nb = nullptr;
   int rc = dev_alloc(c, (void**)&nb, cap * c->lwe_stride * sizeof(uint16_t));
   if (rc) return rc;
-  if (c->d_lwe2n) {
+  if (sc.lwe2n) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    retire_scratch(c, c->d_lwe2n, c->lwe2n_cap * c->lwe_stride * sizeof(uint16_t));
+    retire_scratch(c, sc.lwe2n, sc.lwe2n_cap * c->lwe_stride * sizeof(uint16_t));
   }
-  c->d_lwe2n = nb;
-  c->lwe2n_cap = cap;
+  sc.lwe2n = nb;
+  sc.lwe2n_cap = cap;
   return LUTHE_OK;
 }
 
@@ -145,8 +146,8 @@ static size_t ks_part_words(const luthe_ctx* c, size_t n) {
   return s > 1 ? (size_t)s * n * (nl + 1) : 0;
 }
 
-int ensure_ks_part(luthe_ctx* c, size_t n) {
-  if (n <= c->ks_part_for) return LUTHE_OK;
+int ensure_ks_part(luthe_ctx* c, PbsScratch& sc, size_t n) {
+  if (n <= sc.ks_part_for) return LUTHE_OK;
   // the largest split scratch of any batch size up to n: the split depends on the number of
   // 32-ciphertext tiles only, so the maximum sits at the last size of some tile count
   size_t need = 0;
@@ -155,21 +156,20 @@ int ensure_ks_part(luthe_ctx* c, size_t n) {
     if (w == 0) break;  // more tiles: no split
     need = std::max(need, w);
   }
-  if (need <= c->ks_part_cap) {
-    c->ks_part_for = n;
+  if (need <= sc.ks_part_cap) {
+    sc.ks_part_for = n;
     return LUTHE_OK;
   }
-  if (need <= c->ks_part_cap) return LUTHE_OK;
   uint32_t* nb = nullptr;
   int rc = dev_alloc(c, (void**)&nb, need * sizeof(uint32_t));
   if (rc) return rc;
-  if (c->d_ks_part) {
+  if (sc.ks_part) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    retire_scratch(c, c->d_ks_part, c->ks_part_cap * sizeof(uint32_t));
+    retire_scratch(c, sc.ks_part, sc.ks_part_cap * sizeof(uint32_t));
   }
-  c->d_ks_part = nb;
-  c->ks_part_cap = need;
-  c->ks_part_for = n;
+  sc.ks_part = nb;
+  sc.ks_part_cap = need;
+  sc.ks_part_for = n;
   return LUTHE_OK;
 }
 
@@ -236,20 +236,23 @@ int run_pbs_on(luthe_ctx* c, const PbsJob* d_jobs, int n, hipStream_t s,
   lk::BrArgs ba{(int)p.lwe_dim, c->lwe_stride, p.pbs_base_log};
   HIPCHK(c, lk::launch_blind_rotate(c->d_bsk, c->d_luts, lwe2n, d_jobs, n, c->d_tab, ba,
                                     (int)p.pbs_level, c->br_grid_override, s, latency_max(c), use_pair(c)));
-  c->cnt.pbs += (uint64_t)n;
-  c->cnt.pbs_batches += 1;
-  c->cnt.pbs_max_batch = std::max<uint64_t>(c->cnt.pbs_max_batch, (uint64_t)n);
+  {
+    std::lock_guard<std::mutex> g(c->cnt_mu);  // an attached server's thread counts here too
+    c->cnt.pbs += (uint64_t)n;
+    c->cnt.pbs_batches += 1;
+    c->cnt.pbs_max_batch = std::max<uint64_t>(c->cnt.pbs_max_batch, (uint64_t)n);
+  }
   return LUTHE_OK;
 }
 
 // KS + BR over jobs already on the device, on the context's (grown as needed) scratch.
-int run_pbs(luthe_ctx* c, const PbsJob* d_jobs, int n, hipStream_t s) {
+int run_pbs(luthe_ctx* c, PbsScratch& sc, const PbsJob* d_jobs, int n, hipStream_t s) {
   if (!c->bsk_ok || !c->ksk_ok) return fail(c, LUTHE_ERR_STATE, "keys not uploaded");
   if (n == 0) return LUTHE_OK;
-  int rc = ensure_lwe2n(c, (size_t)n);
-  if (!rc) rc = ensure_ks_part(c, (size_t)n);
+  int rc = ensure_lwe2n(c, sc, (size_t)n);
+  if (!rc) rc = ensure_ks_part(c, sc, (size_t)n);
   if (rc) return rc;
-  return run_pbs_on(c, d_jobs, n, s, c->d_lwe2n, c->d_ks_part);
+  return run_pbs_on(c, d_jobs, n, s, sc.lwe2n, sc.ks_part);
 }
 
 void trace_push(luthe_ctx* c, const luthe_trace_event& e) {
@@ -349,7 +352,8 @@ int luthe_destroy(luthe_ctx* c) {
     if (kv.second.done) (void)hipEventDestroy(kv.second.done);
   for (luthe_graph* g : c->graphs) g->ctx = nullptr;  // their scratch is freed below
   for (auto& r : c->retired) (void)hipFree(r.first);
-  void* bufs[] = {c->d_mem, c->d_luts, c->d_bsk, c->d_ksk, c->d_tab, c->d_lwe2n, c->d_jobs, c->d_stage, c->d_ks_part};
+  void* bufs[] = {c->d_mem,     c->d_luts,      c->d_bsk,   c->d_ksk, c->d_tab,     c->bs.lwe2n,
+                  c->bs.ks_part, c->ds.lwe2n, c->ds.ks_part, c->d_jobs, c->d_stage};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (c->h_stage) (void)hipHostFree(c->h_stage);
@@ -369,6 +373,7 @@ int luthe_get_params(const luthe_ctx* c, luthe_params* out) {
 
 int luthe_get_counters(const luthe_ctx* c, luthe_counters* out) {
   if (!c || !out) return LUTHE_ERR_ARG;
+  std::lock_guard<std::mutex> g(c->cnt_mu);
   *out = c->cnt;
   return LUTHE_OK;
 }
@@ -379,12 +384,14 @@ void* luthe_stream(luthe_ctx* c) { return c ? (void*)c->stream : nullptr; }
 
 int luthe_set_br_grid(luthe_ctx* c, int workgroups) {
   if (!c || workgroups < 0) return LUTHE_ERR_ARG;
+  if (c->server) return fail(c, LUTHE_ERR_BUSY, "keys and launch settings are fixed while a server is attached");
   c->br_grid_override = workgroups;
   return LUTHE_OK;
 }
 
 int luthe_set_br_latency_max(luthe_ctx* c, int max_cohort) {
   if (!c || max_cohort < 0) return LUTHE_ERR_ARG;
+  if (c->server) return fail(c, LUTHE_ERR_BUSY, "keys and launch settings are fixed while a server is attached");
   c->br_latency_max = max_cohort;
   return LUTHE_OK;
 }
@@ -439,6 +446,7 @@ int luthe_sync(luthe_ctx* c) {
 // ---------------------------------------------------------------- keys and memories
 int luthe_upload_bsk(luthe_ctx* c, const uint64_t* bsk, int layout) {
   if (!c || !bsk) return LUTHE_ERR_ARG;
+  if (c->server) return fail(c, LUTHE_ERR_BUSY, "keys and launch settings are fixed while a server is attached");
   if (layout != LUTHE_BSK_COEF && layout != LUTHE_BSK_NTT) return fail(c, LUTHE_ERR_ARG, "bad BSK layout %d", layout);
   const luthe_params& p = c->prm;
   const int K1 = (int)p.glwe_k + 1, rows = K1 * (int)p.pbs_level;
@@ -464,6 +472,7 @@ int luthe_upload_bsk(luthe_ctx* c, const uint64_t* bsk, int layout) {
 
 int luthe_upload_ksk(luthe_ctx* c, const uint32_t* ksk) {
   if (!c || !ksk) return LUTHE_ERR_ARG;
+  if (c->server) return fail(c, LUTHE_ERR_BUSY, "keys and launch settings are fixed while a server is attached");
   const luthe_params& p = c->prm;
   size_t elems = (size_t)p.glwe_k * p.poly_size * p.ks_level * (p.lwe_dim + 1);
   HIPCHK(c, hipSetDevice(c->device));
@@ -517,6 +526,7 @@ int luthe_read_reg(luthe_ctx* c, uint32_t rid, uint64_t* blwe, uint32_t count) {
 // ---------------------------------------------------------------- batched hot path
 int luthe_keyswitch_batch(luthe_ctx* c, const uint64_t* d_blwe, uint16_t* d_lwe2n, size_t n, void* stream) {
   if (!c || (!d_blwe && n) || (!d_lwe2n && n)) return LUTHE_ERR_ARG;
+  if (c->server) return fail(c, LUTHE_ERR_BUSY, "batch scratch is in use by an attached server");
   if (!c->ksk_ok) return fail(c, LUTHE_ERR_STATE, "KSK not uploaded");
   if (n == 0) return LUTHE_OK;
   hipStream_t s = pick_stream(c, stream);
@@ -526,9 +536,9 @@ int luthe_keyswitch_batch(luthe_ctx* c, const uint64_t* d_blwe, uint16_t* d_lwe2
   HIPCHK(c, lk::launch_make_jobs(c->d_jobs, d_blwe, nullptr, nullptr, (int)n, 0, c->prm.lut_slots, (int)c->blen, s));
   const luthe_params& p = c->prm;
   lk::KsArgs ka{(int)(p.glwe_k * p.poly_size), (int)p.lwe_dim, (int)p.lwe_dim + 1, p.ks_base_log, p.ks_mod_log, 12};
-  rc = ensure_ks_part(c, n);
+  rc = ensure_ks_part(c, c->bs, n);
   if (rc) return rc;
-  HIPCHK(c, lk::launch_keyswitch(c->d_jobs, (int)n, c->d_ksk, d_lwe2n, ka, (int)p.ks_level, s, c->d_ks_part,
+  HIPCHK(c, lk::launch_keyswitch(c->d_jobs, (int)n, c->d_ksk, d_lwe2n, ka, (int)p.ks_level, s, c->bs.ks_part,
                                  lk::keyswitch_split((int)n, ka.kN, ka.n_lwe)));
   return LUTHE_OK;
 }
@@ -536,6 +546,7 @@ int luthe_keyswitch_batch(luthe_ctx* c, const uint64_t* d_blwe, uint16_t* d_lwe2
 int luthe_blind_rotate_batch(luthe_ctx* c, const uint16_t* d_lwe2n, uint64_t* d_out, const uint32_t* d_gid,
                              uint32_t log_lut_nb, size_t n, void* stream) {
   if (!c || (n && (!d_lwe2n || !d_out || !d_gid))) return LUTHE_ERR_ARG;
+  if (c->server) return fail(c, LUTHE_ERR_BUSY, "batch scratch is in use by an attached server");
   if (!c->bsk_ok) return fail(c, LUTHE_ERR_STATE, "BSK not uploaded");
   if (log_lut_nb > 3) return fail(c, LUTHE_ERR_ARG, "log_lut_nb must be <= 3");
   if (n == 0) return LUTHE_OK;
@@ -555,6 +566,7 @@ int luthe_blind_rotate_batch(luthe_ctx* c, const uint16_t* d_lwe2n, uint64_t* d_
 int luthe_pbs_batch(luthe_ctx* c, const uint64_t* d_in, uint64_t* d_out, const uint32_t* d_gid, uint32_t log_lut_nb,
                     size_t n, void* stream) {
   if (!c || (n && (!d_in || !d_out || !d_gid))) return LUTHE_ERR_ARG;
+  if (c->server) return fail(c, LUTHE_ERR_BUSY, "batch scratch is in use by an attached server");
   if (log_lut_nb > 3) return fail(c, LUTHE_ERR_ARG, "log_lut_nb must be <= 3");
   if (n == 0) return LUTHE_OK;
   if (n > (size_t)1 << 28) return fail(c, LUTHE_ERR_ARG, "batch too large");
@@ -563,24 +575,25 @@ int luthe_pbs_batch(luthe_ctx* c, const uint64_t* d_in, uint64_t* d_out, const u
   int rc = ensure_jobs(c, n);
   if (rc) return rc;
   HIPCHK(c, lk::launch_make_jobs(c->d_jobs, d_in, d_out, d_gid, (int)n, log_lut_nb, c->prm.lut_slots, (int)c->blen, s));
-  return run_pbs(c, c->d_jobs, (int)n, s);
+  return run_pbs(c, c->bs, c->d_jobs, (int)n, s);
 }
 
 int luthe_pbs_batch_list(luthe_ctx* c, const luthe_pbs_batch_desc* d_descs, uint32_t count, size_t total,
                          void* stream) {
   static_assert(sizeof(luthe_pbs_batch_desc) == sizeof(lk::PbsBatchDesc), "descriptor layouts agree");
   if (!c || (count && !d_descs)) return LUTHE_ERR_ARG;
+  if (c->server) return fail(c, LUTHE_ERR_BUSY, "batch scratch is in use by an attached server");
   if (count == 0 || total == 0) return LUTHE_OK;
   if (total > (size_t)1 << 28 || count > (1u << 20)) return fail(c, LUTHE_ERR_ARG, "batch list too large");
   if (!c->bsk_ok || !c->ksk_ok) return fail(c, LUTHE_ERR_STATE, "keys not uploaded");
   hipStream_t s = pick_stream(c, stream);
   HIPCHK(c, hipSetDevice(c->device));
   int rc = ensure_jobs(c, total);
-  if (!rc) rc = ensure_lwe2n(c, total);
+  if (!rc) rc = ensure_lwe2n(c, c->bs, total);
   if (rc) return rc;
   HIPCHK(c, lk::launch_make_jobs_list(c->d_jobs, (const lk::PbsBatchDesc*)d_descs, (int)count, (int)total,
                                       c->prm.lut_slots, (int)c->blen, s));
-  return run_pbs(c, c->d_jobs, (int)total, s);
+  return run_pbs(c, c->bs, c->d_jobs, (int)total, s);
 }
 
 int luthe_alu_batch(luthe_ctx* c, uint32_t opcode, const uint64_t* d_a, const uint64_t* d_b, uint64_t* d_out,
@@ -599,11 +612,12 @@ int luthe_alu_batch(luthe_ctx* c, uint32_t opcode, const uint64_t* d_a, const ui
 
 int luthe_reserve(luthe_ctx* c, size_t max_batch) {
   if (!c) return LUTHE_ERR_ARG;
+  if (c->server) return fail(c, LUTHE_ERR_BUSY, "batch scratch is in use by an attached server");
   HIPCHK(c, hipSetDevice(c->device));
   int rc = ensure_jobs(c, max_batch);
-  if (!rc) rc = ensure_ks_part(c, max_batch);
+  if (!rc) rc = ensure_ks_part(c, c->bs, max_batch);
   if (rc) return rc;
-  return ensure_lwe2n(c, max_batch);
+  return ensure_lwe2n(c, c->bs, max_batch);
 }
 
 int luthe_capture_begin(luthe_ctx* c) {
@@ -790,7 +804,7 @@ static int execute_group(luthe_ctx* c, const std::vector<dop::Dop>& prog, const 
       HIPCHK(c, lk::launch_alu(c->d_mem, (const AluOp*)(c->d_stage + lp.alu_off), lp.nalu, blen, useful_bits(c),
                                c->stream));
     if (lp.npbs) {
-      rc = run_pbs(c, (const PbsJob*)(c->d_stage + lp.pbs_off), lp.npbs, c->stream);
+      rc = run_pbs(c, c->ds, (const PbsJob*)(c->d_stage + lp.pbs_off), lp.npbs, c->stream);
       if (rc) return rc;
     }
     c->cnt.alu += (uint64_t)lp.nalu;
@@ -945,7 +959,6 @@ extern "C" {
 
 int luthe_submit_dops(luthe_ctx* c, const uint32_t* words, size_t n, uint64_t* ticket) {
   if (!c || (!words && n)) return LUTHE_ERR_ARG;
-  if (c->server) return fail(c, LUTHE_ERR_BUSY, "context stream is owned by an attached server");
   std::vector<dop::Dop> prog;
   std::string why;
   int rc = dop::decode_stream(words, n, c->prm.regfile_size, c->prm.ct_slots, c->prm.lut_slots, prog, why);
@@ -1093,7 +1106,6 @@ int luthe_register_iop(luthe_ctx* c, uint8_t opcode, uint8_t max_align, const ui
 
 int luthe_submit_iop(luthe_ctx* c, const uint32_t* words, size_t n, uint64_t* ticket) {
   if (!c || !words || n == 0) return LUTHE_ERR_ARG;
-  if (c->server) return fail(c, LUTHE_ERR_BUSY, "context stream is owned by an attached server");
   dop::IopBundle iop;
   std::string why;
   int rc = dop::parse_iop(words, n, iop, why);

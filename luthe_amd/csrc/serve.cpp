@@ -392,7 +392,7 @@ int issue_step(luthe_server* s, std::vector<Request*>& admitted) {
     nalu += pre_off[i].second;
   }
   if (!jobs.empty()) {
-    rc = lx::run_pbs(c, (const PbsJob*)(sl.dev + jobs_off), (int)jobs.size(), st);
+    rc = lx::run_pbs(c, c->bs, (const PbsJob*)(sl.dev + jobs_off), (int)jobs.size(), st);
     if (rc) return sfail(s, rc, "cohort: %s", c->err.c_str());
   }
   for (size_t i = 0; i < post.size(); i++) {
@@ -599,6 +599,7 @@ int luthe_server_create(luthe_ctx* c, const luthe_server_config* cfg_in, luthe_s
   if (cfg.cohort_min == 0) cfg.cohort_min = 1;
   if (cfg.queue_max == 0) cfg.queue_max = 65536;
   HIPCHK(c, hipSetDevice(c->device));
+  (void)lx::latency_max(c);  // resolved here, read only by the server thread
   if (cfg.cohort_max == 0) cfg.cohort_max = c->cohort_max > 0 ? (uint32_t)c->cohort_max : (uint32_t)luthe_get_br_grid(c);
   if (cfg.cohort_max > (1u << 20)) return lx::fail(c, LUTHE_ERR_ARG, "cohort_max must be <= 2^20");
   if (cfg.policy > LUTHE_COHORT_SHORTEST) return lx::fail(c, LUTHE_ERR_ARG, "unknown cohort policy %u", cfg.policy);
@@ -626,7 +627,9 @@ int luthe_server_create(luthe_ctx* c, const luthe_server_config* cfg_in, luthe_s
     if (rc) return rc;
   }
   int rc = lx::ensure_jobs(c, 1);
-  if (!rc) rc = lx::ensure_lwe2n(c, cfg.cohort_max);
+  // the cohorts' scratch is sized once here: the server thread never grows context buffers
+  if (!rc) rc = lx::ensure_lwe2n(c, c->bs, cfg.cohort_max);
+  if (!rc) rc = lx::ensure_ks_part(c, c->bs, cfg.cohort_max);
   if (rc) return rc;
   rc = lx::dev_alloc(c, (void**)&s->d_arena, arena_rows * s->row_bytes);
   if (rc) return rc;

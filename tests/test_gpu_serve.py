@@ -206,8 +206,11 @@ def test_server_errors(full):
         with pytest.raises(la.LutheError) as e:
             la.Server(ctx)
         assert e.value.code == -7
-        with pytest.raises(la.LutheError) as e:
-            ctx.submit_dops(np.zeros(1, np.uint32))
+        with pytest.raises(la.LutheError) as e:  # batch calls would share the cohorts' scratch
+            ctx.pbs_batch(ca[:1], np.zeros(1, np.uint32))
+        assert e.value.code == -10
+        with pytest.raises(la.LutheError) as e:  # the server thread reads the keys and launch settings
+            ctx.upload_ksk(keys.ksk)
         assert e.value.code == -10
         with pytest.raises(la.LutheError) as e:
             ctx.close()
@@ -258,3 +261,45 @@ def test_server_short_request_overtakes_long_ones(full):
     assert np.array_equal(add_s, add_o)
     for p, q in zip(muls_s, muls_o):
         assert np.array_equal(p, q)
+
+
+def test_dop_tickets_beside_a_server(full):
+    """DOp/IOp tickets submitted while a server is attached run on the same stream, between its
+    steps, with scratch of their own: a registered-IOp ticket and a raw DOp stream give the same
+    ciphertexts as with no server, while radix requests keep flowing through the server."""
+    prm, ctx, keys = full
+    from luthe_amd import dop
+    rng = np.random.default_rng(14)
+    lut_base = prm.lut_slots - la.IOP_LUTS
+    ctx.write_iop_luts(lut_base)
+    ctx.register_iop_program(la.IOP_ADD, 8, lut_base)
+    _, st = la.iop_program_dops(la.IOP_ADD, 8, prm.regfile_size, lut_base)
+    a, b = (int(x) for x in rng.integers(0, 1 << 16, 2))
+    ctx.write_ct(200, keys.encrypt_msg(la.to_blocks(a, 8), seed=900))
+    ctx.write_ct(240, keys.encrypt_msg(la.to_blocks(b, 8), seed=901))
+    iop = dop.iop_words(st["iop_opcode"], [(280, 8)], [(200, 8), (240, 8)], align=(st["align_bits"], st["align_bits"]))
+    raw = [dop.ld(1, 200), dop.pbs(2, 1, lut_base), dop.st(300, 2)]
+    print("case: IOp and DOp tickets without a server", flush=True)
+    ctx.wait(ctx.submit_iop(iop))
+    ctx.wait(ctx.submit_dops(raw))
+    exp_iop, exp_raw = ctx.read_ct(280, 8), ctx.read_ct(300, 1)
+    ctx.write_ct(280, np.zeros((8, prm.blen), np.uint64))
+    ctx.write_ct(300, np.zeros((1, prm.blen), np.uint64))
+    jobs = []
+    for k in range(6):
+        x, y = (int(v) for v in rng.integers(0, 1 << 32, 2))
+        jobs.append((x, y, keys.encrypt_msg(la.to_blocks(x, 16), seed=910 + k),
+                     keys.encrypt_msg(la.to_blocks(y, 16), seed=920 + k)))
+    with la.Server(ctx) as srv:
+        print("case: the same tickets while six MUL16 requests run through the server", flush=True)
+        ids = [srv.submit(la.IOP_MUL, cx, cy) for _, _, cx, cy in jobs]
+        t1 = ctx.submit_iop(iop)
+        t2 = ctx.submit_dops(raw)
+        ctx.wait(t1)
+        ctx.wait(t2)
+        res = [srv.wait(i, timeout_us=120_000_000) for i in ids]
+    assert np.array_equal(ctx.read_ct(280, 8), exp_iop)
+    assert np.array_equal(ctx.read_ct(300, 1), exp_raw)
+    assert la.from_blocks(keys.decrypt_msg(exp_iop)) == (a + b) & 0xFFFF
+    for (x, y, _, _), (out, _) in zip(jobs, res):
+        assert la.from_blocks(keys.decrypt_msg(out)) == (x * y) & 0xFFFFFFFF
