@@ -331,6 +331,8 @@ typedef struct luthe_tuning {
   int32_t br_pair;          /* throughput kernel: 0 = 256 threads per ciphertext (up to 4 per CU),
                                1 = 512 threads per ciphertext (2 per CU), -1 = default (1 at PBS
                                level 2, 0 at level 1) */
+  int32_t br_split_max;     /* largest cohort for the split variant (one 256-thread workgroup per
+                               GLWE polynomial, two CUs per ciphertext), -1 = CUs / 2, 0 = off */
 } luthe_tuning;
 int luthe_autotune(luthe_ctx* ctx, uint32_t batch, const char* json_path, luthe_tuning* out);
 int luthe_load_tuning(luthe_ctx* ctx, const char* json_path);

@@ -1,8 +1,8 @@
 // Autotuner: times candidate launch shapes on the device with the context's keys and
 // picks (1) the persistent blind-rotation width, i.e. ciphertexts resident per CU,
-// (2) the largest cohort that runs faster on the 512-thread latency variant, (3) the
-// polynomials per NTT workgroup and (4) the serving cohort size (one wave of the chosen
-// width).  Results are applied to the context and persisted as JSON; luthe_load_tuning
+// (2) the largest cohorts that run faster on the split variant (two workgroups per
+// ciphertext) and on the 512-thread latency variant, (3) the polynomials per NTT workgroup
+// and (4) the serving cohort size (one wave of the chosen width).  Results are applied to the context and persisted as JSON; luthe_load_tuning
 // re-applies a saved file.  Inputs are synthetic: timing does not depend on values.
 #include <hip/hip_runtime.h>
 #include <stdio.h>
@@ -40,9 +40,11 @@ struct Timer {
 };
 
 // one warm-up and one timed PBS batch of n ciphertexts with the given kernel settings
-int time_pbs(luthe_ctx* c, const uint64_t* d_in, uint64_t* d_out, int n, int grid, int lat_max, float& ms) {
+int time_pbs(luthe_ctx* c, const uint64_t* d_in, uint64_t* d_out, int n, int grid, int lat_max, float& ms,
+             int split_max = 0) {
   c->br_grid_override = grid;
   c->br_latency_max = lat_max;
+  c->br_split_max = split_max;
   HIPCHK(c, lk::launch_make_jobs(c->d_jobs, d_in, d_out, nullptr, n, 0, c->prm.lut_slots, (int)c->blen, c->stream));
   int rc = lx::run_pbs(c, c->bs, c->d_jobs, n, c->stream);
   if (rc) return rc;
@@ -65,7 +67,8 @@ bool json_int(const std::string& s, const char* key, long& v) {
 }
 
 int apply(luthe_ctx* c, const luthe_tuning& t) {
-  if (t.br_grid < 0 || t.br_latency_max < -1 || t.cohort_max < 0 || t.br_pair < -1 || t.br_pair > 1)
+  if (t.br_grid < 0 || t.br_latency_max < -1 || t.cohort_max < 0 || t.br_pair < -1 || t.br_pair > 1 ||
+      t.br_split_max < -1)
     return LUTHE_ERR_ARG;
   if (t.ntt_polys_per_wg != 1 && t.ntt_polys_per_wg != 2 && t.ntt_polys_per_wg != 4) return LUTHE_ERR_ARG;
   c->br_grid_override = t.br_grid;
@@ -73,6 +76,7 @@ int apply(luthe_ctx* c, const luthe_tuning& t) {
   c->ntt_polys_per_wg = t.ntt_polys_per_wg;
   c->cohort_max = t.cohort_max;
   c->br_pair = t.br_pair;
+  c->br_split_max = t.br_split_max;
   return LUTHE_OK;
 }
 
@@ -87,6 +91,7 @@ int luthe_get_tuning(luthe_ctx* c, luthe_tuning* out) {
   out->ntt_polys_per_wg = c->ntt_polys_per_wg;
   out->cohort_max = c->cohort_max;
   out->br_pair = c->br_pair;
+  out->br_split_max = lx::split_max(c);
   return LUTHE_OK;
 }
 
@@ -119,6 +124,7 @@ int luthe_load_tuning(luthe_ctx* c, const char* path) {
   t.ntt_polys_per_wg = json_int(s, "ntt_polys_per_wg", v) ? (int32_t)v : 2;
   t.cohort_max = json_int(s, "cohort_max", v) ? (int32_t)v : 0;
   t.br_pair = json_int(s, "br_pair", v) ? (int32_t)v : -1;
+  t.br_split_max = json_int(s, "br_split_max", v) ? (int32_t)v : -1;
   int rc = apply(c, t);
   if (rc) return lx::fail(c, rc, "%s: invalid tuning", path);
   return LUTHE_OK;
@@ -135,6 +141,7 @@ int luthe_autotune(luthe_ctx* c, uint32_t batch, const char* json_path, luthe_tu
   const int occ = lk::blind_rotate_occupancy(L, false), per_cu = occ / (cus > 0 ? cus : 1);
   const int occ_pair = lk::blind_rotate_occupancy(L, true);
   const int lat_slots = lk::blind_rotate_latency_slots(L);
+  const int split_slots = lk::blind_rotate_split_slots();
   int rc = lx::ensure_jobs(c, batch);
   if (!rc) rc = lx::ensure_lwe2n(c, c->bs, batch);
   if (rc) return rc;
@@ -156,7 +163,8 @@ int luthe_autotune(luthe_ctx* c, uint32_t batch, const char* json_path, luthe_tu
   (void)hipMemsetAsync(d_in, 0x5A, rows * c->blen * 8, c->stream);
   (void)hipMemsetAsync(d_a, 0x31, npoly * 2048 * 8, c->stream);
   (void)hipMemsetAsync(d_b, 0x17, npoly * 2048 * 8, c->stream);
-  const luthe_tuning saved{c->br_grid_override, c->br_latency_max, c->ntt_polys_per_wg, c->cohort_max, c->br_pair};
+  const luthe_tuning saved{c->br_grid_override, c->br_latency_max, c->ntt_polys_per_wg, c->cohort_max, c->br_pair,
+                           c->br_split_max};
   c->br_pair = 0;  // (1) sweeps the 256-thread kernel
   std::string meas;
   char line[256];
@@ -187,19 +195,23 @@ int luthe_autotune(luthe_ctx* c, uint32_t batch, const char* json_path, luthe_tu
     c->br_pair = best_pair;
   }
   const int width = best_pair ? occ_pair : best_grid;
-  // (2) latency variant: largest cohort where it beats the throughput kernel
+  // (2) split and latency variants: the largest cohorts where each beats the kernels of
+  // larger cohorts (split: both others; latency: the throughput kernel)
   std::string lat;
-  int lat_max = 0;
-  for (int n : {cus / 4, cus / 2, cus, (cus * 3) / 2, 2 * cus, 3 * cus}) {
-    if (rc || n < 1 || n > lat_slots || n > (int)batch) continue;
-    float t_lat = 0, t_thr = 0;
-    rc = time_pbs(c, d_in, d_out, n, best_pair ? 0 : best_grid, n, t_lat);
-    if (!rc) rc = time_pbs(c, d_in, d_out, n, best_pair ? 0 : best_grid, 0, t_thr);
+  int lat_max = 0, split_best = 0;
+  const int grid_thr = best_pair ? 0 : best_grid;
+  for (int n : {1, cus / 8, cus / 4, cus / 2, cus, (cus * 3) / 2, 2 * cus, 3 * cus}) {
+    if (rc || n < 1 || n > (int)batch || (n > lat_slots && n > split_slots)) continue;
+    float t_split = 0, t_lat = 0, t_thr = 0;
+    if (n <= split_slots) rc = time_pbs(c, d_in, d_out, n, grid_thr, 0, t_split, n);
+    if (!rc && n <= lat_slots) rc = time_pbs(c, d_in, d_out, n, grid_thr, n, t_lat);
+    if (!rc) rc = time_pbs(c, d_in, d_out, n, grid_thr, 0, t_thr);
     if (rc) break;
-    snprintf(line, sizeof line, "%s{\"n\": %d, \"latency_ms\": %.3f, \"throughput_ms\": %.3f}", lat.empty() ? "" : ", ",
-             n, t_lat, t_thr);
+    snprintf(line, sizeof line, "%s{\"n\": %d, \"split_ms\": %.3f, \"latency_ms\": %.3f, \"throughput_ms\": %.3f}",
+             lat.empty() ? "" : ", ", n, t_split, t_lat, t_thr);
     lat += line;
-    if (t_lat < t_thr) lat_max = n;
+    if (n <= lat_slots && t_lat < t_thr) lat_max = n;
+    if (n <= split_slots && t_split < t_thr && (n > lat_slots || t_split < t_lat)) split_best = n;
   }
   // (3) polynomials per NTT workgroup (forward, pointwise, inverse over 4096 polynomials)
   std::string ntt;
@@ -226,7 +238,7 @@ int luthe_autotune(luthe_ctx* c, uint32_t batch, const char* json_path, luthe_tu
     (void)apply(c, saved);
     return rc;
   }
-  luthe_tuning t{best_pair || best_grid == occ ? 0 : best_grid, lat_max, best_np, width, best_pair};
+  luthe_tuning t{best_pair || best_grid == occ ? 0 : best_grid, lat_max, best_np, width, best_pair, split_best};
   (void)apply(c, t);
   if (out) *out = t;
   if (json_path) {
@@ -236,10 +248,12 @@ int luthe_autotune(luthe_ctx* c, uint32_t batch, const char* json_path, luthe_tu
     (void)hipGetDeviceProperties(&prop, c->device);
     fprintf(f,
             "{\n  \"device\": \"%s\", \"cus\": %d, \"lwe_dim\": %u, \"pbs_level\": %d, \"batch\": %u,\n"
-            "  \"br_grid\": %d, \"br_pair\": %d, \"br_latency_max\": %d, \"ntt_polys_per_wg\": %d, \"cohort_max\": %d,\n"
+            "  \"br_grid\": %d, \"br_pair\": %d, \"br_split_max\": %d, \"br_latency_max\": %d, \"ntt_polys_per_wg\": %d,\n"
+            "  \"cohort_max\": %d,\n"
             "  \"measured\": {\n    \"blind_rotation_width\": [%s],\n    \"latency_variant\": [%s],\n"
             "    \"ntt_polymul_4096\": [%s]\n  }\n}\n",
-            prop.gcnArchName, cus, c->prm.lwe_dim, L, batch, t.br_grid, t.br_pair, t.br_latency_max, t.ntt_polys_per_wg,
+            prop.gcnArchName, cus, c->prm.lwe_dim, L, batch, t.br_grid, t.br_pair, t.br_split_max, t.br_latency_max,
+            t.ntt_polys_per_wg,
             t.cohort_max, meas.c_str(), lat.c_str(), ntt.c_str());
     fclose(f);
   }

@@ -41,6 +41,8 @@ struct PbsScratch {
   uint32_t* ks_part = nullptr;
   size_t ks_part_cap = 0;  // words
   size_t ks_part_for = 0;  // batch sizes up to this are covered
+  uint64_t* xch = nullptr;  // the split blind rotation's exchange area
+  size_t xch_cap = 0;       // ciphertexts
 };
 
 struct luthe_ctx {
@@ -75,6 +77,9 @@ struct luthe_ctx {
   std::vector<std::pair<void*, size_t>> retired;  // (pointer, bytes)
   uint64_t dev_bytes = 0;
   luthe_counters cnt{};
+  int br_split_max = -1;        // split blind rotation up to this many ciphertexts (-1: CUs / 2)
+  uint32_t* h_status = nullptr;  // pinned: set by a split pair whose exchange timed out
+  uint32_t* d_status = nullptr;  // its device address
   mutable std::mutex cnt_mu;  // counters and the error message are also written by a server thread
   std::mutex err_mu;
   uint64_t next_ticket = 1;
@@ -116,13 +121,23 @@ int ensure_lwe2n(luthe_ctx* c, PbsScratch& sc, size_t n);
 int ensure_jobs(luthe_ctx* c, size_t n);
 // Scratch for the split key switch of batches of up to n ciphertexts.
 int ensure_ks_part(luthe_ctx* c, PbsScratch& sc, size_t n);
+// Exchange area of the split blind rotation for cohorts of up to n ciphertexts (capped at the
+// split threshold; nothing for larger n).
+int ensure_xch(luthe_ctx* c, PbsScratch& sc, size_t n);
+int split_max(luthe_ctx* c);
+// After a synchronisation: LUTHE_ERR_HIP (once) if a split blind rotation's pair exchange
+// timed out since the last check.
+int check_device_status(luthe_ctx* c);
+// A copy of the context's last error message (taken under its lock).
+std::string last_error(luthe_ctx* c);
 int ensure_rename_rows(luthe_ctx* c, size_t n);
 // Key switch + blind rotation + sample extract of n jobs whose descriptors are on the
 // device (their BLWE addresses inside).
 int run_pbs(luthe_ctx* c, PbsScratch& sc, const lk::PbsJob* d_jobs, int n, hipStream_t s);
 // The same with the mod-switched LWEs in caller-provided device memory; allocates nothing.
 // ks_part: the split key switch's scratch (NULL: no split).
-int run_pbs_on(luthe_ctx* c, const lk::PbsJob* d_jobs, int n, hipStream_t s, uint16_t* lwe2n, uint32_t* ks_part);
+int run_pbs_on(luthe_ctx* c, const lk::PbsJob* d_jobs, int n, hipStream_t s, uint16_t* lwe2n, uint32_t* ks_part,
+               uint64_t* xch = nullptr);
 // Looks up the no-allocation entry point's device scratch on `device` (current).
 int noalloc_init(luthe_ctx* c, int device);
 // DOp groups: seal (launch) the open group now; check the sealing rules (force: seal).

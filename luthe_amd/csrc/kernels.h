@@ -67,7 +67,14 @@ int keyswitch_split(int nct, int kN, int n_lwe);
 // the 256-thread persistent kernel, or with `pair` the 512-thread two-per-CU variant.
 hipError_t launch_blind_rotate(const uint64_t* bsk, const uint64_t* luts, const uint16_t* lwe2n, const PbsJob* jobs,
                                int nct, const uint64_t* tab, const BrArgs& a,
-                               int pbs_level, int grid_override, hipStream_t s, int latency_max, bool pair);
+                               int pbs_level, int grid_override, hipStream_t s, int latency_max, bool pair,
+                               int split_max = 0, uint64_t* xch = nullptr, uint32_t* status = nullptr);
+// Split blind rotation (two workgroups per ciphertext) for cohorts of at most split_max
+// ciphertexts when xch (blind_rotate_split_words(pbs_level) u64 per ciphertext) and status
+// (a word the host can read; set to 1 if a pair's exchange timed out) are given.
+size_t blind_rotate_split_words(int pbs_level);
+// Ciphertexts the split variant runs with one workgroup per CU (CUs / 2).
+int blind_rotate_split_slots();
 // Resident workgroups of the throughput kernel (CUs x workgroups per CU).
 int blind_rotate_occupancy(int pbs_level, bool pair);
 // Ciphertexts the latency variant holds resident at once (its workgroups per CU x CUs).

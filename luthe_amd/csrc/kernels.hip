@@ -9,6 +9,8 @@
 //  k_alu, k_copy  DOp ALU and LD/ST on BLWE rows
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "kernels.h"
 #include "ntt2048.h"
 #include "pbs_core.h"
@@ -22,14 +24,76 @@ constexpr int MAX_GRID_Y = 65535;  // launches split their per-op grid dimension
 // ------------------------------------------------------------------------------------
 // NTT kernels: NP polynomials per 256-thread workgroup.
 // ------------------------------------------------------------------------------------
-// tab: twiddle tables (ntt2048.h TABLE_WORDS layout).
+// Twiddle sources of the passes.  TabTw reads the tables (ntt2048.h TABLE_WORDS layout,
+// L1/L2-resident) at each use.  RegTw holds one thread's twiddles of the A2 pass and of the
+// one-polynomial C passes in registers, loaded once per kernel: a kernel with one wave per
+// SIMD cannot hide table loads inside a chain of dependent passes.  Both give the same
+// values, so results do not depend on the source.
+struct TabTw {
+  const uint64_t* tab;
+  __device__ __forceinline__ void a2(uint64_t* v, int r, int q) const { fwd_a2_compute(v, r, q, tab + TW_OFF); }
+  template <int NP>
+  __device__ __forceinline__ void c1(uint64_t (*v)[8], int t) const {
+    fwd_c1_compute<NP>(v, t, tab + C64_OFF);
+  }
+  template <int NP>
+  __device__ __forceinline__ void ic1(uint64_t (*v)[8], int t) const {
+    inv_c1_compute<NP>(v, t, tab + ITW_OFF, tab + IC64_OFF);
+  }
+};
+
+struct RegTw {
+  uint64_t a2w[8];   // tw[(2q + h + 8 k0) * 64 + r] at h * 4 + k0 (A2 map, q = t >> 6)
+  uint64_t c1w[8];   // c64[8 w1 + r0] (one-polynomial C1 map; [0] = 1)
+  uint64_t ic1w[8];  // ic64[8 w1 + r0] ([0] = 1)
+  uint64_t itw[8];   // itw[(r0 + 8 r1) * 32 + k]
+  __device__ __forceinline__ void load(const uint64_t* tab, int t) {
+    const int r = t & 63, q = t >> 6, k = t & 31;
+    const int r0 = c1_r0<1>(0, t >> 6, (t >> 5) & 1);
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+      for (int k0 = 0; k0 < 4; k0++) a2w[h * 4 + k0] = tab[TW_OFF + (2 * q + h + 8 * k0) * 64 + r];
+    c1w[0] = ic1w[0] = 1;
+#pragma unroll
+    for (int w1 = 1; w1 < 8; w1++) {
+      c1w[w1] = tab[C64_OFF + 8 * w1 + r0];
+      ic1w[w1] = tab[IC64_OFF + 8 * w1 + r0];
+    }
+#pragma unroll
+    for (int r1 = 0; r1 < 8; r1++) itw[r1] = tab[ITW_OFF + (r0 + 8 * r1) * 32 + k];
+  }
+  __device__ __forceinline__ void a2(uint64_t* v, int, int) const {
+    dft4x2<48>(v);
+    gl::mul4(v, v, a2w);
+    gl::mul4(v + 4, v + 4, a2w + 4);
+  }
+  template <int NP>
+  __device__ __forceinline__ void c1(uint64_t (*v)[8], int t) const {
+    if constexpr (NP == 1) {
+      dft8<24>(v[0]);
+      gl::mul4(v[0], v[0], c1w);
+      gl::mul4(v[0] + 4, v[0] + 4, c1w + 4);
+    } else {
+      fwd_c1_compute<NP>(v, t, nullptr);  // compile-time twists, no table
+    }
+  }
+  template <int NP>
+  __device__ __forceinline__ void ic1(uint64_t (*v)[8], int) const {
+    static_assert(NP == 1, "register twiddles cover the one-polynomial inverse");
+    gl::mul4(v[0], v[0], ic1w);
+    gl::mul4(v[0] + 4, v[0] + 4, ic1w + 4);
+    idft8(v[0]);
+    gl::mul4(v[0], v[0], itw);
+    gl::mul4(v[0] + 4, v[0] + 4, itw + 4);
+  }
+};
+
 // PRE: the inputs are already pretwisted by psi^(4 m1) (blind-rotation digits, HI path).
-template <int NP, bool PRE = false>
-__device__ __forceinline__ void fwd_passes(uint64_t (*v)[8], uint64_t* lds, const uint64_t* tab, int t) {
+template <int NP, bool PRE, class TW>
+__device__ __forceinline__ void fwd_passes_tw(uint64_t (*v)[8], uint64_t* lds, const TW& tw, int t) {
   const int r = t & 63, m0 = t >> 6, k = t >> 3, j = t & 7;
   const int m0u = wave_uniform(m0);
-  const uint64_t* tw = tab + TW_OFF;
-  const uint64_t* c64 = tab + C64_OFF;
 #pragma unroll
   for (int p = 0; p < NP; p++) fwd_a1_compute(v[p], m0u, PRE);
 #pragma unroll
@@ -38,14 +102,14 @@ __device__ __forceinline__ void fwd_passes(uint64_t (*v)[8], uint64_t* lds, cons
 #pragma unroll
   for (int p = 0; p < NP; p++) {
     fwd_a2_load(v[p], lds + p * N, r, m0);
-    fwd_a2_compute(v[p], r, m0u, tw);
+    tw.a2(v[p], r, m0u);
   }
   __syncthreads();
 #pragma unroll
   for (int p = 0; p < NP; p++) fwd_a2_store(v[p], lds + p * N, r, m0);
   __syncthreads();
   fwd_c1_load<NP>(v, lds, t);
-  fwd_c1_compute<NP>(v, t, c64);
+  tw.template c1<NP>(v, t);
   __syncthreads();
   fwd_c1_store<NP>(v, lds, t);
   __syncthreads();
@@ -55,18 +119,20 @@ __device__ __forceinline__ void fwd_passes(uint64_t (*v)[8], uint64_t* lds, cons
     fwd_c2_compute(v[p]);
   }
 }
+template <int NP, bool PRE = false>
+__device__ __forceinline__ void fwd_passes(uint64_t (*v)[8], uint64_t* lds, const uint64_t* tab, int t) {
+  fwd_passes_tw<NP, PRE>(v, lds, TabTw{tab}, t);
+}
 
 // Entry: v in NTT domain (thread (k, w1) layout), LDS free for writing only after a
 // barrier that orders the previous readers.  Exit: v[p][m1] canonical coefficients.
-template <int NP>
-__device__ __forceinline__ void inv_passes(uint64_t (*v)[8], uint64_t* lds, const uint64_t* tab, int t) {
+template <int NP, class TW>
+__device__ __forceinline__ void inv_passes_tw(uint64_t (*v)[8], uint64_t* lds, const TW& tw, int t) {
   // Opaque copy of the thread id: the index values below are derived here rather than
   // kept live across the caller's code (which would spill at 128 VGPRs).
   asm volatile("" : "+v"(t));
   const int r = t & 63, m0 = t >> 6, k = t >> 3, j = t & 7;
   const int m0u = wave_uniform(m0);
-  const uint64_t* itw = tab + ITW_OFF;
-  const uint64_t* ic64 = tab + IC64_OFF;
 #pragma unroll
   for (int p = 0; p < NP; p++) inv_c2_compute(v[p]);
   __syncthreads();
@@ -74,7 +140,7 @@ __device__ __forceinline__ void inv_passes(uint64_t (*v)[8], uint64_t* lds, cons
   for (int p = 0; p < NP; p++) inv_c2_store(v[p], lds + p * N, k, j);
   __syncthreads();
   inv_c1_load<NP>(v, lds, t);
-  inv_c1_compute<NP>(v, t, itw, ic64);
+  tw.template ic1<NP>(v, t);
   __syncthreads();
   inv_c1_store<NP>(v, lds, t);
   __syncthreads();
@@ -92,6 +158,10 @@ __device__ __forceinline__ void inv_passes(uint64_t (*v)[8], uint64_t* lds, cons
     inv_a1_load(v[p], lds + p * N, r, m0);
     inv_a1_compute(v[p]);
   }
+}
+template <int NP>
+__device__ __forceinline__ void inv_passes(uint64_t (*v)[8], uint64_t* lds, const uint64_t* tab, int t) {
+  inv_passes_tw<NP>(v, lds, TabTw{tab}, t);
 }
 
 // mode 0: forward (out NTT domain), 1: inverse (in NTT domain), 2: polymul
@@ -383,6 +453,190 @@ __global__ __launch_bounds__(512, TP ? 2 : 1) void k_blind_rotate_lat(const uint
 }
 
 // ------------------------------------------------------------------------------------
+// Split variant of the blind rotation for the smallest cohorts: ciphertext c runs on a pair
+// of 256-thread workgroups, one per GLWE polynomial, so that each CU carries half of every
+// CMUX: the rotation and decomposition of its polynomial, the forward NTTs of its L digit
+// polynomials, output column h of the external product and its inverse NTT.  The product
+// also needs the partner's digits: thread t of one workgroup sends its NTT-domain digits to
+// thread t of the other through global memory, as 64-bit words of 32 data bits under a
+// 32-bit tag (CMUX index + 1), stored and polled with relaxed agent-scope atomics.  A word
+// whose tag matches carries its own data, so neither fences nor flags are needed.  The two
+// parities of the exchange area alternate per CMUX; a workgroup overwrites parity i & 1 at
+// CMUX i + 2 only after receiving its partner's CMUX i + 1 words, which the partner sends
+// after it has read CMUX i.  The area must hold no tag of this launch when the launch
+// starts (the launcher zeroes it first).  Workgroups bid and bid ^ 8 form a pair, so that
+// round-robin dispatch over the 8 XCDs puts both on one XCD's L2.  Every wait ends: a
+// partner silent for SPLIT_WAIT_TICKS of the 100 MHz real-time clock makes the workgroup
+// set *status and leave, and its partner then does the same (outputs undefined; the host
+// reports the status as an error).  Arithmetic and CMUX order are those of k_blind_rotate,
+// so results are bit-identical to it.
+// ------------------------------------------------------------------------------------
+constexpr uint64_t SPLIT_WAIT_TICKS = 200000000ull;  // 2 s
+
+template <int L>
+constexpr size_t split_words() {  // [parity 2][side 2][L][w0 8][half 2][T]
+  return (size_t)2 * 2 * L * 16 * T;
+}
+
+__device__ __forceinline__ void split_put(uint64_t* p, uint64_t tag, uint32_t x) {
+  __hip_atomic_store(p, tag | x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int L, bool HI>
+__global__ __launch_bounds__(256, 1) void k_blind_rotate_split(const uint64_t* __restrict__ bsk,
+                                                              const uint64_t* __restrict__ luts,
+                                                              const uint16_t* __restrict__ lwe2n,
+                                                              const PbsJob* __restrict__ jobs, int nct,
+                                                              const uint64_t* __restrict__ tab, BrArgs a,
+                                                              uint64_t* __restrict__ xch, uint32_t* status) {
+  constexpr int K1 = 2, NPF = K1 * L;
+  __shared__ uint64_t lds[L * N];  // this polynomial: accumulator / digit NTT work space
+  __shared__ int abort_;
+  const int bid = blockIdx.x;
+  const int c = (bid >> 4) * 8 + (bid & 7);
+  const int h = (bid >> 3) & 1;
+  if (c >= nct) return;  // both workgroups of a pair leave together
+  const PbsJob job = jobs[c];
+  if (job.out_addr == 0) return;  // an empty slot of a batch list
+  const int t = threadIdx.x;
+  const int r = t & 63, m0 = t >> 6;
+  const size_t bsk_stride = (size_t)8 * NPF * K1 * T;
+  const __amdgpu_buffer_rsrc_t bsk_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)bsk, (short)0, (int)((size_t)a.n_lwe * bsk_stride * 8), 0x00020000);
+  const uint16_t* a2n = lwe2n + (size_t)c * a.lwe_stride;
+  const uint64_t* lut = luts + (size_t)job.gid * K1 * N;
+  uint64_t* xc = xch + (size_t)c * split_words<L>();
+  RegTw twr;
+  twr.load(tab, t);
+  if (t == 0) abort_ = 0;
+  uint64_t acc[8];
+#pragma unroll
+  for (int m1 = 0; m1 < 8; m1++) acc[m1] = lut[h * N + 64 * (m0 + 4 * m1) + r];
+  uint32_t rot_next = a2n[0];
+  for (int i = 0; i < a.n_lwe; i++) {
+    const uint32_t rot = rot_next;
+    rot_next = a2n[i + 1];  // i + 1 <= n_lwe: the body slot, always present
+    int tt = t;
+    asm volatile("" : "+v"(tt));
+    const int rr = tt & 63, mm0 = tt >> 6;
+    // this CMUX's key words of output column h, in flight during the rotation and forward NTT
+    const int slice_off = (int)((size_t)i * bsk_stride * 8);
+    const int lane_off = tt * 8;
+    uint64_t kw[8][NPF];
+#pragma unroll
+    for (int w0 = 0; w0 < 8; w0++)
+#pragma unroll
+      for (int gl_ = 0; gl_ < NPF; gl_++) {
+        const int e = (w0 * NPF + gl_) * K1 + h;
+        const auto w = __builtin_amdgcn_raw_buffer_load_b64(bsk_rsrc, lane_off, slice_off + e * T * 8, 0);
+        kw[w0][gl_] = ((uint64_t)w[1] << 32) | w[0];
+      }
+    __syncthreads();
+    if (abort_) break;
+#pragma unroll
+    for (int m1 = 0; m1 < 8; m1++) lds[64 * (mm0 + 4 * m1) + rr] = acc[m1];
+    __syncthreads();
+    uint64_t v[L][8];
+    pbs::rotate_decompose<L, HI>(lds, acc, tt, rot, a.base_log, v);
+    __syncthreads();
+    fwd_passes_tw<L, HI>(v, lds, twr, tt);
+    // send this polynomial's digit rows h * L + l
+    const uint64_t tag = (uint64_t)(i + 1) << 32;
+    uint64_t* snd = xc + (size_t)((i & 1) * 2 + h) * L * 16 * T + tt;
+#pragma unroll
+    for (int l = 0; l < L; l++)
+#pragma unroll
+      for (int w0 = 0; w0 < 8; w0++) {
+        split_put(snd + ((l * 8 + w0) * 2) * T, tag, (uint32_t)v[l][w0]);
+        split_put(snd + ((l * 8 + w0) * 2 + 1) * T, tag, (uint32_t)(v[l][w0] >> 32));
+      }
+    // own rows of the product while the partner's words travel: 128-bit sums per output
+    uint64_t lo[8], hi[8];
+    uint32_t cc[8];
+#pragma unroll
+    for (int w0 = 0; w0 < 8; w0++) {
+      cc[w0] = 0;
+#pragma unroll
+      for (int l = 0; l < L; l++) {
+        uint64_t plo, phi;
+        gl::mul_full(v[l][w0], kw[w0][h * L + l], plo, phi);
+        if (l == 0)
+          lo[w0] = plo, hi[w0] = phi;
+        else
+          gl::acc128(lo[w0], hi[w0], cc[w0], plo, phi);
+      }
+    }
+    // the partner's rows (1 - h) * L + l
+    const uint64_t* rcv = xc + (size_t)((i & 1) * 2 + (1 - h)) * L * 16 * T + tt;
+    uint64_t got[L][8][2];
+    uint64_t deadline = 0;
+    for (;;) {
+      bool ready = true;
+#pragma unroll
+      for (int l = 0; l < L; l++)
+#pragma unroll
+        for (int w0 = 0; w0 < 8; w0++)
+#pragma unroll
+          for (int q = 0; q < 2; q++) {
+            got[l][w0][q] = __hip_atomic_load(rcv + ((l * 8 + w0) * 2 + q) * T, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+            ready &= (got[l][w0][q] >> 32) == (uint64_t)(i + 1);
+          }
+      if (ready) break;
+      const uint64_t now = __builtin_amdgcn_s_memrealtime();
+      if (deadline == 0) {
+        deadline = now + SPLIT_WAIT_TICKS;
+      } else if (now > deadline) {
+        __hip_atomic_store(status, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        abort_ = 1;  // read by every thread after the next barrier
+        break;
+      }
+    }
+#pragma unroll
+    for (int w0 = 0; w0 < 8; w0++)
+#pragma unroll
+      for (int l = 0; l < L; l++) {
+        const uint64_t d = (got[l][w0][1] << 32) | (uint32_t)got[l][w0][0];
+        uint64_t plo, phi;
+        gl::mul_full(d, kw[w0][(1 - h) * L + l], plo, phi);
+        gl::acc128(lo[w0], hi[w0], cc[w0], plo, phi);
+      }
+    uint64_t z[1][8];
+    gl::reduce160x4(z[0], lo, hi, cc);
+    gl::reduce160x4(z[0] + 4, lo + 4, hi + 4, cc + 4);
+    inv_passes_tw<1>(z, lds, twr, tt);
+#pragma unroll
+    for (int m1 = 0; m1 < 8; m1++) acc[m1] += gl::modswitch_p_to_q(z[0][m1]);
+  }
+  __syncthreads();
+  if (abort_) return;
+  // sample extract: the mask from polynomial 0, the body from polynomial 1
+#pragma unroll
+  for (int m1 = 0; m1 < 8; m1++) lds[64 * (m0 + 4 * m1) + r] = acc[m1];
+  __syncthreads();
+  const uint32_t b = a2n[a.n_lwe];
+  const size_t blen = (size_t)(K1 - 1) * N + 1;
+  const int lut_nb = 1 << job.log_lut_nb;
+  for (int lu = 0; lu < lut_nb; lu++) {
+    const uint32_t offs = (uint32_t)(lu * (N >> job.log_lut_nb));
+    uint64_t* o = reinterpret_cast<uint64_t*>(job.out_addr) + (size_t)lu * blen;
+    if (h == 0) {
+#pragma unroll
+      for (int e = 0; e < N / T; e++) {
+        const int m = t + T * e;
+        o[m] = pbs::extract_coef(lds, 0, m, b, offs);
+      }
+    } else if (t == 0) {
+      o[N] = pbs::extract_coef(lds, 0, 0, b, offs);
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_zero_words(uint64_t* __restrict__ p, size_t n) {
+  for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (size_t)gridDim.x * 256) p[e] = 0;
+}
+
+// ------------------------------------------------------------------------------------
 // ALU DOps (pea_alu_core.sv:55-160) and LD/ST row copies.
 // ------------------------------------------------------------------------------------
 // The scalar constant of ADDS/SUBS/SSUB occupies the top `useful` bits of the body only:
@@ -592,8 +846,13 @@ static int br_lat_max_grid() {
 template <int L, bool HI>
 static void br_launch(const uint64_t* bsk, const uint64_t* luts, const uint16_t* lwe2n, const PbsJob* jobs, int nct,
                       const uint64_t* tab, const BrArgs& a, int grid_override, hipStream_t s,
-                      bool latency, bool pair) {
-  if (latency) {
+                      bool latency, bool pair, uint64_t* xch, uint32_t* status) {
+  if (xch) {
+    const size_t words = (size_t)nct * split_words<L>();
+    k_zero_words<<<dim3((unsigned)std::min<size_t>((words + 255) / 256, 1024)), 256, 0, s>>>(xch, words);
+    k_blind_rotate_split<L, HI><<<dim3((unsigned)(16 * ((nct + 7) / 8))), 256, 0, s>>>(bsk, luts, lwe2n, jobs, nct,
+                                                                                        tab, a, xch, status);
+  } else if (latency) {
     int g = br_lat_max_grid<2, L, false>();
     g = g < nct ? g : nct;
     k_blind_rotate_lat<2, L, HI, false><<<dim3(g), 512, 0, s>>>(bsk, luts, lwe2n, jobs, nct, tab, a);
@@ -610,25 +869,40 @@ static void br_launch(const uint64_t* bsk, const uint64_t* luts, const uint16_t*
 
 hipError_t launch_blind_rotate(const uint64_t* bsk, const uint64_t* luts, const uint16_t* lwe2n, const PbsJob* jobs,
                                int nct, const uint64_t* tab, const BrArgs& a,
-                               int pbs_level, int grid_override, hipStream_t s, int latency_max, bool pair) {
+                               int pbs_level, int grid_override, hipStream_t s, int latency_max, bool pair,
+                               int split_max, uint64_t* xch, uint32_t* status) {
   if (nct == 0) return hipSuccess;
   if (pbs_level != 1 && pbs_level != 2) return hipErrorInvalidValue;
   // the batched kernel addresses the key through a buffer resource of at most 2^31 - 1 bytes
   if ((size_t)a.n_lwe * 8 * (2 * pbs_level) * 2 * T * 8 > 0x7FFFFFFFull) return hipErrorInvalidValue;
   const bool latency = nct <= latency_max;
+  if (!(xch && status && nct <= split_max)) xch = nullptr;
   const bool hi = (uint32_t)pbs_level * a.base_log <= 31 && a.base_log <= 23;
   if (pbs_level == 1) {
     if (hi)
-      br_launch<1, true>(bsk, luts, lwe2n, jobs, nct, tab, a, grid_override, s, latency, pair);
+      br_launch<1, true>(bsk, luts, lwe2n, jobs, nct, tab, a, grid_override, s, latency, pair, xch, status);
     else
-      br_launch<1, false>(bsk, luts, lwe2n, jobs, nct, tab, a, grid_override, s, latency, pair);
+      br_launch<1, false>(bsk, luts, lwe2n, jobs, nct, tab, a, grid_override, s, latency, pair, xch, status);
   } else {
     if (hi)
-      br_launch<2, true>(bsk, luts, lwe2n, jobs, nct, tab, a, grid_override, s, latency, pair);
+      br_launch<2, true>(bsk, luts, lwe2n, jobs, nct, tab, a, grid_override, s, latency, pair, xch, status);
     else
-      br_launch<2, false>(bsk, luts, lwe2n, jobs, nct, tab, a, grid_override, s, latency, pair);
+      br_launch<2, false>(bsk, luts, lwe2n, jobs, nct, tab, a, grid_override, s, latency, pair, xch, status);
   }
   return hipGetLastError();
+}
+
+size_t blind_rotate_split_words(int pbs_level) { return pbs_level == 1 ? split_words<1>() : split_words<2>(); }
+
+int blind_rotate_split_slots() {
+  static int cached = 0;
+  if (!cached) {
+    int dev = 0, cus = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    cached = (cus > 0 ? cus : 256) / 2;
+  }
+  return cached;
 }
 
 int blind_rotate_occupancy(int pbs_level, bool pair) {

@@ -173,6 +173,43 @@ int ensure_ks_part(luthe_ctx* c, PbsScratch& sc, size_t n) {
   return LUTHE_OK;
 }
 
+int split_max(luthe_ctx* c) {
+  if (c->br_split_max < 0) {
+    (void)hipSetDevice(c->device);
+    c->br_split_max = lk::blind_rotate_split_slots();
+  }
+  return c->br_split_max;
+}
+
+int ensure_xch(luthe_ctx* c, PbsScratch& sc, size_t n) {
+  n = std::min(n, (size_t)std::max(split_max(c), 0));
+  if (n <= sc.xch_cap) return LUTHE_OK;
+  const size_t cap = std::min(std::max(n, sc.xch_cap * 2), (size_t)split_max(c));
+  const size_t bytes = lk::blind_rotate_split_words((int)c->prm.pbs_level) * 8;
+  uint64_t* nb = nullptr;
+  int rc = dev_alloc(c, (void**)&nb, cap * bytes);
+  if (rc) return rc;
+  if (sc.xch) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    retire_scratch(c, sc.xch, sc.xch_cap * bytes);
+  }
+  sc.xch = nb;
+  sc.xch_cap = cap;
+  return LUTHE_OK;
+}
+
+std::string last_error(luthe_ctx* c) {
+  std::lock_guard<std::mutex> g(c->err_mu);
+  return c->err;
+}
+
+int check_device_status(luthe_ctx* c) {
+  if (!c->h_status || __atomic_exchange_n(c->h_status, 0u, __ATOMIC_ACQ_REL) == 0) return LUTHE_OK;
+  return fail(c, LUTHE_ERR_HIP,
+              "split blind rotation: a workgroup waited over 2 s for its partner (not resident?); "
+              "the outputs of that launch are undefined");
+}
+
 // Staging of per-submission descriptors: host pinned buffer -> device buffer.
 int stage_upload(luthe_ctx* c, const std::vector<uint8_t>& bytes) {
   size_t n = bytes.size();
@@ -226,7 +263,7 @@ int latency_max(luthe_ctx* c) {
 // KS + BR over jobs already on the device, with the mod-switched LWEs in `lwe2n`
 // (n rows of lwe_stride values).  Allocates nothing.
 int run_pbs_on(luthe_ctx* c, const PbsJob* d_jobs, int n, hipStream_t s,
-               uint16_t* lwe2n, uint32_t* ks_part) {
+               uint16_t* lwe2n, uint32_t* ks_part, uint64_t* xch) {
   if (!c->bsk_ok || !c->ksk_ok) return fail(c, LUTHE_ERR_STATE, "keys not uploaded");
   if (n == 0) return LUTHE_OK;
   const luthe_params& p = c->prm;
@@ -235,7 +272,8 @@ int run_pbs_on(luthe_ctx* c, const PbsJob* d_jobs, int n, hipStream_t s,
   HIPCHK(c, lk::launch_keyswitch(d_jobs, n, c->d_ksk, lwe2n, ka, (int)p.ks_level, s, ks_part, nsplit));
   lk::BrArgs ba{(int)p.lwe_dim, c->lwe_stride, p.pbs_base_log};
   HIPCHK(c, lk::launch_blind_rotate(c->d_bsk, c->d_luts, lwe2n, d_jobs, n, c->d_tab, ba,
-                                    (int)p.pbs_level, c->br_grid_override, s, latency_max(c), use_pair(c)));
+                                    (int)p.pbs_level, c->br_grid_override, s, latency_max(c), use_pair(c),
+                                    split_max(c), xch, c->d_status));
   {
     std::lock_guard<std::mutex> g(c->cnt_mu);  // an attached server's thread counts here too
     c->cnt.pbs += (uint64_t)n;
@@ -251,8 +289,9 @@ int run_pbs(luthe_ctx* c, PbsScratch& sc, const PbsJob* d_jobs, int n, hipStream
   if (n == 0) return LUTHE_OK;
   int rc = ensure_lwe2n(c, sc, (size_t)n);
   if (!rc) rc = ensure_ks_part(c, sc, (size_t)n);
+  if (!rc) rc = ensure_xch(c, sc, (size_t)n);
   if (rc) return rc;
-  return run_pbs_on(c, d_jobs, n, s, sc.lwe2n, sc.ks_part);
+  return run_pbs_on(c, d_jobs, n, s, sc.lwe2n, sc.ks_part, (size_t)n <= sc.xch_cap ? sc.xch : nullptr);
 }
 
 void trace_push(luthe_ctx* c, const luthe_trace_event& e) {
@@ -329,6 +368,9 @@ int luthe_create(const luthe_params* p, int device, luthe_ctx** out) {
   if (hipSetDevice(device) != hipSuccess) return bail(LUTHE_ERR_HIP);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail(LUTHE_ERR_HIP);
   if ((rc = noalloc_init(c, device))) return bail(rc);
+  if (hipHostMalloc((void**)&c->h_status, 64, hipHostMallocMapped) != hipSuccess) return bail(LUTHE_ERR_HIP);
+  *c->h_status = 0;
+  if (hipHostGetDevicePointer((void**)&c->d_status, c->h_status, 0) != hipSuccess) return bail(LUTHE_ERR_HIP);
   if (hipEventCreateWithFlags(&c->stage_copied, hipEventDisableTiming) != hipSuccess) return bail(LUTHE_ERR_HIP);
   size_t rows = (size_t)p->regfile_size + p->ct_slots;
   if ((rc = dev_alloc(c, (void**)&c->d_mem, rows * c->blen * 8))) return bail(rc);
@@ -353,10 +395,11 @@ int luthe_destroy(luthe_ctx* c) {
   for (luthe_graph* g : c->graphs) g->ctx = nullptr;  // their scratch is freed below
   for (auto& r : c->retired) (void)hipFree(r.first);
   void* bufs[] = {c->d_mem,     c->d_luts,      c->d_bsk,   c->d_ksk, c->d_tab,     c->bs.lwe2n,
-                  c->bs.ks_part, c->ds.lwe2n, c->ds.ks_part, c->d_jobs, c->d_stage};
+                  c->bs.ks_part, c->ds.lwe2n, c->ds.ks_part, c->d_jobs, c->d_stage, c->bs.xch, c->ds.xch};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (c->h_stage) (void)hipHostFree(c->h_stage);
+  if (c->h_status) (void)hipHostFree(c->h_status);
   if (c->stage_copied) (void)hipEventDestroy(c->stage_copied);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -440,7 +483,7 @@ int luthe_sync(luthe_ctx* c) {
   int rc = dop_progress(c, true);
   if (rc) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  return LUTHE_OK;
+  return check_device_status(c);
 }
 
 // ---------------------------------------------------------------- keys and memories
@@ -493,6 +536,7 @@ static int rows_io(luthe_ctx* c, uint64_t* dbase, uint32_t first, uint32_t count
   int rc = dop_progress(c, true);  // submitted DOps come first, in submission order
   if (rc) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  if ((rc = check_device_status(c))) return rc;
   if (hin)
     HIPCHK(c, hipMemcpy(dbase + (size_t)first * len, hin, (size_t)count * len * 8, hipMemcpyHostToDevice));
   else
@@ -556,10 +600,12 @@ int luthe_blind_rotate_batch(luthe_ctx* c, const uint16_t* d_lwe2n, uint64_t* d_
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, lk::launch_make_jobs(c->d_jobs, nullptr, d_out, d_gid, (int)n, log_lut_nb, c->prm.lut_slots, (int)c->blen,
                                  s));
+  if ((rc = ensure_xch(c, c->bs, n))) return rc;
   const luthe_params& p = c->prm;
   lk::BrArgs ba{(int)p.lwe_dim, (int)p.lwe_dim + 1, p.pbs_base_log};
   HIPCHK(c, lk::launch_blind_rotate(c->d_bsk, c->d_luts, d_lwe2n, c->d_jobs, (int)n, c->d_tab, ba,
-                                    (int)p.pbs_level, c->br_grid_override, s, latency_max(c), use_pair(c)));
+                                    (int)p.pbs_level, c->br_grid_override, s, latency_max(c), use_pair(c),
+                                    split_max(c), n <= c->bs.xch_cap ? c->bs.xch : nullptr, c->d_status));
   return LUTHE_OK;
 }
 
@@ -616,6 +662,7 @@ int luthe_reserve(luthe_ctx* c, size_t max_batch) {
   HIPCHK(c, hipSetDevice(c->device));
   int rc = ensure_jobs(c, max_batch);
   if (!rc) rc = ensure_ks_part(c, c->bs, max_batch);
+  if (!rc) rc = ensure_xch(c, c->bs, max_batch);
   if (rc) return rc;
   return ensure_lwe2n(c, c->bs, max_batch);
 }
@@ -976,6 +1023,7 @@ int luthe_poll(luthe_ctx* c, uint64_t ticket) {
   if (e == hipErrorNotReady) return 0;
   if (e != hipSuccess) return fail(c, LUTHE_ERR_HIP, "ticket: %s", hipGetErrorString(e));
   release_ticket(c, it);
+  if ((rc = check_device_status(c))) return rc;
   return 1;
 }
 
@@ -1001,7 +1049,7 @@ int luthe_wait(luthe_ctx* c, uint64_t ticket, int64_t timeout_us) {
   }
   release_ticket(c, it);
   (void)dop_progress(c, false);  // a group slot is free: the open group may start
-  return LUTHE_OK;
+  return check_device_status(c);
 }
 
 int luthe_flush(luthe_ctx* c) {

@@ -393,7 +393,7 @@ int issue_step(luthe_server* s, std::vector<Request*>& admitted) {
   }
   if (!jobs.empty()) {
     rc = lx::run_pbs(c, c->bs, (const PbsJob*)(sl.dev + jobs_off), (int)jobs.size(), st);
-    if (rc) return sfail(s, rc, "cohort: %s", c->err.c_str());
+    if (rc) return sfail(s, rc, "cohort: %s", lx::last_error(c).c_str());
   }
   for (size_t i = 0; i < post.size(); i++) {
     SHIP(s, lk::launch_alu(s->d_arena, (const AluOp*)(sl.dev + post_off[i].first), (int)post_off[i].second, blen,
@@ -448,6 +448,7 @@ int retire(luthe_server* s, bool block) {
     if (e == hipErrorNotReady) return 1;
     if (e != hipSuccess) return sfail(s, LUTHE_ERR_HIP, "step event: %s", hipGetErrorString(e));
   }
+  if (int rc = lx::check_device_status(s->c)) return sfail(s, rc, "step: %s", lx::last_error(s->c).c_str());
   const Clock::time_point now = Clock::now();
   for (Request* r : step.finished)
     memcpy(r->out_host, s->h_pinned + (r->pin0 + 2ull * r->nblocks) * s->blen, (size_t)r->nout * s->row_bytes);
@@ -630,6 +631,7 @@ int luthe_server_create(luthe_ctx* c, const luthe_server_config* cfg_in, luthe_s
   // the cohorts' scratch is sized once here: the server thread never grows context buffers
   if (!rc) rc = lx::ensure_lwe2n(c, c->bs, cfg.cohort_max);
   if (!rc) rc = lx::ensure_ks_part(c, c->bs, cfg.cohort_max);
+  if (!rc) rc = lx::ensure_xch(c, c->bs, cfg.cohort_max);
   if (rc) return rc;
   rc = lx::dev_alloc(c, (void**)&s->d_arena, arena_rows * s->row_bytes);
   if (rc) return rc;

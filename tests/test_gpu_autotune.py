@@ -1,6 +1,6 @@
 """GPU: the autotuner times its candidates, writes and re-reads its JSON, and every setting
-it can pick (blind-rotation width, 256- or 512-thread throughput kernel, latency-variant
-threshold, NTT polynomials per workgroup) leaves the results bit-identical."""
+it can pick (blind-rotation width, 256- or 512-thread throughput kernel, split- and
+latency-variant thresholds, NTT polynomials per workgroup) leaves the results bit-identical."""
 import json
 import os
 import tempfile
@@ -38,14 +38,20 @@ def test_autotune_and_settings_are_bit_exact():
         ctx.set_tuning(br_grid=0, br_latency_max=0, ntt_polys_per_wg=2, cohort_max=0)
         ctx.load_tuning(path)
         assert ctx.tuning() == {**t, "br_latency_max": t["br_latency_max"]}
-    for grid, lat, np_, pair in [(256, 0, 1, 0), (512, 700, 4, 0), (1024, 0, 2, 0), (0, -1, 2, -1), (0, 0, 2, 1),
-                                 (300, 0, 2, 1)]:
-        print(f"case: settings grid={grid} latency_max={lat} ntt_polys_per_wg={np_} pair={pair}", flush=True)
-        ctx.set_tuning(br_grid=grid, br_latency_max=lat, ntt_polys_per_wg=np_, br_pair=pair)
+    for grid, lat, np_, pair, split in [(256, 0, 1, 0, 0), (512, 700, 4, 0, 0), (1024, 0, 2, 0, 0),
+                                        (0, -1, 2, -1, -1), (0, 0, 2, 1, 0), (300, 0, 2, 1, 0), (0, 0, 2, 0, 700)]:
+        print(f"case: settings grid={grid} latency_max={lat} ntt_polys_per_wg={np_} pair={pair} split_max={split}",
+              flush=True)
+        ctx.set_tuning(br_grid=grid, br_latency_max=lat, ntt_polys_per_wg=np_, br_pair=pair, br_split_max=split)
         assert np.array_equal(ctx.pbs_batch(cts, gids), ref)
         assert np.array_equal(ctx.ntt_polymul(a, bn), ref_ntt)
+    print("case: split variant over the first 100 ciphertexts", flush=True)
+    ctx.set_tuning(br_split_max=100)
+    assert np.array_equal(ctx.pbs_batch(cts[:100], gids[:100]), ref[:100])
     with pytest.raises(la.LutheError):
         ctx.set_tuning(ntt_polys_per_wg=3)
     with pytest.raises(la.LutheError):
         ctx.set_tuning(br_pair=2)
+    with pytest.raises(la.LutheError):
+        ctx.set_tuning(br_split_max=-2)
     ctx.close()

@@ -68,10 +68,11 @@ def test_last_slot_of_a_large_memory(oracle):
                                   dict(lwe_dim=200, pbs_base_log=20, pbs_level=2),
                                   dict(lwe_dim=200, pbs_base_log=28, pbs_level=1)])
 def test_other_parameters(oracle, over):
-    """Both blind-rotation kernels (the 512-thread latency variant that small batches take by
-    default, and the persistent throughput kernel, forced with the latency threshold 0); the
-    last two sets exceed the high-word decomposition's bounds (L * base_log > 31 or
-    base_log > 23), so they run the generic 64-bit decomposition and unpretwisted lift."""
+    """Every blind-rotation kernel: the split variant (two workgroups per ciphertext) that
+    small batches take by default, the 512-thread latency variant (split threshold 0) and the
+    persistent throughput kernel (latency threshold 0 as well); the last two sets exceed the
+    high-word decomposition's bounds (L * base_log > 31 or base_log > 23), so they run the
+    generic 64-bit decomposition and unpretwisted lift."""
     print(f"case: parameters {over}", flush=True)
     prm, keys, ctx, rp, bsk_ntt = _setup(oracle, 405, **over)
     tables = np.random.default_rng(406).integers(0, 16, (4, 16), dtype=np.uint8)
@@ -80,6 +81,9 @@ def test_other_parameters(oracle, over):
     cts = keys.encrypt_msg([0, 5, 10, 15], seed=407)
     gids = np.arange(4, dtype=np.uint32)
     exp = oracle.pbs_batch(rp, bsk_ntt, keys.ksk, luts, gids, cts, threads=THREADS)
+    got = ctx.pbs_batch(cts, gids)
+    assert np.array_equal(got, exp), "split kernel"
+    ctx.set_tuning(br_split_max=0)
     got = ctx.pbs_batch(cts, gids)
     assert np.array_equal(got, exp), "latency kernel"
     assert la.lib().luthe_set_br_latency_max(ctx.handle, 0) == 0

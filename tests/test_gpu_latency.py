@@ -19,6 +19,7 @@ def test_latency_variant_bit_exact(oracle, level, base_log):
     ctx = la.Context(prm, device=0)
     ctx.upload_bsk(keys.bsk)
     ctx.upload_ksk(keys.ksk)
+    ctx.set_tuning(br_split_max=0)  # small batches on the 512-thread kernel, not the split variant
     rng = np.random.default_rng(level)
     tables = rng.integers(0, 16, (8, 16), dtype=np.uint8)
     luts = np.stack([la.lut_encode(prm, t) for t in tables])
