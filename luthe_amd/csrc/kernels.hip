@@ -90,10 +90,17 @@ struct RegTw {
 };
 
 // PRE: the inputs are already pretwisted by psi^(4 m1) (blind-rotation digits, HI path).
-template <int NP, bool PRE, class TW>
-__device__ __forceinline__ void fwd_passes_tw(uint64_t (*v)[8], uint64_t* lds, const TW& tw, int t) {
+// PP = false: the passes work in place in lds, with a barrier before each store (the
+// previous pass's readers) and after it.  PP = true: the stores alternate between lds (A1,
+// C1) and lds2 (A2) and only the barrier after each store remains; the caller guarantees that
+// no thread still reads lds or lds2 at entry except lds2 by reads that precede the first
+// barrier here.  Exit: C2 has read lds.
+template <int NP, bool PRE, class TW, bool PP = false>
+__device__ __forceinline__ void fwd_passes_tw(uint64_t (*v)[8], uint64_t* lds, const TW& tw, int t,
+                                              uint64_t* lds2 = nullptr) {
   const int r = t & 63, m0 = t >> 6, k = t >> 3, j = t & 7;
   const int m0u = wave_uniform(m0);
+  uint64_t* mid = PP ? lds2 : lds;
 #pragma unroll
   for (int p = 0; p < NP; p++) fwd_a1_compute(v[p], m0u, PRE);
 #pragma unroll
@@ -104,13 +111,13 @@ __device__ __forceinline__ void fwd_passes_tw(uint64_t (*v)[8], uint64_t* lds, c
     fwd_a2_load(v[p], lds + p * N, r, m0);
     tw.a2(v[p], r, m0u);
   }
-  __syncthreads();
+  if constexpr (!PP) __syncthreads();
 #pragma unroll
-  for (int p = 0; p < NP; p++) fwd_a2_store(v[p], lds + p * N, r, m0);
+  for (int p = 0; p < NP; p++) fwd_a2_store(v[p], mid + p * N, r, m0);
   __syncthreads();
-  fwd_c1_load<NP>(v, lds, t);
+  fwd_c1_load<NP>(v, mid, t);
   tw.template c1<NP>(v, t);
-  __syncthreads();
+  if constexpr (!PP) __syncthreads();
   fwd_c1_store<NP>(v, lds, t);
   __syncthreads();
 #pragma unroll
@@ -126,30 +133,35 @@ __device__ __forceinline__ void fwd_passes(uint64_t (*v)[8], uint64_t* lds, cons
 
 // Entry: v in NTT domain (thread (k, w1) layout), LDS free for writing only after a
 // barrier that orders the previous readers.  Exit: v[p][m1] canonical coefficients.
-template <int NP, class TW>
-__device__ __forceinline__ void inv_passes_tw(uint64_t (*v)[8], uint64_t* lds, const TW& tw, int t) {
+// PP = true: the stores alternate between lds (C2, A2) and lds2 (C1) with one barrier
+// per pass; at entry no thread may still read lds, and lds2 only by reads that precede the
+// first barrier here.  Exit: A1 has read lds.
+template <int NP, class TW, bool PP = false>
+__device__ __forceinline__ void inv_passes_tw(uint64_t (*v)[8], uint64_t* lds, const TW& tw, int t,
+                                              uint64_t* lds2 = nullptr) {
   // Opaque copy of the thread id: the index values below are derived here rather than
   // kept live across the caller's code (which would spill at 128 VGPRs).
   asm volatile("" : "+v"(t));
   const int r = t & 63, m0 = t >> 6, k = t >> 3, j = t & 7;
   const int m0u = wave_uniform(m0);
+  uint64_t* mid = PP ? lds2 : lds;
 #pragma unroll
   for (int p = 0; p < NP; p++) inv_c2_compute(v[p]);
-  __syncthreads();
+  if constexpr (!PP) __syncthreads();
 #pragma unroll
   for (int p = 0; p < NP; p++) inv_c2_store(v[p], lds + p * N, k, j);
   __syncthreads();
   inv_c1_load<NP>(v, lds, t);
   tw.template ic1<NP>(v, t);
-  __syncthreads();
-  inv_c1_store<NP>(v, lds, t);
+  if constexpr (!PP) __syncthreads();
+  inv_c1_store<NP>(v, mid, t);
   __syncthreads();
 #pragma unroll
   for (int p = 0; p < NP; p++) {
-    inv_a2_load(v[p], lds + p * N, r, m0);
+    inv_a2_load(v[p], mid + p * N, r, m0);
     inv_a2_compute(v[p], m0u);
   }
-  __syncthreads();
+  if constexpr (!PP) __syncthreads();
 #pragma unroll
   for (int p = 0; p < NP; p++) inv_a2_store(v[p], lds + p * N, r, m0);
   __syncthreads();
@@ -490,7 +502,9 @@ __global__ __launch_bounds__(256, 1) void k_blind_rotate_split(const uint64_t* _
                                                               const uint64_t* __restrict__ tab, BrArgs a,
                                                               uint64_t* __restrict__ xch, uint32_t* status) {
   constexpr int K1 = 2, NPF = K1 * L;
-  __shared__ uint64_t lds[L * N];  // this polynomial: accumulator / digit NTT work space
+  // this polynomial's accumulator and digit NTT work space, two buffers used alternately
+  // (each store goes to the buffer whose readers have passed the last barrier)
+  __shared__ uint64_t lds[2 * L * N];
   __shared__ int abort_;
   const int bid = blockIdx.x;
   const int c = (bid >> 4) * 8 + (bid & 7);
@@ -513,33 +527,43 @@ __global__ __launch_bounds__(256, 1) void k_blind_rotate_split(const uint64_t* _
 #pragma unroll
   for (int m1 = 0; m1 < 8; m1++) acc[m1] = lut[h * N + 64 * (m0 + 4 * m1) + r];
   uint32_t rot_next = a2n[0];
+  // Buffers per CMUX (x: accumulator, y: the other): acc -> x | rotation reads x | forward
+  // A1 -> y, A2 -> x, C1 -> y | inverse C2 -> x, C1 -> y, A2 -> x | A1 reads x; the next
+  // CMUX swaps x and y.  Every store's buffer was last read before the preceding barrier.
+  uint64_t* bx = lds;
+  uint64_t* by = lds + L * N;
+  __syncthreads();  // abort_ initialised
   for (int i = 0; i < a.n_lwe; i++) {
     const uint32_t rot = rot_next;
     rot_next = a2n[i + 1];  // i + 1 <= n_lwe: the body slot, always present
     int tt = t;
     asm volatile("" : "+v"(tt));
     const int rr = tt & 63, mm0 = tt >> 6;
-    // this CMUX's key words of output column h, in flight during the rotation and forward NTT
+    // set before the last barrier of the previous CMUX (or the one above), so uniform here
+    if (abort_) break;
+    // this CMUX's key words of output column h, in flight during the rotation and forward NTT:
+    // kw[w0][0][l] for this workgroup's rows h * L + l, kw[w0][1][l] for the partner's
+    // (scalar offsets, so that no register array is indexed by h)
     const int slice_off = (int)((size_t)i * bsk_stride * 8);
     const int lane_off = tt * 8;
-    uint64_t kw[8][NPF];
+    uint64_t kw[8][2][L];
 #pragma unroll
     for (int w0 = 0; w0 < 8; w0++)
 #pragma unroll
-      for (int gl_ = 0; gl_ < NPF; gl_++) {
-        const int e = (w0 * NPF + gl_) * K1 + h;
-        const auto w = __builtin_amdgcn_raw_buffer_load_b64(bsk_rsrc, lane_off, slice_off + e * T * 8, 0);
-        kw[w0][gl_] = ((uint64_t)w[1] << 32) | w[0];
-      }
-    __syncthreads();
-    if (abort_) break;
+      for (int side = 0; side < 2; side++)
 #pragma unroll
-    for (int m1 = 0; m1 < 8; m1++) lds[64 * (mm0 + 4 * m1) + rr] = acc[m1];
+        for (int l = 0; l < L; l++) {
+          const int g = side ? 1 - h : h;
+          const int e = (w0 * NPF + g * L + l) * K1 + h;
+          const auto w = __builtin_amdgcn_raw_buffer_load_b64(bsk_rsrc, lane_off, slice_off + e * T * 8, 0);
+          kw[w0][side][l] = ((uint64_t)w[1] << 32) | w[0];
+        }
+#pragma unroll
+    for (int m1 = 0; m1 < 8; m1++) bx[64 * (mm0 + 4 * m1) + rr] = acc[m1];
     __syncthreads();
     uint64_t v[L][8];
-    pbs::rotate_decompose<L, HI>(lds, acc, tt, rot, a.base_log, v);
-    __syncthreads();
-    fwd_passes_tw<L, HI>(v, lds, twr, tt);
+    pbs::rotate_decompose<L, HI>(bx, acc, tt, rot, a.base_log, v);
+    fwd_passes_tw<L, HI, RegTw, true>(v, by, twr, tt, bx);
     // send this polynomial's digit rows h * L + l
     const uint64_t tag = (uint64_t)(i + 1) << 32;
     uint64_t* snd = xc + (size_t)((i & 1) * 2 + h) * L * 16 * T + tt;
@@ -559,7 +583,7 @@ __global__ __launch_bounds__(256, 1) void k_blind_rotate_split(const uint64_t* _
 #pragma unroll
       for (int l = 0; l < L; l++) {
         uint64_t plo, phi;
-        gl::mul_full(v[l][w0], kw[w0][h * L + l], plo, phi);
+        gl::mul_full(v[l][w0], kw[w0][0][l], plo, phi);
         if (l == 0)
           lo[w0] = plo, hi[w0] = phi;
         else
@@ -598,15 +622,18 @@ __global__ __launch_bounds__(256, 1) void k_blind_rotate_split(const uint64_t* _
       for (int l = 0; l < L; l++) {
         const uint64_t d = (got[l][w0][1] << 32) | (uint32_t)got[l][w0][0];
         uint64_t plo, phi;
-        gl::mul_full(d, kw[w0][(1 - h) * L + l], plo, phi);
+        gl::mul_full(d, kw[w0][1][l], plo, phi);
         gl::acc128(lo[w0], hi[w0], cc[w0], plo, phi);
       }
     uint64_t z[1][8];
     gl::reduce160x4(z[0], lo, hi, cc);
     gl::reduce160x4(z[0] + 4, lo + 4, hi + 4, cc + 4);
-    inv_passes_tw<1>(z, lds, twr, tt);
+    inv_passes_tw<1, RegTw, true>(z, bx, twr, tt, by);
 #pragma unroll
     for (int m1 = 0; m1 < 8; m1++) acc[m1] += gl::modswitch_p_to_q(z[0][m1]);
+    uint64_t* sw = bx;
+    bx = by;
+    by = sw;
   }
   __syncthreads();
   if (abort_) return;
