@@ -266,7 +266,9 @@ int luthe_alu_batch(luthe_ctx* ctx, uint32_t opcode, const uint64_t* d_a, const 
                     uint32_t mul_factor, uint32_t msg_cst, size_t n, void* stream);
 
 /* luthe_pbs_batch without any allocation: no hipMalloc, hipFree or allocating library call,
- * at any batch size it accepts.  Same kernels and results as luthe_pbs_batch; its scratch
+ * at any batch size it accepts.  Same results as luthe_pbs_batch (small batches run on the
+ * one-CU latency kernel instead of the two-CU split kernel, whose exchange area is not part of
+ * the fixed array); its scratch
  * (job descriptors and mod-switched LWEs) is a fixed device array of 16.1875 MiB that each GPU
  * holds for the life of the process, so batches are limited to n <= LUTHE_NOALLOC_MAX_BATCH and
  * n * lwe_stride <= LUTHE_NOALLOC_MAX_LWE_WORDS, with lwe_stride = lwe_dim + 1 rounded up to a
@@ -299,7 +301,9 @@ int luthe_capture_end(luthe_ctx* ctx, luthe_graph** out);
 int luthe_graph_launch(luthe_ctx* ctx, luthe_graph* g);
 int luthe_graph_destroy(luthe_graph* g);
 
-/* Synchronise the context stream. */
+/* Synchronise the context stream.  LUTHE_ERR_HIP if a split blind rotation gave up waiting
+ * for its partner workgroup since the last synchronising call (outputs of that launch are
+ * undefined; see luthe_tuning.br_split_max). */
 int luthe_sync(luthe_ctx* ctx);
 /* The context's HIP stream (hipStream_t), on which DOp submissions and NULL-stream
  * batch calls run.  Batch calls share per-context scratch: calls that overlap in time
