@@ -337,6 +337,10 @@ typedef struct luthe_tuning {
                                level 2, 0 at level 1) */
   int32_t br_split_max;     /* largest cohort for the split variant (one 256-thread workgroup per
                                GLWE polynomial, two CUs per ciphertext), -1 = CUs / 2, 0 = off */
+  int32_t br_split_wait_us; /* split variant: longest wait of a workgroup for its partner before it
+                               gives up (LUTHE_ERR_HIP at the next synchronising call, outputs of
+                               that launch undefined); -1 = 2 s, 0 = give up at the first missing
+                               word (for tests of that path).  Start from luthe_get_tuning. */
 } luthe_tuning;
 int luthe_autotune(luthe_ctx* ctx, uint32_t batch, const char* json_path, luthe_tuning* out);
 int luthe_load_tuning(luthe_ctx* ctx, const char* json_path);

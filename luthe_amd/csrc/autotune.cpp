@@ -68,7 +68,7 @@ bool json_int(const std::string& s, const char* key, long& v) {
 
 int apply(luthe_ctx* c, const luthe_tuning& t) {
   if (t.br_grid < 0 || t.br_latency_max < -1 || t.cohort_max < 0 || t.br_pair < -1 || t.br_pair > 1 ||
-      t.br_split_max < -1)
+      t.br_split_max < -1 || t.br_split_wait_us < -1)
     return LUTHE_ERR_ARG;
   if (t.ntt_polys_per_wg != 1 && t.ntt_polys_per_wg != 2 && t.ntt_polys_per_wg != 4) return LUTHE_ERR_ARG;
   c->br_grid_override = t.br_grid;
@@ -77,6 +77,7 @@ int apply(luthe_ctx* c, const luthe_tuning& t) {
   c->cohort_max = t.cohort_max;
   c->br_pair = t.br_pair;
   c->br_split_max = t.br_split_max;
+  c->br_split_wait_us = t.br_split_wait_us;
   return LUTHE_OK;
 }
 
@@ -92,6 +93,7 @@ int luthe_get_tuning(luthe_ctx* c, luthe_tuning* out) {
   out->cohort_max = c->cohort_max;
   out->br_pair = c->br_pair;
   out->br_split_max = lx::split_max(c);
+  out->br_split_wait_us = c->br_split_wait_us;
   return LUTHE_OK;
 }
 
@@ -125,6 +127,7 @@ int luthe_load_tuning(luthe_ctx* c, const char* path) {
   t.cohort_max = json_int(s, "cohort_max", v) ? (int32_t)v : 0;
   t.br_pair = json_int(s, "br_pair", v) ? (int32_t)v : -1;
   t.br_split_max = json_int(s, "br_split_max", v) ? (int32_t)v : -1;
+  t.br_split_wait_us = c->br_split_wait_us;  // a robustness limit, not a tuned choice
   int rc = apply(c, t);
   if (rc) return lx::fail(c, rc, "%s: invalid tuning", path);
   return LUTHE_OK;
@@ -164,8 +167,9 @@ int luthe_autotune(luthe_ctx* c, uint32_t batch, const char* json_path, luthe_tu
   (void)hipMemsetAsync(d_a, 0x31, npoly * 2048 * 8, c->stream);
   (void)hipMemsetAsync(d_b, 0x17, npoly * 2048 * 8, c->stream);
   const luthe_tuning saved{c->br_grid_override, c->br_latency_max, c->ntt_polys_per_wg, c->cohort_max, c->br_pair,
-                           c->br_split_max};
+                           c->br_split_max, c->br_split_wait_us};
   c->br_pair = 0;  // (1) sweeps the 256-thread kernel
+  c->br_split_wait_us = -1;  // the split kernel's default wait limit while timing
   std::string meas;
   char line[256];
   // (1) resident ciphertexts per CU
@@ -234,11 +238,13 @@ int luthe_autotune(luthe_ctx* c, uint32_t batch, const char* json_path, luthe_tu
   }
   (void)hipStreamSynchronize(c->stream);
   cleanup();
+  if (!rc) rc = lx::check_device_status(c);
   if (rc) {
     (void)apply(c, saved);
     return rc;
   }
-  luthe_tuning t{best_pair || best_grid == occ ? 0 : best_grid, lat_max, best_np, width, best_pair, split_best};
+  luthe_tuning t{best_pair || best_grid == occ ? 0 : best_grid, lat_max, best_np, width, best_pair, split_best,
+                 saved.br_split_wait_us};
   (void)apply(c, t);
   if (out) *out = t;
   if (json_path) {

@@ -78,6 +78,7 @@ struct luthe_ctx {
   uint64_t dev_bytes = 0;
   luthe_counters cnt{};
   int br_split_max = -1;        // split blind rotation up to this many ciphertexts (-1: CUs / 2)
+  int br_split_wait_us = -1;    // its longest wait for a partner workgroup (-1: 2 s)
   uint32_t* h_status = nullptr;  // pinned: set by a split pair whose exchange timed out
   uint32_t* d_status = nullptr;  // its device address
   mutable std::mutex cnt_mu;  // counters and the error message are also written by a server thread

@@ -49,7 +49,8 @@ struct KsArgs {
 struct BrArgs {
   int n_lwe;
   int lwe_stride;
-  uint32_t base_log;  // PBS_B_W
+  uint32_t base_log;      // PBS_B_W
+  int split_wait_us = -1;  // split kernel: longest wait for the partner (-1: 2 s; 0: none)
 };
 
 // polys_per_wg: polynomials per 256-thread workgroup (1, 2 or 4; tuned).

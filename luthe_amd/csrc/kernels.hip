@@ -480,7 +480,8 @@ __global__ __launch_bounds__(512, TP ? 2 : 1) void k_blind_rotate_lat(const uint
 // round-robin dispatch over the 8 XCDs puts both on one XCD's L2.  Every wait ends: a
 // partner silent for SPLIT_WAIT_TICKS of the 100 MHz real-time clock makes the workgroup
 // set *status and leave, and its partner then does the same (outputs undefined; the host
-// reports the status as an error).  Arithmetic and CMUX order are those of k_blind_rotate,
+// reports the status as an error).  BrArgs::split_wait_us overrides the limit (0: give up at
+// the first poll that finds a word missing, which exercises this path in tests).  Arithmetic and CMUX order are those of k_blind_rotate,
 // so results are bit-identical to it.
 // ------------------------------------------------------------------------------------
 constexpr uint64_t SPLIT_WAIT_TICKS = 200000000ull;  // 2 s
@@ -523,6 +524,7 @@ __global__ __launch_bounds__(256, 1) void k_blind_rotate_split(const uint64_t* _
   RegTw twr;
   twr.load(tab, t);
   if (t == 0) abort_ = 0;
+  const uint64_t wait_ticks = a.split_wait_us < 0 ? SPLIT_WAIT_TICKS : (uint64_t)a.split_wait_us * 100;
   uint64_t acc[8];
 #pragma unroll
   for (int m1 = 0; m1 < 8; m1++) acc[m1] = lut[h * N + 64 * (m0 + 4 * m1) + r];
@@ -608,9 +610,8 @@ __global__ __launch_bounds__(256, 1) void k_blind_rotate_split(const uint64_t* _
           }
       if (ready) break;
       const uint64_t now = __builtin_amdgcn_s_memrealtime();
-      if (deadline == 0) {
-        deadline = now + SPLIT_WAIT_TICKS;
-      } else if (now > deadline) {
+      if (deadline == 0) deadline = now + wait_ticks;
+      if (now >= deadline) {
         __hip_atomic_store(status, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         abort_ = 1;  // read by every thread after the next barrier
         break;

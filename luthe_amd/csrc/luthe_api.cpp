@@ -206,7 +206,7 @@ std::string last_error(luthe_ctx* c) {
 int check_device_status(luthe_ctx* c) {
   if (!c->h_status || __atomic_exchange_n(c->h_status, 0u, __ATOMIC_ACQ_REL) == 0) return LUTHE_OK;
   return fail(c, LUTHE_ERR_HIP,
-              "split blind rotation: a workgroup waited over 2 s for its partner (not resident?); "
+              "split blind rotation: a workgroup gave up waiting for its partner (not resident?); "
               "the outputs of that launch are undefined");
 }
 
@@ -270,7 +270,7 @@ int run_pbs_on(luthe_ctx* c, const PbsJob* d_jobs, int n, hipStream_t s,
   lk::KsArgs ka{(int)(p.glwe_k * p.poly_size), (int)p.lwe_dim, c->lwe_stride, p.ks_base_log, p.ks_mod_log, 12};
   const int nsplit = ks_part ? lk::keyswitch_split(n, ka.kN, ka.n_lwe) : 1;
   HIPCHK(c, lk::launch_keyswitch(d_jobs, n, c->d_ksk, lwe2n, ka, (int)p.ks_level, s, ks_part, nsplit));
-  lk::BrArgs ba{(int)p.lwe_dim, c->lwe_stride, p.pbs_base_log};
+  lk::BrArgs ba{(int)p.lwe_dim, c->lwe_stride, p.pbs_base_log, c->br_split_wait_us};
   HIPCHK(c, lk::launch_blind_rotate(c->d_bsk, c->d_luts, lwe2n, d_jobs, n, c->d_tab, ba,
                                     (int)p.pbs_level, c->br_grid_override, s, latency_max(c), use_pair(c),
                                     split_max(c), xch, c->d_status));
@@ -602,7 +602,7 @@ int luthe_blind_rotate_batch(luthe_ctx* c, const uint16_t* d_lwe2n, uint64_t* d_
                                  s));
   if ((rc = ensure_xch(c, c->bs, n))) return rc;
   const luthe_params& p = c->prm;
-  lk::BrArgs ba{(int)p.lwe_dim, (int)p.lwe_dim + 1, p.pbs_base_log};
+  lk::BrArgs ba{(int)p.lwe_dim, (int)p.lwe_dim + 1, p.pbs_base_log, c->br_split_wait_us};
   HIPCHK(c, lk::launch_blind_rotate(c->d_bsk, c->d_luts, d_lwe2n, c->d_jobs, (int)n, c->d_tab, ba,
                                     (int)p.pbs_level, c->br_grid_override, s, latency_max(c), use_pair(c),
                                     split_max(c), n <= c->bs.xch_cap ? c->bs.xch : nullptr, c->d_status));

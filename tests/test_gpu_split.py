@@ -112,3 +112,27 @@ def test_split_batch_list_empty_slots(env):
     ctx.sync()
     for k in range(len(spec)):
         assert np.array_equal(bufs[k][2].numpy(), exp[k]), k
+
+
+def test_split_wait_limit(env):
+    """The split kernel's bounded wait: with no waiting allowed (br_split_wait_us = 0) a workgroup
+    that finds a partner word missing gives up, its partner then does the same, the next
+    synchronising call reports LUTHE_ERR_HIP once, and the context works as before afterwards."""
+    level, _, prm, keys, ctx, _, rng = env
+    print(f"case: split variant with no waiting allowed, L={level}", flush=True)
+    cts = keys.encrypt_msg(rng.integers(0, 16, 3), seed=1400)
+    gids = np.zeros(3, np.uint32)
+    ctx.set_tuning(br_split_max=-1, br_split_wait_us=-1)
+    exp = ctx.pbs_batch(cts, gids)
+    ctx.set_tuning(br_split_wait_us=0)
+    d_in, d_g = la.DeviceBuffer.from_numpy(cts), la.DeviceBuffer.from_numpy(gids)
+    d_out = la.DeviceBuffer((3, prm.blen), np.uint64)
+    ctx.pbs_batch_dev(d_in, d_out, d_g, 3)
+    with pytest.raises(la.LutheError) as e:
+        ctx.sync()
+    assert e.value.code == -3  # LUTHE_ERR_HIP
+    ctx.sync()  # reported once
+    ctx.set_tuning(br_split_wait_us=-1)
+    assert np.array_equal(ctx.pbs_batch(cts, gids), exp)
+    with pytest.raises(la.LutheError):
+        ctx.set_tuning(br_split_wait_us=-2)
