@@ -13,6 +13,7 @@
 
 #include "kernels.h"
 #include "ntt2048.h"
+#include "ntt2048x4.h"
 #include "pbs_core.h"
 
 using namespace ntt;
@@ -24,83 +25,14 @@ constexpr int MAX_GRID_Y = 65535;  // launches split their per-op grid dimension
 // ------------------------------------------------------------------------------------
 // NTT kernels: NP polynomials per 256-thread workgroup.
 // ------------------------------------------------------------------------------------
-// Twiddle sources of the passes.  TabTw reads the tables (ntt2048.h TABLE_WORDS layout,
-// L1/L2-resident) at each use.  RegTw holds one thread's twiddles of the A2 pass and of the
-// one-polynomial C passes in registers, loaded once per kernel: a kernel with one wave per
-// SIMD cannot hide table loads inside a chain of dependent passes.  Both give the same
-// values, so results do not depend on the source.
-struct TabTw {
-  const uint64_t* tab;
-  __device__ __forceinline__ void a2(uint64_t* v, int r, int q) const { fwd_a2_compute(v, r, q, tab + TW_OFF); }
-  template <int NP>
-  __device__ __forceinline__ void c1(uint64_t (*v)[8], int t) const {
-    fwd_c1_compute<NP>(v, t, tab + C64_OFF);
-  }
-  template <int NP>
-  __device__ __forceinline__ void ic1(uint64_t (*v)[8], int t) const {
-    inv_c1_compute<NP>(v, t, tab + ITW_OFF, tab + IC64_OFF);
-  }
-};
-
-struct RegTw {
-  uint64_t a2w[8];   // tw[(2q + h + 8 k0) * 64 + r] at h * 4 + k0 (A2 map, q = t >> 6)
-  uint64_t c1w[8];   // c64[8 w1 + r0] (one-polynomial C1 map; [0] = 1)
-  uint64_t ic1w[8];  // ic64[8 w1 + r0] ([0] = 1)
-  uint64_t itw[8];   // itw[(r0 + 8 r1) * 32 + k]
-  __device__ __forceinline__ void load(const uint64_t* tab, int t) {
-    const int r = t & 63, q = t >> 6, k = t & 31;
-    const int r0 = c1_r0<1>(0, t >> 6, (t >> 5) & 1);
-#pragma unroll
-    for (int h = 0; h < 2; h++)
-#pragma unroll
-      for (int k0 = 0; k0 < 4; k0++) a2w[h * 4 + k0] = tab[TW_OFF + (2 * q + h + 8 * k0) * 64 + r];
-    c1w[0] = ic1w[0] = 1;
-#pragma unroll
-    for (int w1 = 1; w1 < 8; w1++) {
-      c1w[w1] = tab[C64_OFF + 8 * w1 + r0];
-      ic1w[w1] = tab[IC64_OFF + 8 * w1 + r0];
-    }
-#pragma unroll
-    for (int r1 = 0; r1 < 8; r1++) itw[r1] = tab[ITW_OFF + (r0 + 8 * r1) * 32 + k];
-  }
-  __device__ __forceinline__ void a2(uint64_t* v, int, int) const {
-    dft4x2<48>(v);
-    gl::mul4(v, v, a2w);
-    gl::mul4(v + 4, v + 4, a2w + 4);
-  }
-  template <int NP>
-  __device__ __forceinline__ void c1(uint64_t (*v)[8], int t) const {
-    if constexpr (NP == 1) {
-      dft8<24>(v[0]);
-      gl::mul4(v[0], v[0], c1w);
-      gl::mul4(v[0] + 4, v[0] + 4, c1w + 4);
-    } else {
-      fwd_c1_compute<NP>(v, t, nullptr);  // compile-time twists, no table
-    }
-  }
-  template <int NP>
-  __device__ __forceinline__ void ic1(uint64_t (*v)[8], int) const {
-    static_assert(NP == 1, "register twiddles cover the one-polynomial inverse");
-    gl::mul4(v[0], v[0], ic1w);
-    gl::mul4(v[0] + 4, v[0] + 4, ic1w + 4);
-    idft8(v[0]);
-    gl::mul4(v[0], v[0], itw);
-    gl::mul4(v[0] + 4, v[0] + 4, itw + 4);
-  }
-};
-
+// tab: twiddle tables (ntt2048.h TABLE_WORDS layout).
 // PRE: the inputs are already pretwisted by psi^(4 m1) (blind-rotation digits, HI path).
-// PP = false: the passes work in place in lds, with a barrier before each store (the
-// previous pass's readers) and after it.  PP = true: the stores alternate between lds (A1,
-// C1) and lds2 (A2) and only the barrier after each store remains; the caller guarantees that
-// no thread still reads lds or lds2 at entry except lds2 by reads that precede the first
-// barrier here.  Exit: C2 has read lds.
-template <int NP, bool PRE, class TW, bool PP = false>
-__device__ __forceinline__ void fwd_passes_tw(uint64_t (*v)[8], uint64_t* lds, const TW& tw, int t,
-                                              uint64_t* lds2 = nullptr) {
+template <int NP, bool PRE = false>
+__device__ __forceinline__ void fwd_passes(uint64_t (*v)[8], uint64_t* lds, const uint64_t* tab, int t) {
   const int r = t & 63, m0 = t >> 6, k = t >> 3, j = t & 7;
   const int m0u = wave_uniform(m0);
-  uint64_t* mid = PP ? lds2 : lds;
+  const uint64_t* tw = tab + TW_OFF;
+  const uint64_t* c64 = tab + C64_OFF;
 #pragma unroll
   for (int p = 0; p < NP; p++) fwd_a1_compute(v[p], m0u, PRE);
 #pragma unroll
@@ -109,15 +41,15 @@ __device__ __forceinline__ void fwd_passes_tw(uint64_t (*v)[8], uint64_t* lds, c
 #pragma unroll
   for (int p = 0; p < NP; p++) {
     fwd_a2_load(v[p], lds + p * N, r, m0);
-    tw.a2(v[p], r, m0u);
+    fwd_a2_compute(v[p], r, m0u, tw);
   }
-  if constexpr (!PP) __syncthreads();
-#pragma unroll
-  for (int p = 0; p < NP; p++) fwd_a2_store(v[p], mid + p * N, r, m0);
   __syncthreads();
-  fwd_c1_load<NP>(v, mid, t);
-  tw.template c1<NP>(v, t);
-  if constexpr (!PP) __syncthreads();
+#pragma unroll
+  for (int p = 0; p < NP; p++) fwd_a2_store(v[p], lds + p * N, r, m0);
+  __syncthreads();
+  fwd_c1_load<NP>(v, lds, t);
+  fwd_c1_compute<NP>(v, t, c64);
+  __syncthreads();
   fwd_c1_store<NP>(v, lds, t);
   __syncthreads();
 #pragma unroll
@@ -126,42 +58,35 @@ __device__ __forceinline__ void fwd_passes_tw(uint64_t (*v)[8], uint64_t* lds, c
     fwd_c2_compute(v[p]);
   }
 }
-template <int NP, bool PRE = false>
-__device__ __forceinline__ void fwd_passes(uint64_t (*v)[8], uint64_t* lds, const uint64_t* tab, int t) {
-  fwd_passes_tw<NP, PRE>(v, lds, TabTw{tab}, t);
-}
 
 // Entry: v in NTT domain (thread (k, w1) layout), LDS free for writing only after a
 // barrier that orders the previous readers.  Exit: v[p][m1] canonical coefficients.
-// PP = true: the stores alternate between lds (C2, A2) and lds2 (C1) with one barrier
-// per pass; at entry no thread may still read lds, and lds2 only by reads that precede the
-// first barrier here.  Exit: A1 has read lds.
-template <int NP, class TW, bool PP = false>
-__device__ __forceinline__ void inv_passes_tw(uint64_t (*v)[8], uint64_t* lds, const TW& tw, int t,
-                                              uint64_t* lds2 = nullptr) {
+template <int NP>
+__device__ __forceinline__ void inv_passes(uint64_t (*v)[8], uint64_t* lds, const uint64_t* tab, int t) {
   // Opaque copy of the thread id: the index values below are derived here rather than
   // kept live across the caller's code (which would spill at 128 VGPRs).
   asm volatile("" : "+v"(t));
   const int r = t & 63, m0 = t >> 6, k = t >> 3, j = t & 7;
   const int m0u = wave_uniform(m0);
-  uint64_t* mid = PP ? lds2 : lds;
+  const uint64_t* itw = tab + ITW_OFF;
+  const uint64_t* ic64 = tab + IC64_OFF;
 #pragma unroll
   for (int p = 0; p < NP; p++) inv_c2_compute(v[p]);
-  if constexpr (!PP) __syncthreads();
+  __syncthreads();
 #pragma unroll
   for (int p = 0; p < NP; p++) inv_c2_store(v[p], lds + p * N, k, j);
   __syncthreads();
   inv_c1_load<NP>(v, lds, t);
-  tw.template ic1<NP>(v, t);
-  if constexpr (!PP) __syncthreads();
-  inv_c1_store<NP>(v, mid, t);
+  inv_c1_compute<NP>(v, t, itw, ic64);
+  __syncthreads();
+  inv_c1_store<NP>(v, lds, t);
   __syncthreads();
 #pragma unroll
   for (int p = 0; p < NP; p++) {
-    inv_a2_load(v[p], mid + p * N, r, m0);
+    inv_a2_load(v[p], lds + p * N, r, m0);
     inv_a2_compute(v[p], m0u);
   }
-  if constexpr (!PP) __syncthreads();
+  __syncthreads();
 #pragma unroll
   for (int p = 0; p < NP; p++) inv_a2_store(v[p], lds + p * N, r, m0);
   __syncthreads();
@@ -170,10 +95,6 @@ __device__ __forceinline__ void inv_passes_tw(uint64_t (*v)[8], uint64_t* lds, c
     inv_a1_load(v[p], lds + p * N, r, m0);
     inv_a1_compute(v[p]);
   }
-}
-template <int NP>
-__device__ __forceinline__ void inv_passes(uint64_t (*v)[8], uint64_t* lds, const uint64_t* tab, int t) {
-  inv_passes_tw<NP>(v, lds, TabTw{tab}, t);
 }
 
 // mode 0: forward (out NTT domain), 1: inverse (in NTT domain), 2: polymul
@@ -466,45 +387,130 @@ __global__ __launch_bounds__(512, TP ? 2 : 1) void k_blind_rotate_lat(const uint
 
 // ------------------------------------------------------------------------------------
 // Split variant of the blind rotation for the smallest cohorts: ciphertext c runs on a pair
-// of 256-thread workgroups, one per GLWE polynomial, so that each CU carries half of every
-// CMUX: the rotation and decomposition of its polynomial, the forward NTTs of its L digit
-// polynomials, output column h of the external product and its inverse NTT.  The product
-// also needs the partner's digits: thread t of one workgroup sends its NTT-domain digits to
-// thread t of the other through global memory, as 64-bit words of 32 data bits under a
-// 32-bit tag (CMUX index + 1), stored and polled with relaxed agent-scope atomics.  A word
-// whose tag matches carries its own data, so neither fences nor flags are needed.  The two
-// parities of the exchange area alternate per CMUX; a workgroup overwrites parity i & 1 at
-// CMUX i + 2 only after receiving its partner's CMUX i + 1 words, which the partner sends
-// after it has read CMUX i.  The area must hold no tag of this launch when the launch
-// starts (the launcher zeroes it first).  Workgroups bid and bid ^ 8 form a pair, so that
-// round-robin dispatch over the 8 XCDs puts both on one XCD's L2.  Every wait ends: a
-// partner silent for SPLIT_WAIT_TICKS of the 100 MHz real-time clock makes the workgroup
-// set *status and leave, and its partner then does the same (outputs undefined; the host
-// reports the status as an error).  BrArgs::split_wait_us overrides the limit (0: give up at
-// the first poll that finds a word missing, which exercises this path in tests).  Arithmetic and CMUX order are those of k_blind_rotate,
-// so results are bit-identical to it.
+// of 512-thread workgroups, one per GLWE polynomial, so that each CU carries half of every
+// CMUX with two waves per SIMD: the rotation and decomposition of its polynomial, the forward
+// NTTs of its L digit polynomials (ntt2048x4.h, four values per thread), output column h of
+// the external product and its inverse NTT.  The product also needs the partner's digits:
+// thread t of one workgroup sends its NTT-domain digits to thread t of the other through
+// global memory, as 64-bit words of 32 data bits under a 32-bit tag (CMUX index + 1), stored
+// and polled with relaxed agent-scope atomics.  A word whose tag matches carries its own
+// data, so neither fences nor flags are needed.  The two parities of the exchange area
+// alternate per CMUX; a workgroup overwrites parity i & 1 at CMUX i + 2 only after receiving
+// its partner's CMUX i + 1 words, which the partner sends after it has read CMUX i.  The area
+// must hold no tag of this launch when the launch starts (the launcher zeroes it first).
+// Workgroups bid and bid ^ 8 form a pair, so that round-robin dispatch over the 8 XCDs puts
+// both on one XCD's L2.  Every wait ends: a partner silent for SPLIT_WAIT_TICKS of the
+// 100 MHz real-time clock makes the workgroup set *status and leave, and its partner then
+// does the same (outputs undefined; the host reports the status as an error).
+// BrArgs::split_wait_us overrides the limit (0: give up at the first poll that finds a word
+// missing, which exercises this path in tests).  Arithmetic and CMUX order are those of
+// k_blind_rotate, so results are bit-identical to it.
 // ------------------------------------------------------------------------------------
 constexpr uint64_t SPLIT_WAIT_TICKS = 200000000ull;  // 2 s
 
 template <int L>
-constexpr size_t split_words() {  // [parity 2][side 2][L][w0 8][half 2][T]
-  return (size_t)2 * 2 * L * 16 * T;
+constexpr size_t split_words() {  // [parity 2][side 2][L][j 4][half 2][512 threads]
+  return (size_t)2 * 2 * L * 8 * ntt4::T4;
 }
 
 __device__ __forceinline__ void split_put(uint64_t* p, uint64_t tag, uint32_t x) {
   __hip_atomic_store(p, tag | x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+__global__ __launch_bounds__(256) void k_zero_words(uint64_t* __restrict__ p, size_t n) {
+  for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (size_t)gridDim.x * 256) p[e] = 0;
+}
+
+// 512-thread passes (ntt2048x4.h) over NP polynomials, with the two LDS buffers p and q used
+// alternately (one barrier per pass): A1a -> q, A1b -> p, A2 -> q, C1 -> p, C2 -> q, C3 reads q.
+// Entry: no thread still reads q, and p only by reads that precede the first barrier here.
+template <int NP, bool PRE>
+__device__ __forceinline__ void fwd_passes_x4(uint64_t (*v)[4], uint64_t* p, uint64_t* q, const ntt4::Tw4& tw, int t) {
+  const ntt4::Map m(t);
+  const int W = wave_uniform(t >> 6);
+#pragma unroll
+  for (int pp = 0; pp < NP; pp++) ntt4::fwd_a1a_compute<PRE>(v[pp], W >> 2);
+#pragma unroll
+  for (int pp = 0; pp < NP; pp++) ntt4::fwd_a1a_store(v[pp], q + pp * N, m);
+  __syncthreads();
+#pragma unroll
+  for (int pp = 0; pp < NP; pp++) {
+    ntt4::fwd_a1b_load(v[pp], q + pp * N, m);
+    ntt4::fwd_a1b_compute(v[pp], W);
+  }
+#pragma unroll
+  for (int pp = 0; pp < NP; pp++) ntt4::fwd_a1b_store(v[pp], p + pp * N, m);
+  __syncthreads();
+#pragma unroll
+  for (int pp = 0; pp < NP; pp++) {
+    ntt4::fwd_a2_load(v[pp], p + pp * N, m);
+    ntt4::fwd_a2_compute(v[pp], tw.tw);
+  }
+#pragma unroll
+  for (int pp = 0; pp < NP; pp++) ntt4::fwd_a2_store(v[pp], q + pp * N, m);
+  __syncthreads();
+#pragma unroll
+  for (int pp = 0; pp < NP; pp++) {
+    ntt4::fwd_c1_load(v[pp], q + pp * N, m);
+    ntt4::fwd_c1_compute(v[pp], tw.c1);
+  }
+#pragma unroll
+  for (int pp = 0; pp < NP; pp++) ntt4::fwd_c1_store(v[pp], p + pp * N, m);
+  __syncthreads();
+#pragma unroll
+  for (int pp = 0; pp < NP; pp++) {
+    ntt4::fwd_c2_load(v[pp], p + pp * N, m);
+    ntt4::fwd_c2_compute(v[pp], W & 3);
+  }
+#pragma unroll
+  for (int pp = 0; pp < NP; pp++) ntt4::fwd_c2_store(v[pp], q + pp * N, m);
+  __syncthreads();
+#pragma unroll
+  for (int pp = 0; pp < NP; pp++) {
+    ntt4::fwd_c3_load(v[pp], q + pp * N, m);
+    ntt4::fwd_c3_compute(v[pp]);
+  }
+}
+// The inverse of one polynomial: IC3 -> p, IC2 -> q, IC1 -> p, IA2 -> q, IA1b -> p, IA1a reads
+// p.  Entry: no thread still reads p, and q only by reads that precede the first barrier here.
+__device__ __forceinline__ void inv_passes_x4(uint64_t* v, uint64_t* p, uint64_t* q, const ntt4::Tw4& tw, int t) {
+  const ntt4::Map m(t);
+  const int W = wave_uniform(t >> 6);
+  ntt4::inv_c3_compute(v);
+  ntt4::inv_c3_store(v, p, m);
+  __syncthreads();
+  ntt4::inv_c2_load(v, p, m);
+  ntt4::inv_c2_compute(v, W & 3);
+  ntt4::inv_c2_store(v, q, m);
+  __syncthreads();
+  ntt4::inv_c1_load(v, q, m);
+  ntt4::inv_c1_compute(v, tw.ic1, tw.itw);
+  ntt4::inv_c1_store(v, p, m);
+  __syncthreads();
+  ntt4::inv_a2_load(v, p, m);
+  ntt4::inv_a2_compute(v, W);
+  ntt4::inv_a2_store(v, q, m);
+  __syncthreads();
+  ntt4::inv_a1b_load(v, q, m);
+  ntt4::inv_a1b_compute(v);
+  ntt4::inv_a1b_store(v, p, m);
+  __syncthreads();
+  ntt4::inv_a1a_load(v, p, m);
+  ntt4::inv_a1a_compute(v, W >> 2);
+}
+
+// The split kernel's workgroup: 512 threads on one GLWE polynomial.  Thread t holds
+// NTT-domain indices 64k + w1 + 8 (wd + 2j) (ntt2048x4.h's C3 map), so it reads the key words
+// of thread t & 255 of the blind-rotation key layout, rows w0 = wd + 2j, and exchanges 8 L
+// words with thread t of its partner.
 template <int L, bool HI>
-__global__ __launch_bounds__(256, 1) void k_blind_rotate_split(const uint64_t* __restrict__ bsk,
-                                                              const uint64_t* __restrict__ luts,
-                                                              const uint16_t* __restrict__ lwe2n,
-                                                              const PbsJob* __restrict__ jobs, int nct,
-                                                              const uint64_t* __restrict__ tab, BrArgs a,
-                                                              uint64_t* __restrict__ xch, uint32_t* status) {
-  constexpr int K1 = 2, NPF = K1 * L;
-  // this polynomial's accumulator and digit NTT work space, two buffers used alternately
-  // (each store goes to the buffer whose readers have passed the last barrier)
+__global__ __launch_bounds__(512, 1) void k_blind_rotate_split(const uint64_t* __restrict__ bsk,
+                                                               const uint64_t* __restrict__ luts,
+                                                               const uint16_t* __restrict__ lwe2n,
+                                                               const PbsJob* __restrict__ jobs, int nct,
+                                                               const uint64_t* __restrict__ tab, BrArgs a,
+                                                               uint64_t* __restrict__ xch, uint32_t* status) {
+  constexpr int K1 = 2, NPF = K1 * L, T4 = ntt4::T4;
   __shared__ uint64_t lds[2 * L * N];
   __shared__ int abort_;
   const int bid = blockIdx.x;
@@ -514,99 +520,92 @@ __global__ __launch_bounds__(256, 1) void k_blind_rotate_split(const uint64_t* _
   const PbsJob job = jobs[c];
   if (job.out_addr == 0) return;  // an empty slot of a batch list
   const int t = threadIdx.x;
-  const int r = t & 63, m0 = t >> 6;
+  const int r = t & 63, m0 = (t >> 6) & 3, g = t >> 8;
   const size_t bsk_stride = (size_t)8 * NPF * K1 * T;
   const __amdgpu_buffer_rsrc_t bsk_rsrc = __builtin_amdgcn_make_buffer_rsrc(
       (void*)bsk, (short)0, (int)((size_t)a.n_lwe * bsk_stride * 8), 0x00020000);
   const uint16_t* a2n = lwe2n + (size_t)c * a.lwe_stride;
   const uint64_t* lut = luts + (size_t)job.gid * K1 * N;
   uint64_t* xc = xch + (size_t)c * split_words<L>();
-  RegTw twr;
-  twr.load(tab, t);
+  ntt4::Tw4 tw;
+  tw.load(tab, t);
   if (t == 0) abort_ = 0;
   const uint64_t wait_ticks = a.split_wait_us < 0 ? SPLIT_WAIT_TICKS : (uint64_t)a.split_wait_us * 100;
-  uint64_t acc[8];
+  uint64_t acc[4];
 #pragma unroll
-  for (int m1 = 0; m1 < 8; m1++) acc[m1] = lut[h * N + 64 * (m0 + 4 * m1) + r];
+  for (int i = 0; i < 4; i++) acc[i] = lut[h * N + 64 * (m0 + 4 * (2 * g + (i & 1) + 4 * (i >> 1))) + r];
   uint32_t rot_next = a2n[0];
-  // Buffers per CMUX (x: accumulator, y: the other): acc -> x | rotation reads x | forward
-  // A1 -> y, A2 -> x, C1 -> y | inverse C2 -> x, C1 -> y, A2 -> x | A1 reads x; the next
-  // CMUX swaps x and y.  Every store's buffer was last read before the preceding barrier.
+  // Buffers per CMUX (x: accumulator, y: the other), one barrier per pass: acc -> x |
+  // rotation reads x | forward A1a -> y, A1b -> x, A2 -> y, C1 -> x, C2 -> y, C3 reads y |
+  // inverse IC3 -> x, IC2 -> y, IC1 -> x, IA2 -> y, IA1b -> x, IA1a reads x; the next CMUX
+  // swaps x and y.  Every store's buffer was last read before the preceding barrier.
   uint64_t* bx = lds;
   uint64_t* by = lds + L * N;
   __syncthreads();  // abort_ initialised
   for (int i = 0; i < a.n_lwe; i++) {
     const uint32_t rot = rot_next;
     rot_next = a2n[i + 1];  // i + 1 <= n_lwe: the body slot, always present
-    int tt = t;
-    asm volatile("" : "+v"(tt));
-    const int rr = tt & 63, mm0 = tt >> 6;
-    // set before the last barrier of the previous CMUX (or the one above), so uniform here
-    if (abort_) break;
-    // this CMUX's key words of output column h, in flight during the rotation and forward NTT:
-    // kw[w0][0][l] for this workgroup's rows h * L + l, kw[w0][1][l] for the partner's
-    // (scalar offsets, so that no register array is indexed by h)
+    if (abort_) break;  // set before the last barrier of the previous CMUX: uniform here
+    // key words of output column h for this thread's rows w0 = wd + 2j: own rows first
     const int slice_off = (int)((size_t)i * bsk_stride * 8);
-    const int lane_off = tt * 8;
-    uint64_t kw[8][2][L];
+    const int lane_off = (t & 255) * 8;
+    uint64_t kw[4][2][L];
 #pragma unroll
-    for (int w0 = 0; w0 < 8; w0++)
+    for (int j = 0; j < 4; j++)
 #pragma unroll
       for (int side = 0; side < 2; side++)
 #pragma unroll
         for (int l = 0; l < L; l++) {
-          const int g = side ? 1 - h : h;
-          const int e = (w0 * NPF + g * L + l) * K1 + h;
+          const int gg = side ? 1 - h : h;
+          const int e = ((g + 2 * j) * NPF + gg * L + l) * K1 + h;
           const auto w = __builtin_amdgcn_raw_buffer_load_b64(bsk_rsrc, lane_off, slice_off + e * T * 8, 0);
-          kw[w0][side][l] = ((uint64_t)w[1] << 32) | w[0];
+          kw[j][side][l] = ((uint64_t)w[1] << 32) | w[0];
         }
 #pragma unroll
-    for (int m1 = 0; m1 < 8; m1++) bx[64 * (mm0 + 4 * m1) + rr] = acc[m1];
+    for (int k = 0; k < 4; k++) bx[64 * (m0 + 4 * (2 * g + (k & 1) + 4 * (k >> 1))) + r] = acc[k];
     __syncthreads();
-    uint64_t v[L][8];
-    pbs::rotate_decompose<L, HI>(bx, acc, tt, rot, a.base_log, v);
-    fwd_passes_tw<L, HI, RegTw, true>(v, by, twr, tt, bx);
+    uint64_t v[L][4];
+    pbs::rotate_decompose4<L, HI>(bx, acc, t, rot, a.base_log, v);
+    fwd_passes_x4<L, HI>(v, bx, by, tw, t);
     // send this polynomial's digit rows h * L + l
     const uint64_t tag = (uint64_t)(i + 1) << 32;
-    uint64_t* snd = xc + (size_t)((i & 1) * 2 + h) * L * 16 * T + tt;
+    uint64_t* snd = xc + (size_t)((i & 1) * 2 + h) * L * 8 * T4 + t;
 #pragma unroll
     for (int l = 0; l < L; l++)
 #pragma unroll
-      for (int w0 = 0; w0 < 8; w0++) {
-        split_put(snd + ((l * 8 + w0) * 2) * T, tag, (uint32_t)v[l][w0]);
-        split_put(snd + ((l * 8 + w0) * 2 + 1) * T, tag, (uint32_t)(v[l][w0] >> 32));
+      for (int j = 0; j < 4; j++) {
+        split_put(snd + ((l * 4 + j) * 2) * T4, tag, (uint32_t)v[l][j]);
+        split_put(snd + ((l * 4 + j) * 2 + 1) * T4, tag, (uint32_t)(v[l][j] >> 32));
       }
-    // own rows of the product while the partner's words travel: 128-bit sums per output
-    uint64_t lo[8], hi[8];
-    uint32_t cc[8];
+    uint64_t lo[4], hi[4];
+    uint32_t cc[4];
 #pragma unroll
-    for (int w0 = 0; w0 < 8; w0++) {
-      cc[w0] = 0;
+    for (int j = 0; j < 4; j++) {
+      cc[j] = 0;
 #pragma unroll
       for (int l = 0; l < L; l++) {
         uint64_t plo, phi;
-        gl::mul_full(v[l][w0], kw[w0][0][l], plo, phi);
+        gl::mul_full(v[l][j], kw[j][0][l], plo, phi);
         if (l == 0)
-          lo[w0] = plo, hi[w0] = phi;
+          lo[j] = plo, hi[j] = phi;
         else
-          gl::acc128(lo[w0], hi[w0], cc[w0], plo, phi);
+          gl::acc128(lo[j], hi[j], cc[j], plo, phi);
       }
     }
-    // the partner's rows (1 - h) * L + l
-    const uint64_t* rcv = xc + (size_t)((i & 1) * 2 + (1 - h)) * L * 16 * T + tt;
-    uint64_t got[L][8][2];
+    const uint64_t* rcv = xc + (size_t)((i & 1) * 2 + (1 - h)) * L * 8 * T4 + t;
+    uint64_t got[L][4][2];
     uint64_t deadline = 0;
     for (;;) {
       bool ready = true;
 #pragma unroll
       for (int l = 0; l < L; l++)
 #pragma unroll
-        for (int w0 = 0; w0 < 8; w0++)
+        for (int j = 0; j < 4; j++)
 #pragma unroll
           for (int q = 0; q < 2; q++) {
-            got[l][w0][q] = __hip_atomic_load(rcv + ((l * 8 + w0) * 2 + q) * T, __ATOMIC_RELAXED,
-                                              __HIP_MEMORY_SCOPE_AGENT);
-            ready &= (got[l][w0][q] >> 32) == (uint64_t)(i + 1);
+            got[l][j][q] = __hip_atomic_load(rcv + ((l * 4 + j) * 2 + q) * T4, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+            ready &= (got[l][j][q] >> 32) == (uint64_t)(i + 1);
           }
       if (ready) break;
       const uint64_t now = __builtin_amdgcn_s_memrealtime();
@@ -618,20 +617,19 @@ __global__ __launch_bounds__(256, 1) void k_blind_rotate_split(const uint64_t* _
       }
     }
 #pragma unroll
-    for (int w0 = 0; w0 < 8; w0++)
+    for (int j = 0; j < 4; j++)
 #pragma unroll
       for (int l = 0; l < L; l++) {
-        const uint64_t d = (got[l][w0][1] << 32) | (uint32_t)got[l][w0][0];
+        const uint64_t d = (got[l][j][1] << 32) | (uint32_t)got[l][j][0];
         uint64_t plo, phi;
-        gl::mul_full(d, kw[w0][1][l], plo, phi);
-        gl::acc128(lo[w0], hi[w0], cc[w0], plo, phi);
+        gl::mul_full(d, kw[j][1][l], plo, phi);
+        gl::acc128(lo[j], hi[j], cc[j], plo, phi);
       }
-    uint64_t z[1][8];
-    gl::reduce160x4(z[0], lo, hi, cc);
-    gl::reduce160x4(z[0] + 4, lo + 4, hi + 4, cc + 4);
-    inv_passes_tw<1, RegTw, true>(z, bx, twr, tt, by);
+    uint64_t z[4];
+    gl::reduce160x4(z, lo, hi, cc);
+    inv_passes_x4(z, bx, by, tw, t);
 #pragma unroll
-    for (int m1 = 0; m1 < 8; m1++) acc[m1] += gl::modswitch_p_to_q(z[0][m1]);
+    for (int k = 0; k < 4; k++) acc[k] += gl::modswitch_p_to_q(z[k]);
     uint64_t* sw = bx;
     bx = by;
     by = sw;
@@ -640,7 +638,7 @@ __global__ __launch_bounds__(256, 1) void k_blind_rotate_split(const uint64_t* _
   if (abort_) return;
   // sample extract: the mask from polynomial 0, the body from polynomial 1
 #pragma unroll
-  for (int m1 = 0; m1 < 8; m1++) lds[64 * (m0 + 4 * m1) + r] = acc[m1];
+  for (int k = 0; k < 4; k++) lds[64 * (m0 + 4 * (2 * g + (k & 1) + 4 * (k >> 1))) + r] = acc[k];
   __syncthreads();
   const uint32_t b = a2n[a.n_lwe];
   const size_t blen = (size_t)(K1 - 1) * N + 1;
@@ -650,9 +648,9 @@ __global__ __launch_bounds__(256, 1) void k_blind_rotate_split(const uint64_t* _
     uint64_t* o = reinterpret_cast<uint64_t*>(job.out_addr) + (size_t)lu * blen;
     if (h == 0) {
 #pragma unroll
-      for (int e = 0; e < N / T; e++) {
-        const int m = t + T * e;
-        o[m] = pbs::extract_coef(lds, 0, m, b, offs);
+      for (int e = 0; e < N / T4; e++) {
+        const int mm = t + T4 * e;
+        o[mm] = pbs::extract_coef(lds, 0, mm, b, offs);
       }
     } else if (t == 0) {
       o[N] = pbs::extract_coef(lds, 0, 0, b, offs);
@@ -660,9 +658,6 @@ __global__ __launch_bounds__(256, 1) void k_blind_rotate_split(const uint64_t* _
   }
 }
 
-__global__ __launch_bounds__(256) void k_zero_words(uint64_t* __restrict__ p, size_t n) {
-  for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (size_t)gridDim.x * 256) p[e] = 0;
-}
 
 // ------------------------------------------------------------------------------------
 // ALU DOps (pea_alu_core.sv:55-160) and LD/ST row copies.
@@ -878,8 +873,8 @@ static void br_launch(const uint64_t* bsk, const uint64_t* luts, const uint16_t*
   if (xch) {
     const size_t words = (size_t)nct * split_words<L>();
     k_zero_words<<<dim3((unsigned)std::min<size_t>((words + 255) / 256, 1024)), 256, 0, s>>>(xch, words);
-    k_blind_rotate_split<L, HI><<<dim3((unsigned)(16 * ((nct + 7) / 8))), 256, 0, s>>>(bsk, luts, lwe2n, jobs, nct,
-                                                                                        tab, a, xch, status);
+    k_blind_rotate_split<L, HI><<<dim3((unsigned)(16 * ((nct + 7) / 8))), 512, 0, s>>>(bsk, luts, lwe2n, jobs, nct,
+                                                                                         tab, a, xch, status);
   } else if (latency) {
     int g = br_lat_max_grid<2, L, false>();
     g = g < nct ? g : nct;

@@ -119,6 +119,43 @@ GL_HD void rotate_decompose_from(const uint64_t* acc_lds, const uint64_t* acc_re
   }
 }
 
+// rotate_decompose for the 512-thread layout (ntt2048x4.h A1a map: thread (r, m0, g) owns
+// coefficients 64 (m0 + 4 m1) + r for m1 = 2g, 2g+1, 2g+4, 2g+5, in that register order);
+// g is uniform per wave, so the pre-twist exponents stay compile-time.
+template <int L, bool HI, int G, int I = 0>
+GL_HD void rotate_decompose4_from(const uint64_t* acc_lds, const uint64_t* acc_reg, int r, int m0, uint32_t a,
+                                  uint32_t base_log, uint64_t (*out)[4]) {
+  if constexpr (I < 4) {
+    constexpr int m1 = 2 * G + (I & 1) + 4 * (I >> 1);
+    const int n = 64 * (m0 + 4 * m1) + r;
+    const uint32_t idx = ((uint32_t)n + 2 * N - a) & (2 * N - 1);
+    uint64_t val = acc_lds[idx & (N - 1)];
+    val = (idx >= (uint32_t)N) ? (0 - val) : val;
+    const uint64_t ct10 = val - acc_reg[I];
+    if constexpr (HI) {
+      int32_t d[L];
+      decompose_hi<L>((uint32_t)(ct10 >> 32), base_log, d);
+#pragma unroll
+      for (int l = 0; l < L; l++) out[l][I] = gl::pretwist_digit<m1>(d[l]);
+    } else {
+      int64_t d[L];
+      decompose<L>(ct10, base_log, d);
+#pragma unroll
+      for (int l = 0; l < L; l++) out[l][I] = gl::from_signed(d[l]);
+    }
+    rotate_decompose4_from<L, HI, G, I + 1>(acc_lds, acc_reg, r, m0, a, base_log, out);
+  }
+}
+template <int L, bool HI>
+GL_HD void rotate_decompose4(const uint64_t* acc_lds, const uint64_t* acc_reg, int t, uint32_t a, uint32_t base_log,
+                             uint64_t (*out)[4]) {
+  const int r = t & 63, m0 = (t >> 6) & 3;
+  if (ntt::wave_uniform(t >> 8) == 0)
+    rotate_decompose4_from<L, HI, 0>(acc_lds, acc_reg, r, m0, a, base_log, out);
+  else
+    rotate_decompose4_from<L, HI, 1>(acc_lds, acc_reg, r, m0, a, base_log, out);
+}
+
 // Pointwise external product for one thread: z[gl][w0] are the NTT-domain digits of
 // row gl = g*L + l; the key slice of this CMUX is in the internal layout [w0][K1*L][K1][T],
 // and key(e) returns this thread's word e*T + t of it.  Result written over z[j][w0] for

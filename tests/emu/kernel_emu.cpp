@@ -263,3 +263,248 @@ extern "C" long emu_pretwist_mismatches() {
   return pretwist_check<0>() + pretwist_check<1>() + pretwist_check<2>() + pretwist_check<3>() +
          pretwist_check<4>() + pretwist_check<5>() + pretwist_check<6>() + pretwist_check<7>();
 }
+
+// ---------------------------------------------------------------- the 512-thread NTT
+#include "../../luthe_amd/csrc/ntt2048x4.h"
+
+namespace {
+struct Tw4All {
+  std::vector<ntt4::Tw4> tw;
+  Tw4All() : tw(ntt4::T4) {
+    for (int t = 0; t < ntt4::T4; t++) tw[t].load(tables().tab, t);
+  }
+};
+const Tw4All& tw4() {
+  static Tw4All t;
+  return t;
+}
+}  // namespace
+
+extern "C" {
+
+// Forward NTT of one polynomial through the 512-thread passes: in coefficient order, out in
+// NTT-domain order (index 64k + w), as emu_ntt_fwd.  pretwisted: the input already carries
+// psi^(4 m1) (the digit lift's form).
+void emu_ntt4_fwd(const uint64_t* in, uint64_t* out, int pretwisted) {
+  const Tw4All& tw = tw4();
+  std::vector<uint64_t> lds(ntt::N);
+  static uint64_t v[ntt4::T4][4];
+  for (int t = 0; t < ntt4::T4; t++) {
+    const ntt4::Map m(t);
+    const int r = m.r(), m0 = m.m0(), g = m.hi2();
+    const int m1s[4] = {2 * g, 2 * g + 1, 2 * g + 4, 2 * g + 5};
+    for (int i = 0; i < 4; i++) {
+      uint64_t x = in[64 * (m0 + 4 * m1s[i]) + r];
+      v[t][i] = pretwisted ? gl::mul_pow2(x, 12 * m1s[i]) : x;
+    }
+    if (pretwisted)
+      ntt4::fwd_a1a_compute<true>(v[t], g);
+    else
+      ntt4::fwd_a1a_compute<false>(v[t], g);
+  }
+  for (int t = 0; t < ntt4::T4; t++) ntt4::fwd_a1a_store(v[t], lds.data(), ntt4::Map(t));
+  for (int t = 0; t < ntt4::T4; t++) {
+    ntt4::fwd_a1b_load(v[t], lds.data(), ntt4::Map(t));
+    ntt4::fwd_a1b_compute(v[t], ntt4::Map(t).W);
+  }
+  for (int t = 0; t < ntt4::T4; t++) ntt4::fwd_a1b_store(v[t], lds.data(), ntt4::Map(t));
+  for (int t = 0; t < ntt4::T4; t++) {
+    ntt4::fwd_a2_load(v[t], lds.data(), ntt4::Map(t));
+    ntt4::fwd_a2_compute(v[t], tw.tw[t].tw);
+  }
+  for (int t = 0; t < ntt4::T4; t++) ntt4::fwd_a2_store(v[t], lds.data(), ntt4::Map(t));
+  for (int t = 0; t < ntt4::T4; t++) {
+    ntt4::fwd_c1_load(v[t], lds.data(), ntt4::Map(t));
+    ntt4::fwd_c1_compute(v[t], tw.tw[t].c1);
+  }
+  for (int t = 0; t < ntt4::T4; t++) ntt4::fwd_c1_store(v[t], lds.data(), ntt4::Map(t));
+  for (int t = 0; t < ntt4::T4; t++) {
+    ntt4::fwd_c2_load(v[t], lds.data(), ntt4::Map(t));
+    ntt4::fwd_c2_compute(v[t], ntt4::Map(t).ra());
+  }
+  for (int t = 0; t < ntt4::T4; t++) ntt4::fwd_c2_store(v[t], lds.data(), ntt4::Map(t));
+  for (int t = 0; t < ntt4::T4; t++) {
+    const ntt4::Map m(t);
+    ntt4::fwd_c3_load(v[t], lds.data(), m);
+    ntt4::fwd_c3_compute(v[t]);
+    for (int j = 0; j < 4; j++) out[64 * m.k3() + m.w1() + 8 * (m.wd() + 2 * j)] = gl::canonical(v[t][j]);
+  }
+}
+
+// Inverse: NTT-domain order in, canonical coefficients out, as emu_ntt_inv.
+void emu_ntt4_inv(const uint64_t* in, uint64_t* out) {
+  const Tw4All& tw = tw4();
+  std::vector<uint64_t> lds(ntt::N);
+  static uint64_t v[ntt4::T4][4];
+  for (int t = 0; t < ntt4::T4; t++) {
+    const ntt4::Map m(t);
+    for (int j = 0; j < 4; j++) v[t][j] = in[64 * m.k3() + m.w1() + 8 * (m.wd() + 2 * j)];
+    ntt4::inv_c3_compute(v[t]);
+  }
+  for (int t = 0; t < ntt4::T4; t++) ntt4::inv_c3_store(v[t], lds.data(), ntt4::Map(t));
+  for (int t = 0; t < ntt4::T4; t++) {
+    ntt4::inv_c2_load(v[t], lds.data(), ntt4::Map(t));
+    ntt4::inv_c2_compute(v[t], ntt4::Map(t).ra());
+  }
+  for (int t = 0; t < ntt4::T4; t++) ntt4::inv_c2_store(v[t], lds.data(), ntt4::Map(t));
+  for (int t = 0; t < ntt4::T4; t++) {
+    ntt4::inv_c1_load(v[t], lds.data(), ntt4::Map(t));
+    ntt4::inv_c1_compute(v[t], tw.tw[t].ic1, tw.tw[t].itw);
+  }
+  for (int t = 0; t < ntt4::T4; t++) ntt4::inv_c1_store(v[t], lds.data(), ntt4::Map(t));
+  for (int t = 0; t < ntt4::T4; t++) {
+    ntt4::inv_a2_load(v[t], lds.data(), ntt4::Map(t));
+    ntt4::inv_a2_compute(v[t], ntt4::Map(t).W);
+  }
+  for (int t = 0; t < ntt4::T4; t++) ntt4::inv_a2_store(v[t], lds.data(), ntt4::Map(t));
+  for (int t = 0; t < ntt4::T4; t++) {
+    ntt4::inv_a1b_load(v[t], lds.data(), ntt4::Map(t));
+    ntt4::inv_a1b_compute(v[t]);
+  }
+  for (int t = 0; t < ntt4::T4; t++) ntt4::inv_a1b_store(v[t], lds.data(), ntt4::Map(t));
+  for (int t = 0; t < ntt4::T4; t++) {
+    const ntt4::Map m(t);
+    ntt4::inv_a1a_load(v[t], lds.data(), m);
+    ntt4::inv_a1a_compute(v[t], m.hi2());
+    const int r = m.r(), m0 = m.m0(), g = m.hi2();
+    const int m1s[4] = {2 * g, 2 * g + 1, 2 * g + 4, 2 * g + 5};
+    for (int i = 0; i < 4; i++) out[64 * (m0 + 4 * m1s[i]) + r] = v[t][i];
+  }
+}
+}
+
+// ---------------------------------------------------------------- the 512-thread blind rotation
+// The split kernel's per-thread CMUX (k_blind_rotate_split): both GLWE polynomials of one
+// ciphertext, each on 512 "threads" with four values, the key read in the 256-thread internal
+// layout (thread t & 255, rows w0 = (t >> 8) + 2j), the partner's digits taken directly.
+namespace {
+template <int L>
+void blind_rotate_split(int n, const uint64_t* bsk_int, const uint64_t* lut, const uint16_t* a2n, uint32_t base_log,
+                   uint64_t* acc_out) {
+  constexpr int K1 = 2, NPF = K1 * L, T4 = ntt4::T4;
+  const bool hi = L * base_log <= 31 && base_log <= 23;
+  const Tw4All& tw = tw4();
+  std::vector<uint64_t> lds(ntt::N);
+  static uint64_t acc[K1][T4][4], v[NPF][T4][4], z[K1][T4][4];
+  auto coef = [](int t, int i) {
+    const int g = t >> 8;
+    return 64 * (((t >> 6) & 3) + 4 * (2 * g + (i & 1) + 4 * (i >> 1))) + (t & 63);
+  };
+  for (int p = 0; p < K1; p++)
+    for (int t = 0; t < T4; t++)
+      for (int i = 0; i < 4; i++) acc[p][t][i] = lut[p * ntt::N + coef(t, i)];
+  for (int it = 0; it < n; it++) {
+    const uint32_t a = a2n[it];
+    for (int p = 0; p < K1; p++) {
+      for (int t = 0; t < T4; t++)
+        for (int i = 0; i < 4; i++) lds[coef(t, i)] = acc[p][t][i];
+      for (int t = 0; t < T4; t++) {
+        uint64_t d[L][4];
+        if (hi)
+          pbs::rotate_decompose4<L, true>(lds.data(), acc[p][t], t, a, base_log, d);
+        else
+          pbs::rotate_decompose4<L, false>(lds.data(), acc[p][t], t, a, base_log, d);
+        for (int l = 0; l < L; l++)
+          for (int i = 0; i < 4; i++) v[p * L + l][t][i] = d[l][i];
+      }
+    }
+    // forward: one digit polynomial at a time through the six passes
+    for (int q = 0; q < NPF; q++) {
+      for (int t = 0; t < T4; t++) {
+        if (hi)
+          ntt4::fwd_a1a_compute<true>(v[q][t], t >> 8);
+        else
+          ntt4::fwd_a1a_compute<false>(v[q][t], t >> 8);
+      }
+      for (int t = 0; t < T4; t++) ntt4::fwd_a1a_store(v[q][t], lds.data(), ntt4::Map(t));
+      for (int t = 0; t < T4; t++) {
+        ntt4::fwd_a1b_load(v[q][t], lds.data(), ntt4::Map(t));
+        ntt4::fwd_a1b_compute(v[q][t], t >> 6);
+      }
+      for (int t = 0; t < T4; t++) ntt4::fwd_a1b_store(v[q][t], lds.data(), ntt4::Map(t));
+      for (int t = 0; t < T4; t++) {
+        ntt4::fwd_a2_load(v[q][t], lds.data(), ntt4::Map(t));
+        ntt4::fwd_a2_compute(v[q][t], tw.tw[t].tw);
+      }
+      for (int t = 0; t < T4; t++) ntt4::fwd_a2_store(v[q][t], lds.data(), ntt4::Map(t));
+      for (int t = 0; t < T4; t++) {
+        ntt4::fwd_c1_load(v[q][t], lds.data(), ntt4::Map(t));
+        ntt4::fwd_c1_compute(v[q][t], tw.tw[t].c1);
+      }
+      for (int t = 0; t < T4; t++) ntt4::fwd_c1_store(v[q][t], lds.data(), ntt4::Map(t));
+      for (int t = 0; t < T4; t++) {
+        ntt4::fwd_c2_load(v[q][t], lds.data(), ntt4::Map(t));
+        ntt4::fwd_c2_compute(v[q][t], (t >> 6) & 3);
+      }
+      for (int t = 0; t < T4; t++) ntt4::fwd_c2_store(v[q][t], lds.data(), ntt4::Map(t));
+      for (int t = 0; t < T4; t++) {
+        ntt4::fwd_c3_load(v[q][t], lds.data(), ntt4::Map(t));
+        ntt4::fwd_c3_compute(v[q][t]);
+      }
+    }
+    // external product: output column j of thread t, rows w0 = (t >> 8) + 2 i of key thread t & 255
+    const uint64_t* bsk_i = bsk_int + (size_t)it * 8 * NPF * K1 * ntt::T;
+    for (int j = 0; j < K1; j++)
+      for (int t = 0; t < T4; t++)
+        for (int i = 0; i < 4; i++) {
+          uint64_t lo = 0, hi_ = 0;
+          uint32_t c = 0;
+          for (int q = 0; q < NPF; q++) {
+            const int e = (((t >> 8) + 2 * i) * NPF + q) * K1 + j;
+            uint64_t plo, phi;
+            gl::mul_full(v[q][t][i], bsk_i[(size_t)e * ntt::T + (t & 255)], plo, phi);
+            if (q == 0)
+              lo = plo, hi_ = phi;
+            else
+              gl::acc128(lo, hi_, c, plo, phi);
+          }
+          z[j][t][i] = gl::reduce160(lo, hi_, c);
+        }
+    for (int j = 0; j < K1; j++) {
+      for (int t = 0; t < T4; t++) {
+        ntt4::inv_c3_compute(z[j][t]);
+        ntt4::inv_c3_store(z[j][t], lds.data(), ntt4::Map(t));
+      }
+      for (int t = 0; t < T4; t++) {
+        ntt4::inv_c2_load(z[j][t], lds.data(), ntt4::Map(t));
+        ntt4::inv_c2_compute(z[j][t], (t >> 6) & 3);
+      }
+      for (int t = 0; t < T4; t++) ntt4::inv_c2_store(z[j][t], lds.data(), ntt4::Map(t));
+      for (int t = 0; t < T4; t++) {
+        ntt4::inv_c1_load(z[j][t], lds.data(), ntt4::Map(t));
+        ntt4::inv_c1_compute(z[j][t], tw.tw[t].ic1, tw.tw[t].itw);
+      }
+      for (int t = 0; t < T4; t++) ntt4::inv_c1_store(z[j][t], lds.data(), ntt4::Map(t));
+      for (int t = 0; t < T4; t++) {
+        ntt4::inv_a2_load(z[j][t], lds.data(), ntt4::Map(t));
+        ntt4::inv_a2_compute(z[j][t], t >> 6);
+      }
+      for (int t = 0; t < T4; t++) ntt4::inv_a2_store(z[j][t], lds.data(), ntt4::Map(t));
+      for (int t = 0; t < T4; t++) {
+        ntt4::inv_a1b_load(z[j][t], lds.data(), ntt4::Map(t));
+        ntt4::inv_a1b_compute(z[j][t]);
+      }
+      for (int t = 0; t < T4; t++) ntt4::inv_a1b_store(z[j][t], lds.data(), ntt4::Map(t));
+      for (int t = 0; t < T4; t++) {
+        ntt4::inv_a1a_load(z[j][t], lds.data(), ntt4::Map(t));
+        ntt4::inv_a1a_compute(z[j][t], t >> 8);
+        for (int i = 0; i < 4; i++) acc[j][t][i] += gl::modswitch_p_to_q(z[j][t][i]);
+      }
+    }
+  }
+  for (int p = 0; p < K1; p++)
+    for (int t = 0; t < T4; t++)
+      for (int i = 0; i < 4; i++) acc_out[p * ntt::N + coef(t, i)] = acc[p][t][i];
+}
+}  // namespace
+
+extern "C" int emu_blind_rotate_split(int n, int L, const uint64_t* bsk_int, const uint64_t* lut, const uint16_t* a2n,
+                                 uint32_t base_log, uint64_t* acc_out) {
+  if (L == 1)
+    blind_rotate_split<1>(n, bsk_int, lut, a2n, base_log, acc_out);
+  else if (L == 2)
+    blind_rotate_split<2>(n, bsk_int, lut, a2n, base_log, acc_out);
+  else
+    return -1;
+  return 0;
+}

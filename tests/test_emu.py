@@ -106,6 +106,54 @@ def test_kernel_blind_rotation_matches_oracle(emu, L, base_log):
     assert np.array_equal(acc, exp)
 
 
+def test_ntt_512_thread_passes(emu):
+    """The 512-thread, four-values-per-thread NTT (ntt2048x4.h, the split blind rotation's
+    transform) equals the 256-thread one (itself pinned to the oracle above) word for word,
+    forward (from plain and from pretwisted coefficients) and inverse."""
+    u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+    emu.emu_ntt4_fwd.argtypes = [u64p, u64p, C.c_int]
+    emu.emu_ntt4_inv.argtypes = [u64p, u64p]
+    for trial in range(4):
+        a = rng.integers(0, P, 2048, dtype=np.uint64)
+        if trial == 1:
+            a[:] = P - 1
+        if trial == 2:
+            a[:] = 0
+            a[7] = 1
+        f, f4, f4p = np.empty_like(a), np.empty_like(a), np.empty_like(a)
+        emu.emu_ntt_fwd(a, f)
+        emu.emu_ntt4_fwd(a, f4, 0)
+        emu.emu_ntt4_fwd(a, f4p, 1)
+        assert np.array_equal(f4, f) and np.array_equal(f4p, f)
+        b, b4 = np.empty_like(a), np.empty_like(a)
+        emu.emu_ntt_inv(f, b)
+        emu.emu_ntt4_inv(f, b4)
+        assert np.array_equal(b4, b) and np.array_equal(b4, a)
+
+
+@pytest.mark.parametrize("L,base_log", [(1, 23), (2, 15), (2, 20)])
+def test_split_kernel_blind_rotation_matches_oracle(emu, L, base_log):
+    """The 512-thread split kernel's CMUX (rotation and decomposition of the four-value layout,
+    the six-pass transforms, the key read as thread t & 255 rows (t >> 8) + 2j) against the
+    oracle's blind rotation; 2^20 x 2 takes the generic decomposition and unpretwisted lift."""
+    n, K1, N = 4, 2, 2048
+    bsk = rng.integers(0, P, (n, K1, L, K1, N), dtype=np.uint64)
+    lut = rng.integers(0, 2**64, (K1, N), dtype=np.uint64)
+    a2n = rng.integers(0, 4096, n + 1, dtype=np.uint16)
+    a2n[:3] = [2047, 2048, 4095]
+    prm = ref.params(lwe_dim=n, pbs_base_log=base_log, pbs_level=L)
+    exp = ref.blind_rotate(prm, ref.bsk_to_ntt(prm, bsk.reshape(-1)), lut.reshape(-1), a2n)
+    bint = np.empty(n * 8 * K1 * L * K1 * 256, np.uint64)
+    vp = C.c_void_p
+    emu.emu_bsk_to_internal(C.c_int(n), C.c_int(L), np.ascontiguousarray(bsk.reshape(-1)).ctypes.data_as(vp),
+                            bint.ctypes.data_as(vp))
+    acc = np.empty(K1 * N, np.uint64)
+    assert emu.emu_blind_rotate_split(C.c_int(n), C.c_int(L), bint.ctypes.data_as(vp),
+                                 np.ascontiguousarray(lut.reshape(-1)).ctypes.data_as(vp),
+                                 a2n.ctypes.data_as(vp), C.c_uint32(base_log), acc.ctypes.data_as(vp)) == 0
+    assert np.array_equal(acc, exp)
+
+
 def test_pretwisted_digit_lift(emu):
     """The blind rotation lifts each digit already multiplied by the forward NTT's pre-twist
     2^(12 m1); every digit |d| <= 2^22 and every m1 against lift-then-shift (itself checked
