@@ -335,7 +335,7 @@ typedef struct luthe_tuning {
   int32_t br_pair;          /* throughput kernel: 0 = 256 threads per ciphertext (up to 4 per CU),
                                1 = 512 threads per ciphertext (2 per CU), -1 = default (1 at PBS
                                level 2, 0 at level 1) */
-  int32_t br_split_max;     /* largest cohort for the split variant (one 256-thread workgroup per
+  int32_t br_split_max;     /* largest cohort for the split variant (one 512-thread workgroup per
                                GLWE polynomial, two CUs per ciphertext), -1 = CUs / 2, 0 = off */
   int32_t br_split_wait_us; /* split variant: longest wait of a workgroup for its partner before it
                                gives up (LUTHE_ERR_HIP at the next synchronising call, outputs of

@@ -1,4 +1,4 @@
-"""GPU: the split variant of the blind rotation (one 256-thread workgroup per GLWE polynomial,
+"""GPU: the split variant of the blind rotation (one 512-thread workgroup per GLWE polynomial,
 the two exchanging NTT-domain digits through global memory once per CMUX) gives the bits of the
 512-thread latency kernel and of the oracle, at PBS level 1 and 2, with Many-LUT, for pair counts
 that do and do not fill a group of eight, over launches issued back to back on one exchange area
