@@ -840,29 +840,29 @@ hipError_t launch_bsk_prepare(const uint64_t* src, uint64_t* dst, int n_lwe, int
   return hipGetLastError();
 }
 
+// Occupancy figures are computed once per process (function-local statics: thread-safe
+// initialisation, as an attached server's thread may launch at the same time as the host).
 template <int K1, int L>
 static int br_max_grid() {
-  static int cached = 0;
-  if (!cached) {
+  static const int cached = [] {
     int dev = 0, cus = 0, per = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_blind_rotate<K1, L, true>, 256, 0);
-    cached = (cus > 0 ? cus : 256) * (per > 0 ? per : 1);
-  }
+    return (cus > 0 ? cus : 256) * (per > 0 ? per : 1);
+  }();
   return cached;
 }
 
 template <int K1, int L, bool TP>
 static int br_lat_max_grid() {
-  static int cached = 0;
-  if (!cached) {
+  static const int cached = [] {
     int dev = 0, cus = 0, per = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_blind_rotate_lat<K1, L, true, TP>, 512, 0);
-    cached = (cus > 0 ? cus : 256) * (per > 0 ? per : 1);
-  }
+    return (cus > 0 ? cus : 256) * (per > 0 ? per : 1);
+  }();
   return cached;
 }
 
@@ -918,13 +918,12 @@ hipError_t launch_blind_rotate(const uint64_t* bsk, const uint64_t* luts, const 
 size_t blind_rotate_split_words(int pbs_level) { return pbs_level == 1 ? split_words<1>() : split_words<2>(); }
 
 int blind_rotate_split_slots() {
-  static int cached = 0;
-  if (!cached) {
+  static const int cached = [] {
     int dev = 0, cus = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    cached = (cus > 0 ? cus : 256) / 2;
-  }
+    return (cus > 0 ? cus : 256) / 2;
+  }();
   return cached;
 }
 
