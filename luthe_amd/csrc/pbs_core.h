@@ -52,6 +52,14 @@ GL_HD void decompose(uint64_t x, uint32_t b, int64_t* d) {
 // of xh gives closest in its top L*b bits.  For L = 1 the digit is closest read as a
 // signed L*b-bit number, except closest = 2^(b-1) reached without rounding (top = 2^(b-1),
 // rb = 0), whose sign is 0 and whose digit stays +2^(b-1).
+//
+// For L >= 2 the carry chain is one integer addition.  With y_i = chunk_i + c_i + nmsb_i +
+// (H - 1), the chain's propagate [v_i + nmsb_i > H] is [y_i >= 2^b] and y_i < 2^(b+1), so
+// the carries c_i are those of Y = closest + sum_i (H - 1 + nmsb_i) 2^(b i), and digit i is
+// chunk i of Y minus (H - 1 + nmsb_i).  The sign (the top level's next msb) is the msb of x,
+// flipped exactly when closest = 2^(R-1): for msb = 1 the rule's (below != 0) | rb is false
+// only for top + rb = 2^(R-1); for msb = 0 its (below == all ones) & rb is true only there.
+// Y < 2^(R+1) <= 2^32.  No branch, no compare chain.
 template <int L>
 GL_HD void decompose_hi(uint32_t xh, uint32_t b, int32_t* d) {
   const uint32_t R = (uint32_t)L * b;
@@ -62,19 +70,18 @@ GL_HD void decompose_hi(uint32_t xh, uint32_t b, int32_t* d) {
     d[0] = tie ? (int32_t)(1u << (R - 1)) : sx;
   } else {
     const uint32_t closest = rounded >> (32 - R);
-    const uint32_t top = xh >> (32 - R), rb = (xh >> (31 - R)) & 1;
-    const uint32_t below_mask = (1u << (R - 1)) - 1, below = top & below_mask;
-    const uint32_t sign = (xh >> 31) ? (uint32_t)((below != 0) | rb) : (uint32_t)((below == below_mask) & rb);
-    const uint32_t bmask = (1u << b) - 1, H = 1u << (b - 1);
-    uint32_t c = 0;
+    const uint32_t sign = (xh >> 31) ^ (uint32_t)(closest == (1u << (R - 1)));
+    const uint32_t bmask = (1u << b) - 1, h1 = (1u << (b - 1)) - 1;
+    uint32_t off[L];
+    uint32_t y = closest;
 #pragma unroll
     for (int i = 0; i < L; i++) {
-      uint32_t v = ((closest >> (i * b)) & bmask) + c;
-      uint32_t nmsb = (i + 1 < L) ? ((closest >> ((i + 1) * b + b - 1)) & 1) : sign;
-      uint32_t prop = (v + nmsb) > H;
-      d[i] = (int32_t)v - (int32_t)(prop << b);
-      c = prop;
+      const uint32_t nmsb = (i + 1 < L) ? ((closest >> ((i + 2) * b - 1)) & 1) : sign;
+      off[i] = h1 + nmsb;
+      y += off[i] << (i * b);
     }
+#pragma unroll
+    for (int i = 0; i < L; i++) d[i] = (int32_t)((y >> (i * b)) & bmask) - (int32_t)off[i];
   }
 }
 

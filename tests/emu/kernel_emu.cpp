@@ -229,6 +229,27 @@ int emu_decompose_hi(uint64_t x, uint32_t b, int L, int32_t* d) {
   return 0;
 }
 uint64_t emu_from_signed32(int32_t d) { return gl::from_signed32(d); }
+// decompose_hi<L> against decompose<L> for every top-bit / round-bit pattern of an R = L*b
+// <= 22 bit closest value (bits below the round bit zero, one, or all ones); returns the
+// number of mismatching inputs, or -1 when (L, b) is out of range.
+long emu_decompose_hi_mismatches(int L, uint32_t b) {
+  const uint32_t R = (uint32_t)L * b;
+  if (L < 1 || L > 5 || b < 1 || R > 22) return -1;
+  const int ofs = 64 - (int)R;
+  long bad = 0;
+  int64_t d64[5];
+  int32_t d32[5];
+  for (uint64_t top = 0; top < (1ull << R); top++)
+    for (uint64_t rb = 0; rb < 2; rb++)
+      for (int lowk = 0; lowk < 3; lowk++) {
+        const uint64_t low = lowk == 0 ? 0 : lowk == 1 ? 1 : (1ull << (ofs - 1)) - 1;
+        const uint64_t x = (top << ofs) | (rb << (ofs - 1)) | low;
+        emu_decompose(x, b, L, d64);
+        emu_decompose_hi(x, b, L, d32);
+        for (int i = 0; i < L; i++) bad += d64[i] != (int64_t)d32[i];
+      }
+  return bad;
+}
 // pretwist_digit<m1>(d) against mul_pow2c<12 m1>(from_signed32(d)) for every |d| <= 2^22 and
 // every m1; returns the number of mismatches.
 long emu_pretwist_mismatches();

@@ -200,6 +200,13 @@ def test_kernel_decomposition_and_extract(emu):
                 emu.emu_decompose(x, b, Lv, d64.ctypes.data_as(vp))
                 assert emu.emu_decompose_hi(x, b, Lv, d32.ctypes.data_as(vp)) == 0
                 assert list(d32) == list(d64), (Lv, b, hex(x))
+    # exhaustive over every top-bit / round-bit pattern up to 22 bits of closest value
+    # (the high-word path's closed-form carry chain against the generic chain)
+    emu.emu_decompose_hi_mismatches.restype = C.c_long
+    emu.emu_decompose_hi_mismatches.argtypes = [C.c_int, C.c_uint32]
+    for Lv, bs in [(2, range(1, 12)), (3, range(1, 8)), (4, range(1, 6)), (5, range(1, 5))]:
+        for b in bs:
+            assert emu.emu_decompose_hi_mismatches(Lv, b) == 0, (Lv, b)
     # the same structured cases through the generic path against the oracle, at the
     # PBS and key-switch parameter sets
     for Lv, b in [(1, 23), (2, 15), (5, 3), (1, 5), (2, 4)]:
