@@ -310,24 +310,40 @@ GL_HD uint64_t lift64(int64_t v) {
   return (uint64_t)v + (((uint64_t)m << 32) | (m & 1u));
 }
 
+// lift64(d * f) for a signed 32-bit d and 0 < f < 2^31: the sign word of d * f is that of d,
+// so on the device the product, its sign extension and the lift's addend are one
+// v_mad_i64_i32 (d * f + (m & 1, m)), with no 64-bit shift of a sign-extended d.
+GL_HD uint64_t lift_mul(int32_t d, int32_t f) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const uint32_t m = (uint32_t)(d >> 31);
+  const uint64_t addend = ((uint64_t)m << 32) | (m & 1u);
+  uint64_t r, c;
+  asm("v_mad_i64_i32 %0, %1, %2, %3, %4" : "=v"(r), "=s"(c) : "v"(d), "s"(f), "v"(addend));
+  return r;
+#else
+  return lift64((int64_t)d * f);
+#endif
+}
+
 // d * 2^(12 M1) mod p for a digit |d| <= 2^22, canonical: the forward NTT's pre-twist
 // psi^(4 m1) (psi = 2^3) fused into the digit lift.  Products that fit a signed 64-bit word
-// are lifted directly (2^12, 2^24, 2^36); 2^72 is (d 2^8) * 2^64 == (d 2^8) * EPS, also
-// exact in 64 bits; 2^48, 2^60 and 2^84 are an exact product times a constant shift.
+// are lifted directly (2^12, 2^24, 2^36 = (d 2^8) 2^28); 2^72 is (d 2^8) * 2^64 ==
+// (d 2^8) * EPS, also exact in 64 bits; 2^48, 2^60 and 2^84 are an exact product times a
+// constant shift.
 template <int M1>
 GL_HD uint64_t pretwist_digit(int32_t d) {
   const int64_t D = d;
-  if constexpr (M1 == 0) return lift64(D);
-  else if constexpr (M1 == 1) return lift64(D * (1ll << 12));
-  else if constexpr (M1 == 2) return lift64(D * (1ll << 24));
-  else if constexpr (M1 == 3) return lift64(D * (1ll << 36));
-  else if constexpr (M1 == 4) return mul_pow2c<24>(lift64(D * (1ll << 24)));
-  else if constexpr (M1 == 5) return mul_pow2c<24>(lift64(D * (1ll << 36)));
+  if constexpr (M1 == 0) return lift_mul(d, 1);
+  else if constexpr (M1 == 1) return lift_mul(d, 1 << 12);
+  else if constexpr (M1 == 2) return lift_mul(d, 1 << 24);
+  else if constexpr (M1 == 3) return lift_mul(d * (1 << 8), 1 << 28);
+  else if constexpr (M1 == 4) return mul_pow2c<24>(lift_mul(d, 1 << 24));
+  else if constexpr (M1 == 5) return mul_pow2c<24>(lift_mul(d * (1 << 8), 1 << 28));
   else if constexpr (M1 == 6) {
     const int64_t w = D * (1ll << 8);  // |w| <= 2^30: w * EPS fits
     return lift64(w * (1ll << 32) - w);
   } else {
-    return mul_pow2c<72>(lift64(D * (1ll << 12)));
+    return mul_pow2c<72>(lift_mul(d, 1 << 12));
   }
 }
 
