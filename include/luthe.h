@@ -322,7 +322,8 @@ int luthe_get_br_latency_max(luthe_ctx* ctx);
 /* Tuning knobs and the autotuner.  luthe_autotune times, with the context's keys and
  * synthetic inputs, (1) the persistent blind-rotation width (ciphertexts resident per CU,
  * batches of `batch`), (2) the largest cohort for which the latency variant is faster,
- * (3) polynomials per NTT workgroup (1, 2, 4) and sets (4) the serving cohort size to one
+ * (3) polynomials per NTT workgroup (1, 2, 4), (4) the smallest batch for which the
+ * matrix-core key switch beats the VALU one, and sets (5) the serving cohort size to one
  * wave of the chosen width; applies them and writes them to json_path (optional).
  * luthe_load_tuning applies a saved file (refused if tuned for another PBS level).
  * No reference counterpart: the HPU's equivalents are build-time parameters
@@ -341,6 +342,10 @@ typedef struct luthe_tuning {
                                gives up (LUTHE_ERR_HIP at the next synchronising call, outputs of
                                that launch undefined); -1 = 2 s, 0 = give up at the first missing
                                word (for tests of that path).  Start from luthe_get_tuning. */
+  int32_t ks_mfma_min;      /* smallest batch whose key switch runs on the int8 matrix cores (the
+                               KSK as three balanced byte planes, exact int32 sums); smaller batches
+                               take the VALU kernel.  -1 = default; a value above every batch size
+                               turns it off.  Results are identical. */
 } luthe_tuning;
 int luthe_autotune(luthe_ctx* ctx, uint32_t batch, const char* json_path, luthe_tuning* out);
 int luthe_load_tuning(luthe_ctx* ctx, const char* json_path);

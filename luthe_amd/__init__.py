@@ -111,7 +111,7 @@ TRC_NONE, TRC_RDUNLOCK, TRC_RETIRE, TRC_REFILL, TRC_ISSUE = 0, 1, 2, 3, 4
 
 class Tuning(C.Structure):
     _fields_ = [(n, C.c_int32) for n in ("br_grid", "br_latency_max", "ntt_polys_per_wg", "cohort_max", "br_pair",
-                                              "br_split_max", "br_split_wait_us")]
+                                              "br_split_max", "br_split_wait_us", "ks_mfma_min")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

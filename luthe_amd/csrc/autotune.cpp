@@ -1,8 +1,9 @@
 // Autotuner: times candidate launch shapes on the device with the context's keys and
 // picks (1) the persistent blind-rotation width, i.e. ciphertexts resident per CU,
 // (2) the largest cohorts that run faster on the split variant (two workgroups per
-// ciphertext) and on the 512-thread latency variant, (3) the polynomials per NTT workgroup
-// and (4) the serving cohort size (one wave of the chosen width).  Results are applied to the context and persisted as JSON; luthe_load_tuning
+// ciphertext) and on the 512-thread latency variant, (3) the polynomials per NTT workgroup,
+// (4) the smallest batch for the matrix-core key switch and (5) the serving cohort size (one
+// wave of the chosen width).  Results are applied to the context and persisted as JSON; luthe_load_tuning
 // re-applies a saved file.  Inputs are synthetic: timing does not depend on values.
 #include <hip/hip_runtime.h>
 #include <stdio.h>
@@ -68,7 +69,7 @@ bool json_int(const std::string& s, const char* key, long& v) {
 
 int apply(luthe_ctx* c, const luthe_tuning& t) {
   if (t.br_grid < 0 || t.br_latency_max < -1 || t.cohort_max < 0 || t.br_pair < -1 || t.br_pair > 1 ||
-      t.br_split_max < -1 || t.br_split_wait_us < -1)
+      t.br_split_max < -1 || t.br_split_wait_us < -1 || t.ks_mfma_min < -1)
     return LUTHE_ERR_ARG;
   if (t.ntt_polys_per_wg != 1 && t.ntt_polys_per_wg != 2 && t.ntt_polys_per_wg != 4) return LUTHE_ERR_ARG;
   c->br_grid_override = t.br_grid;
@@ -78,6 +79,7 @@ int apply(luthe_ctx* c, const luthe_tuning& t) {
   c->br_pair = t.br_pair;
   c->br_split_max = t.br_split_max;
   c->br_split_wait_us = t.br_split_wait_us;
+  c->ks_mfma_min = t.ks_mfma_min;
   return LUTHE_OK;
 }
 
@@ -94,6 +96,7 @@ int luthe_get_tuning(luthe_ctx* c, luthe_tuning* out) {
   out->br_pair = c->br_pair;
   out->br_split_max = lx::split_max(c);
   out->br_split_wait_us = c->br_split_wait_us;
+  out->ks_mfma_min = lx::ks_mfma_min(c);
   return LUTHE_OK;
 }
 
@@ -128,6 +131,7 @@ int luthe_load_tuning(luthe_ctx* c, const char* path) {
   t.br_pair = json_int(s, "br_pair", v) ? (int32_t)v : -1;
   t.br_split_max = json_int(s, "br_split_max", v) ? (int32_t)v : -1;
   t.br_split_wait_us = c->br_split_wait_us;  // a robustness limit, not a tuned choice
+  t.ks_mfma_min = json_int(s, "ks_mfma_min", v) ? (int32_t)v : -1;
   int rc = apply(c, t);
   if (rc) return lx::fail(c, rc, "%s: invalid tuning", path);
   return LUTHE_OK;
@@ -167,7 +171,7 @@ int luthe_autotune(luthe_ctx* c, uint32_t batch, const char* json_path, luthe_tu
   (void)hipMemsetAsync(d_a, 0x31, npoly * 2048 * 8, c->stream);
   (void)hipMemsetAsync(d_b, 0x17, npoly * 2048 * 8, c->stream);
   const luthe_tuning saved{c->br_grid_override, c->br_latency_max, c->ntt_polys_per_wg, c->cohort_max, c->br_pair,
-                           c->br_split_max, c->br_split_wait_us};
+                           c->br_split_max, c->br_split_wait_us, c->ks_mfma_min};
   c->br_pair = 0;  // (1) sweeps the 256-thread kernel
   c->br_split_wait_us = -1;  // the split kernel's default wait limit while timing
   std::string meas;
@@ -236,6 +240,37 @@ int luthe_autotune(luthe_ctx* c, uint32_t batch, const char* json_path, luthe_tu
     ntt += line;
     if (ms < best_ntt * 0.99f) best_ntt = ms, best_np = np;
   }
+  // (4) key switch: the smallest tested batch from which the matrix-core kernel is faster
+  // than the VALU kernel at that size and every larger tested size
+  std::string ksm;
+  int ks_min = 1 << 30;
+  if (!rc && c->d_kskp) rc = lx::ensure_ks_part(c, c->bs, batch);
+  if (!rc && c->d_kskp) {
+    HIPCHK(c, lk::launch_make_jobs(c->d_jobs, d_in, d_out, nullptr, (int)batch, 0, c->prm.lut_slots, (int)c->blen,
+                                   c->stream));
+    const int sizes[] = {(int)batch, 3 * cus, cus, cus / 4, 32, 8, 1};
+    bool wins = true;
+    for (int n : sizes) {
+      if (rc || n < 1 || n > (int)batch) continue;
+      float ms[2] = {0, 0};
+      for (int m = 0; m < 2 && !rc; m++) {
+        c->ks_mfma_min = m ? 1 : (1 << 30);
+        rc = lx::launch_ks(c, c->d_jobs, n, c->bs.lwe2n, c->lwe_stride, c->bs.ks_part, c->stream);
+        Timer t(c->stream);
+        t.start();
+        for (int r = 0; r < 3 && !rc; r++)
+          rc = lx::launch_ks(c, c->d_jobs, n, c->bs.lwe2n, c->lwe_stride, c->bs.ks_part, c->stream);
+        ms[m] = t.stop() / 3;
+      }
+      if (rc) break;
+      snprintf(line, sizeof line, "%s{\"n\": %d, \"valu_ms\": %.4f, \"mfma_ms\": %.4f}", ksm.empty() ? "" : ", ", n,
+               ms[0], ms[1]);
+      ksm += line;
+      wins = wins && ms[1] < ms[0];
+      if (wins) ks_min = n;
+    }
+    c->ks_mfma_min = saved.ks_mfma_min;
+  }
   (void)hipStreamSynchronize(c->stream);
   cleanup();
   if (!rc) rc = lx::check_device_status(c);
@@ -244,7 +279,7 @@ int luthe_autotune(luthe_ctx* c, uint32_t batch, const char* json_path, luthe_tu
     return rc;
   }
   luthe_tuning t{best_pair || best_grid == occ ? 0 : best_grid, lat_max, best_np, width, best_pair, split_best,
-                 saved.br_split_wait_us};
+                 saved.br_split_wait_us, c->d_kskp ? ks_min : saved.ks_mfma_min};
   (void)apply(c, t);
   if (out) *out = t;
   if (json_path) {
@@ -255,12 +290,12 @@ int luthe_autotune(luthe_ctx* c, uint32_t batch, const char* json_path, luthe_tu
     fprintf(f,
             "{\n  \"device\": \"%s\", \"cus\": %d, \"lwe_dim\": %u, \"pbs_level\": %d, \"batch\": %u,\n"
             "  \"br_grid\": %d, \"br_pair\": %d, \"br_split_max\": %d, \"br_latency_max\": %d, \"ntt_polys_per_wg\": %d,\n"
-            "  \"cohort_max\": %d,\n"
+            "  \"cohort_max\": %d, \"ks_mfma_min\": %d,\n"
             "  \"measured\": {\n    \"blind_rotation_width\": [%s],\n    \"latency_variant\": [%s],\n"
-            "    \"ntt_polymul_4096\": [%s]\n  }\n}\n",
+            "    \"ntt_polymul_4096\": [%s],\n    \"key_switch\": [%s]\n  }\n}\n",
             prop.gcnArchName, cus, c->prm.lwe_dim, L, batch, t.br_grid, t.br_pair, t.br_split_max, t.br_latency_max,
             t.ntt_polys_per_wg,
-            t.cohort_max, meas.c_str(), lat.c_str(), ntt.c_str());
+            t.cohort_max, t.ks_mfma_min, meas.c_str(), lat.c_str(), ntt.c_str(), ksm.c_str());
     fclose(f);
   }
   return LUTHE_OK;

@@ -58,6 +58,7 @@ struct luthe_ctx {
   uint64_t* d_luts = nullptr;
   uint64_t* d_bsk = nullptr;
   uint32_t* d_ksk = nullptr;
+  int8_t* d_kskp = nullptr;   // the KSK as balanced byte planes for the matrix-core key switch
   uint64_t* d_tab = nullptr;  // NTT twiddle tables (ntt2048.h layout)
   bool bsk_ok = false, ksk_ok = false;
   // scratch (grow-only)
@@ -79,6 +80,7 @@ struct luthe_ctx {
   luthe_counters cnt{};
   int br_split_max = -1;        // split blind rotation up to this many ciphertexts (-1: CUs / 2)
   int br_split_wait_us = -1;    // its longest wait for a partner workgroup (-1: 2 s)
+  int ks_mfma_min = -1;         // smallest batch for the matrix-core key switch (-1: default)
   uint32_t* h_status = nullptr;  // pinned: set by a split pair whose exchange timed out
   uint32_t* d_status = nullptr;  // its device address
   mutable std::mutex cnt_mu;  // counters and the error message are also written by a server thread
@@ -150,6 +152,14 @@ uint32_t useful_bits(const luthe_ctx* c);
 bool use_pair(const luthe_ctx* c);
 // Largest cohort that runs the 512-thread latency kernel.
 int latency_max(luthe_ctx* c);
+// Key-switch arguments of the context's parameters with outputs lwe_stride apart.
+lk::KsArgs ks_args(const luthe_ctx* c, int lwe_stride);
+// Smallest batch that takes the matrix-core key switch (when the KSK planes exist).
+int ks_mfma_min(const luthe_ctx* c);
+// Key switch of n jobs into lwe2n: the matrix-core kernel, or the VALU kernel split over the
+// inputs with ks_part (NULL: no split).  Allocates nothing.
+int launch_ks(luthe_ctx* c, const lk::PbsJob* d_jobs, int n, uint16_t* lwe2n, int lwe_stride, uint32_t* ks_part,
+              hipStream_t s);
 
 }  // namespace lx
 

@@ -64,6 +64,15 @@ hipError_t launch_keyswitch(const PbsJob* jobs, int nct, const uint32_t* ksk, ui
                             const KsArgs& a, int ks_level, hipStream_t s, uint32_t* part = nullptr, int nsplit = 1);
 // The split launch_keyswitch uses for a batch of nct ciphertexts (1 = none).
 int keyswitch_split(int nct, int kN, int n_lwe);
+// Key switch on the int8 matrix cores (ks_mfma.hip): the KSK as three balanced byte planes in
+// fragment order (ksk_planes_bytes, built by launch_ksk_planes from the u32 KSK), digits
+// decomposed in the kernel.  Allocates nothing; results equal launch_keyswitch's.
+// keyswitch_mfma_ok: the parameters it covers (kN % 32 == 0, base_log * level <= 31).
+bool keyswitch_mfma_ok(const KsArgs& a, int ks_level);
+size_t ksk_planes_bytes(int kN, int n_lwe, int ks_level);
+hipError_t launch_ksk_planes(const uint32_t* ksk, int8_t* planes, const KsArgs& a, int ks_level, hipStream_t s);
+hipError_t launch_keyswitch_mfma(const PbsJob* jobs, int nct, const int8_t* planes, uint16_t* lwe2n, const KsArgs& a,
+                                 int ks_level, hipStream_t s);
 // Cohorts of at most latency_max ciphertexts run the 512-thread latency variant; larger ones
 // the 256-thread persistent kernel, or with `pair` the 512-thread two-per-CU variant.
 hipError_t launch_blind_rotate(const uint64_t* bsk, const uint64_t* luts, const uint16_t* lwe2n, const PbsJob* jobs,

@@ -260,6 +260,30 @@ int latency_max(luthe_ctx* c) {
   return c->br_latency_max;
 }
 
+lk::KsArgs ks_args(const luthe_ctx* c, int lwe_stride) {
+  const luthe_params& p = c->prm;
+  return lk::KsArgs{(int)(p.glwe_k * p.poly_size), (int)p.lwe_dim, lwe_stride, p.ks_base_log, p.ks_mod_log, 12};
+}
+
+constexpr int KS_MFMA_MIN_DEFAULT = 1;
+
+// The matrix-core key switch for batches of at least this many ciphertexts (when the KSK
+// planes exist); smaller ones take the VALU kernel, split over the inputs.
+int ks_mfma_min(const luthe_ctx* c) { return c->ks_mfma_min < 0 ? KS_MFMA_MIN_DEFAULT : c->ks_mfma_min; }
+
+int launch_ks(luthe_ctx* c, const PbsJob* d_jobs, int n, uint16_t* lwe2n, int lwe_stride, uint32_t* ks_part,
+              hipStream_t s) {
+  const luthe_params& p = c->prm;
+  const lk::KsArgs ka = ks_args(c, lwe_stride);
+  if (c->d_kskp && n >= ks_mfma_min(c)) {
+    HIPCHK(c, lk::launch_keyswitch_mfma(d_jobs, n, c->d_kskp, lwe2n, ka, (int)p.ks_level, s));
+    return LUTHE_OK;
+  }
+  const int nsplit = ks_part ? lk::keyswitch_split(n, ka.kN, ka.n_lwe) : 1;
+  HIPCHK(c, lk::launch_keyswitch(d_jobs, n, c->d_ksk, lwe2n, ka, (int)p.ks_level, s, ks_part, nsplit));
+  return LUTHE_OK;
+}
+
 // KS + BR over jobs already on the device, with the mod-switched LWEs in `lwe2n`
 // (n rows of lwe_stride values).  Allocates nothing.
 int run_pbs_on(luthe_ctx* c, const PbsJob* d_jobs, int n, hipStream_t s,
@@ -267,9 +291,8 @@ int run_pbs_on(luthe_ctx* c, const PbsJob* d_jobs, int n, hipStream_t s,
   if (!c->bsk_ok || !c->ksk_ok) return fail(c, LUTHE_ERR_STATE, "keys not uploaded");
   if (n == 0) return LUTHE_OK;
   const luthe_params& p = c->prm;
-  lk::KsArgs ka{(int)(p.glwe_k * p.poly_size), (int)p.lwe_dim, c->lwe_stride, p.ks_base_log, p.ks_mod_log, 12};
-  const int nsplit = ks_part ? lk::keyswitch_split(n, ka.kN, ka.n_lwe) : 1;
-  HIPCHK(c, lk::launch_keyswitch(d_jobs, n, c->d_ksk, lwe2n, ka, (int)p.ks_level, s, ks_part, nsplit));
+  int rc = launch_ks(c, d_jobs, n, lwe2n, c->lwe_stride, ks_part, s);
+  if (rc) return rc;
   lk::BrArgs ba{(int)p.lwe_dim, c->lwe_stride, p.pbs_base_log, c->br_split_wait_us};
   HIPCHK(c, lk::launch_blind_rotate(c->d_bsk, c->d_luts, lwe2n, d_jobs, n, c->d_tab, ba,
                                     (int)p.pbs_level, c->br_grid_override, s, latency_max(c), use_pair(c),
@@ -394,7 +417,7 @@ int luthe_destroy(luthe_ctx* c) {
     if (kv.second.done) (void)hipEventDestroy(kv.second.done);
   for (luthe_graph* g : c->graphs) g->ctx = nullptr;  // their scratch is freed below
   for (auto& r : c->retired) (void)hipFree(r.first);
-  void* bufs[] = {c->d_mem,     c->d_luts,      c->d_bsk,   c->d_ksk, c->d_tab,     c->bs.lwe2n,
+  void* bufs[] = {c->d_mem,     c->d_luts,      c->d_bsk,   c->d_ksk, c->d_kskp, c->d_tab,     c->bs.lwe2n,
                   c->bs.ks_part, c->ds.lwe2n, c->ds.ks_part, c->d_jobs, c->d_stage, c->bs.xch, c->ds.xch};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -524,6 +547,16 @@ int luthe_upload_ksk(luthe_ctx* c, const uint32_t* ksk) {
     if (rc) return rc;
   }
   HIPCHK(c, hipMemcpy(c->d_ksk, ksk, elems * 4, hipMemcpyHostToDevice));
+  // the byte planes of the matrix-core key switch, when the parameters allow it
+  const lk::KsArgs ka = ks_args(c, (int)p.lwe_dim + 1);
+  if (lk::keyswitch_mfma_ok(ka, (int)p.ks_level)) {
+    if (!c->d_kskp) {
+      int rc = dev_alloc(c, (void**)&c->d_kskp, lk::ksk_planes_bytes(ka.kN, ka.n_lwe, (int)p.ks_level));
+      if (rc) return rc;
+    }
+    HIPCHK(c, lk::launch_ksk_planes(c->d_ksk, c->d_kskp, ka, (int)p.ks_level, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
   c->ksk_ok = true;
   return LUTHE_OK;
 }
@@ -578,13 +611,9 @@ int luthe_keyswitch_batch(luthe_ctx* c, const uint64_t* d_blwe, uint16_t* d_lwe2
   if (rc) return rc;
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, lk::launch_make_jobs(c->d_jobs, d_blwe, nullptr, nullptr, (int)n, 0, c->prm.lut_slots, (int)c->blen, s));
-  const luthe_params& p = c->prm;
-  lk::KsArgs ka{(int)(p.glwe_k * p.poly_size), (int)p.lwe_dim, (int)p.lwe_dim + 1, p.ks_base_log, p.ks_mod_log, 12};
   rc = ensure_ks_part(c, c->bs, n);
   if (rc) return rc;
-  HIPCHK(c, lk::launch_keyswitch(c->d_jobs, (int)n, c->d_ksk, d_lwe2n, ka, (int)p.ks_level, s, c->bs.ks_part,
-                                 lk::keyswitch_split((int)n, ka.kN, ka.n_lwe)));
-  return LUTHE_OK;
+  return launch_ks(c, c->d_jobs, (int)n, d_lwe2n, (int)c->prm.lwe_dim + 1, c->bs.ks_part, s);
 }
 
 int luthe_blind_rotate_batch(luthe_ctx* c, const uint16_t* d_lwe2n, uint64_t* d_out, const uint32_t* d_gid,
