@@ -155,7 +155,7 @@ int latency_max(luthe_ctx* c);
 // Key-switch arguments of the context's parameters with outputs lwe_stride apart.
 lk::KsArgs ks_args(const luthe_ctx* c, int lwe_stride);
 // Smallest batch that takes the matrix-core key switch (when the KSK planes exist).
-int ks_mfma_min(const luthe_ctx* c);
+int ks_mfma_min(luthe_ctx* c);
 // Key switch of n jobs into lwe2n: the matrix-core kernel, or the VALU kernel split over the
 // inputs with ks_part (NULL: no split).  Allocates nothing.
 int launch_ks(luthe_ctx* c, const lk::PbsJob* d_jobs, int n, uint16_t* lwe2n, int lwe_stride, uint32_t* ks_part,

@@ -265,11 +265,19 @@ lk::KsArgs ks_args(const luthe_ctx* c, int lwe_stride) {
   return lk::KsArgs{(int)(p.glwe_k * p.poly_size), (int)p.lwe_dim, lwe_stride, p.ks_base_log, p.ks_mod_log, 12};
 }
 
-constexpr int KS_MFMA_MIN_DEFAULT = 1;
-
 // The matrix-core key switch for batches of at least this many ciphertexts (when the KSK
-// planes exist); smaller ones take the VALU kernel, split over the inputs.
-int ks_mfma_min(const luthe_ctx* c) { return c->ks_mfma_min < 0 ? KS_MFMA_MIN_DEFAULT : c->ks_mfma_min; }
+// planes exist); smaller ones take the VALU kernel, split over the inputs.  Default: three
+// 64-row tiles per CU (measured crossover on MI355X: 768 = 3 x 256 CUs).
+int ks_mfma_min(luthe_ctx* c) {
+  if (c->ks_mfma_min < 0) {
+    int cus = 0;
+    (void)hipSetDevice(c->device);
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus < 1)
+      cus = 256;
+    c->ks_mfma_min = 3 * cus;
+  }
+  return c->ks_mfma_min;
+}
 
 int launch_ks(luthe_ctx* c, const PbsJob* d_jobs, int n, uint16_t* lwe2n, int lwe_stride, uint32_t* ks_part,
               hipStream_t s) {
