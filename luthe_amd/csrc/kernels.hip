@@ -557,7 +557,9 @@ __global__ __launch_bounds__(512, 1) void k_blind_rotate_split(const uint64_t* _
 #pragma unroll
         for (int l = 0; l < L; l++) {
           const int gg = side ? 1 - h : h;
-          const int e = ((g + 2 * j) * NPF + gg * L + l) * K1 + h;
+          // g is uniform per wave: its readfirstlane copy keeps the scalar offset in SGPRs
+          // (a per-lane value there compiles to a loop over the distinct lane values)
+          const int e = ((wave_uniform(g) + 2 * j) * NPF + gg * L + l) * K1 + h;
           const auto w = __builtin_amdgcn_raw_buffer_load_b64(bsk_rsrc, lane_off, slice_off + e * T * 8, 0);
           kw[j][side][l] = ((uint64_t)w[1] << 32) | w[0];
         }
