@@ -601,6 +601,7 @@ int luthe_server_create(luthe_ctx* c, const luthe_server_config* cfg_in, luthe_s
   if (cfg.queue_max == 0) cfg.queue_max = 65536;
   HIPCHK(c, hipSetDevice(c->device));
   (void)lx::latency_max(c);  // resolved here, read only by the server thread
+  (void)lx::ks_mfma_min(c);  // likewise
   if (cfg.cohort_max == 0) cfg.cohort_max = c->cohort_max > 0 ? (uint32_t)c->cohort_max : (uint32_t)luthe_get_br_grid(c);
   if (cfg.cohort_max > (1u << 20)) return lx::fail(c, LUTHE_ERR_ARG, "cohort_max must be <= 2^20");
   if (cfg.policy > LUTHE_COHORT_SHORTEST) return lx::fail(c, LUTHE_ERR_ARG, "unknown cohort policy %u", cfg.policy);
