@@ -344,7 +344,7 @@ typedef struct luthe_tuning {
                                word (for tests of that path).  Start from luthe_get_tuning. */
   int32_t ks_mfma_min;      /* smallest batch whose key switch runs on the int8 matrix cores (the
                                KSK as three balanced byte planes, exact int32 sums); smaller batches
-                               take the VALU kernel.  -1 = default (3 x CUs); a value above every
+                               take the VALU kernel.  -1 = default (CUs); a value above every
                                batch size turns it off.  Results are identical. */
 } luthe_tuning;
 int luthe_autotune(luthe_ctx* ctx, uint32_t batch, const char* json_path, luthe_tuning* out);

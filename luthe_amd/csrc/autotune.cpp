@@ -248,7 +248,7 @@ int luthe_autotune(luthe_ctx* c, uint32_t batch, const char* json_path, luthe_tu
   if (!rc && c->d_kskp) {
     HIPCHK(c, lk::launch_make_jobs(c->d_jobs, d_in, d_out, nullptr, (int)batch, 0, c->prm.lut_slots, (int)c->blen,
                                    c->stream));
-    const int sizes[] = {(int)batch, 3 * cus, cus, cus / 4, 32, 8, 1};
+    const int sizes[] = {(int)batch, 3 * cus, cus, cus / 2, cus / 4, 32, 8, 1};
     bool wins = true;
     for (int n : sizes) {
       if (rc || n < 1 || n > (int)batch) continue;

@@ -266,15 +266,16 @@ lk::KsArgs ks_args(const luthe_ctx* c, int lwe_stride) {
 }
 
 // The matrix-core key switch for batches of at least this many ciphertexts (when the KSK
-// planes exist); smaller ones take the VALU kernel, split over the inputs.  Default: three
-// 64-row tiles per CU (measured crossover on MI355X: 768 = 3 x 256 CUs).
+// planes exist); smaller ones take the VALU kernel, split over the inputs.  Default: one
+// ciphertext per CU (measured on MI355X, 256 CUs: 0.13 against 0.24 ms at 256 ciphertexts, a
+// tie at 128, the VALU kernel ahead below).
 int ks_mfma_min(luthe_ctx* c) {
   if (c->ks_mfma_min < 0) {
     int cus = 0;
     (void)hipSetDevice(c->device);
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus < 1)
       cus = 256;
-    c->ks_mfma_min = 3 * cus;
+    c->ks_mfma_min = cus;
   }
   return c->ks_mfma_min;
 }
