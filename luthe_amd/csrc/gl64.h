@@ -140,6 +140,92 @@ GL_HD void sub4(uint64_t& r0, uint64_t& r1, uint64_t& r2, uint64_t& r3, uint64_t
 #endif
 }
 
+// Two and three independent sub_folds with their carry chains interleaved.  With two chains
+// each carry is read two instructions after it is written when an s_nop 0 sits between the
+// steps (three s_nop 0 instead of six s_nop 1 for two separate sub_folds); with three chains
+// the other two chains' steps cover the wait states.
+GL_HD void sub2(uint64_t& r0, uint64_t& r1, uint64_t a0, uint64_t a1, uint64_t b0, uint64_t b1) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint32_t l0, h0, l1, h1;
+  uint64_t w0, w1, c0, c1;
+  asm("v_sub_co_u32 %0, %4, %8, %12\n\t"
+      "v_sub_co_u32 %2, %5, %10, %14\n\t"
+      "s_nop 0\n\t"
+      "v_subb_co_u32 %1, %4, %9, %13, %4\n\t"
+      "v_subb_co_u32 %3, %5, %11, %15, %5\n\t"
+      "s_nop 0\n\t"
+      "v_addc_co_u32 %0, %6, %0, 0, %4\n\t"
+      "v_addc_co_u32 %2, %7, %2, 0, %5\n\t"
+      "s_nop 0\n\t"
+      "v_addc_co_u32 %1, %6, %1, 0, %6\n\t"
+      "v_addc_co_u32 %3, %7, %3, 0, %7\n\t"
+      "v_subb_co_u32 %1, %4, %1, 0, %4\n\t"
+      "v_subb_co_u32 %3, %5, %3, 0, %5"
+      : "=&v"(l0), "=&v"(h0), "=&v"(l1), "=&v"(h1), "=&s"(w0), "=&s"(w1), "=&s"(c0), "=&s"(c1)
+      : "v"((uint32_t)a0), "v"((uint32_t)(a0 >> 32)), "v"((uint32_t)a1), "v"((uint32_t)(a1 >> 32)),
+        "v"((uint32_t)b0), "v"((uint32_t)(b0 >> 32)), "v"((uint32_t)b1), "v"((uint32_t)(b1 >> 32)));
+  r0 = ((uint64_t)h0 << 32) | l0;
+  r1 = ((uint64_t)h1 << 32) | l1;
+#else
+  r0 = sub_fold(a0, b0);
+  r1 = sub_fold(a1, b1);
+#endif
+}
+
+GL_HD void sub3(uint64_t& r0, uint64_t& r1, uint64_t& r2, uint64_t a0, uint64_t a1, uint64_t a2, uint64_t b0,
+                uint64_t b1, uint64_t b2) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint32_t l0, h0, l1, h1, l2, h2;
+  uint64_t w0, w1, w2, c0, c1, c2;
+  asm("v_sub_co_u32 %0, %6, %12, %18\n\t"
+      "v_sub_co_u32 %2, %7, %14, %20\n\t"
+      "v_sub_co_u32 %4, %8, %16, %22\n\t"
+      "v_subb_co_u32 %1, %6, %13, %19, %6\n\t"
+      "v_subb_co_u32 %3, %7, %15, %21, %7\n\t"
+      "v_subb_co_u32 %5, %8, %17, %23, %8\n\t"
+      "v_addc_co_u32 %0, %9, %0, 0, %6\n\t"
+      "v_addc_co_u32 %2, %10, %2, 0, %7\n\t"
+      "v_addc_co_u32 %4, %11, %4, 0, %8\n\t"
+      "v_addc_co_u32 %1, %9, %1, 0, %9\n\t"
+      "v_addc_co_u32 %3, %10, %3, 0, %10\n\t"
+      "v_addc_co_u32 %5, %11, %5, 0, %11\n\t"
+      "v_subb_co_u32 %1, %6, %1, 0, %6\n\t"
+      "v_subb_co_u32 %3, %7, %3, 0, %7\n\t"
+      "v_subb_co_u32 %5, %8, %5, 0, %8"
+      : "=&v"(l0), "=&v"(h0), "=&v"(l1), "=&v"(h1), "=&v"(l2), "=&v"(h2), "=&s"(w0), "=&s"(w1), "=&s"(w2),
+        "=&s"(c0), "=&s"(c1), "=&s"(c2)
+      : "v"((uint32_t)a0), "v"((uint32_t)(a0 >> 32)), "v"((uint32_t)a1), "v"((uint32_t)(a1 >> 32)),
+        "v"((uint32_t)a2), "v"((uint32_t)(a2 >> 32)), "v"((uint32_t)b0), "v"((uint32_t)(b0 >> 32)),
+        "v"((uint32_t)b1), "v"((uint32_t)(b1 >> 32)), "v"((uint32_t)b2), "v"((uint32_t)(b2 >> 32)));
+  r0 = ((uint64_t)h0 << 32) | l0;
+  r1 = ((uint64_t)h1 << 32) | l1;
+  r2 = ((uint64_t)h2 << 32) | l2;
+#else
+  r0 = sub_fold(a0, b0);
+  r1 = sub_fold(a1, b1);
+  r2 = sub_fold(a2, b2);
+#endif
+}
+
+// r[i] = sub_fold(a[i], b[i]) for i < NS, in interleaved groups of four, three and two (a
+// lone sub_fold only for NS = 1).
+template <int NS>
+GL_HD void subn(uint64_t* r, const uint64_t* a, const uint64_t* b) {
+  if constexpr (NS == 1) {
+    r[0] = sub_fold(a[0], b[0]);
+  } else if constexpr (NS == 2) {
+    sub2(r[0], r[1], a[0], a[1], b[0], b[1]);
+  } else if constexpr (NS == 3) {
+    sub3(r[0], r[1], r[2], a[0], a[1], a[2], b[0], b[1], b[2]);
+  } else if constexpr (NS == 5 || NS == 6) {
+    sub3(r[0], r[1], r[2], a[0], a[1], a[2], b[0], b[1], b[2]);
+    subn<NS - 3>(r + 3, a + 3, b + 3);
+  } else if constexpr (NS >= 4) {
+    sub4(r[0], r[1], r[2], r[3], a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3]);
+    subn<NS - 4>(r + 4, a + 4, b + 4);
+  }
+}
+
 GL_HD uint64_t neg(uint64_t a) { return sub(0, a); }
 
 // (lo + hi * 2^64) mod p, canonical.  hi * 2^64 = hh * 2^96 + hl * 2^64 == -hh + hl * EPS.
@@ -190,6 +276,56 @@ GL_HD void acc128(uint64_t& lo, uint64_t& hi, uint32_t& c, uint64_t plo, uint64_
   const bool o2 = __builtin_add_overflow(t1, (uint64_t)k1, &nhi);
   lo = nlo, hi = nhi;
   c += (uint32_t)o1 + (uint32_t)o2;
+#endif
+}
+
+// acc128 over four lanes with the four carry chains interleaved: every carry is read three
+// instructions after it is written, so no s_nop is needed (acc128 needs four of them).
+GL_HD void acc128x4(uint64_t* lo, uint64_t* hi, uint32_t* c, const uint64_t* plo, const uint64_t* phi) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint32_t w[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+    w[i][0] = (uint32_t)lo[i], w[i][1] = (uint32_t)(lo[i] >> 32), w[i][2] = (uint32_t)hi[i],
+    w[i][3] = (uint32_t)(hi[i] >> 32);
+  uint64_t k0, k1, k2, k3;
+  asm("v_add_co_u32 %0, %20, %0, %24\n\t"
+      "v_add_co_u32 %4, %21, %4, %28\n\t"
+      "v_add_co_u32 %8, %22, %8, %32\n\t"
+      "v_add_co_u32 %12, %23, %12, %36\n\t"
+      "v_addc_co_u32 %1, %20, %1, %25, %20\n\t"
+      "v_addc_co_u32 %5, %21, %5, %29, %21\n\t"
+      "v_addc_co_u32 %9, %22, %9, %33, %22\n\t"
+      "v_addc_co_u32 %13, %23, %13, %37, %23\n\t"
+      "v_addc_co_u32 %2, %20, %2, %26, %20\n\t"
+      "v_addc_co_u32 %6, %21, %6, %30, %21\n\t"
+      "v_addc_co_u32 %10, %22, %10, %34, %22\n\t"
+      "v_addc_co_u32 %14, %23, %14, %38, %23\n\t"
+      "v_addc_co_u32 %3, %20, %3, %27, %20\n\t"
+      "v_addc_co_u32 %7, %21, %7, %31, %21\n\t"
+      "v_addc_co_u32 %11, %22, %11, %35, %22\n\t"
+      "v_addc_co_u32 %15, %23, %15, %39, %23\n\t"
+      "v_addc_co_u32 %16, %20, %16, 0, %20\n\t"
+      "v_addc_co_u32 %17, %21, %17, 0, %21\n\t"
+      "v_addc_co_u32 %18, %22, %18, 0, %22\n\t"
+      "v_addc_co_u32 %19, %23, %19, 0, %23"
+      // early-clobber: an input may hold the same value as a word written before it is read
+      : "+&v"(w[0][0]), "+&v"(w[0][1]), "+&v"(w[0][2]), "+&v"(w[0][3]), "+&v"(w[1][0]), "+&v"(w[1][1]),
+        "+&v"(w[1][2]), "+&v"(w[1][3]), "+&v"(w[2][0]), "+&v"(w[2][1]), "+&v"(w[2][2]), "+&v"(w[2][3]),
+        "+&v"(w[3][0]), "+&v"(w[3][1]), "+&v"(w[3][2]), "+&v"(w[3][3]), "+&v"(c[0]), "+&v"(c[1]), "+&v"(c[2]),
+        "+&v"(c[3]), "=&s"(k0), "=&s"(k1), "=&s"(k2), "=&s"(k3)
+      : "v"((uint32_t)plo[0]), "v"((uint32_t)(plo[0] >> 32)), "v"((uint32_t)phi[0]), "v"((uint32_t)(phi[0] >> 32)),
+        "v"((uint32_t)plo[1]), "v"((uint32_t)(plo[1] >> 32)), "v"((uint32_t)phi[1]), "v"((uint32_t)(phi[1] >> 32)),
+        "v"((uint32_t)plo[2]), "v"((uint32_t)(plo[2] >> 32)), "v"((uint32_t)phi[2]), "v"((uint32_t)(phi[2] >> 32)),
+        "v"((uint32_t)plo[3]), "v"((uint32_t)(plo[3] >> 32)), "v"((uint32_t)phi[3]), "v"((uint32_t)(phi[3] >> 32)));
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    lo[i] = ((uint64_t)w[i][1] << 32) | w[i][0];
+    hi[i] = ((uint64_t)w[i][3] << 32) | w[i][2];
+  }
+#else
+#pragma unroll
+  for (int i = 0; i < 4; i++) acc128(lo[i], hi[i], c[i], plo[i], phi[i]);
 #endif
 }
 
@@ -280,6 +416,70 @@ GL_HD uint64_t mul_pow2c(uint64_t x) {
   } else {
     const uint64_t r = (e <= 32) ? reduce96(x << e, x >> (64 - e)) : reduce128(x << e, x >> (64 - e));
     return negate ? neg(r) : r;
+  }
+}
+
+// mul_pow2c over several lanes, v[i] *= 2^S_i, with the lanes' sub_folds issued together
+// (subn): first the reductions of shifts in (32, 64) and the subtractions of shifts in
+// [64, 96), then the negations of shifts in [96, 192) whose remainder is below 64.  Same
+// results as mul_pow2c lane by lane.
+struct Pw2 {
+  int e;     // shift mod 96
+  bool neg;  // S mod 192 >= 96
+  int kind;  // 0: e = 0, 1: e <= 32, 2: e < 64, 3: e < 96
+};
+GL_HD constexpr Pw2 pw2_of(int S) {
+  const int s = ((S % 192) + 192) % 192;
+  const bool ng = s >= 96;
+  const int e = ng ? s - 96 : s;
+  return Pw2{e, ng, e == 0 ? 0 : e <= 32 ? 1 : e < 64 ? 2 : 3};
+}
+template <int... S>
+GL_HD void mul_pow2c_n(uint64_t* v) {
+  constexpr int K = sizeof...(S);
+  constexpr Pw2 I[K] = {pw2_of(S)...};
+  constexpr int N1 = ((pw2_of(S).kind >= 2 ? 1 : 0) + ... + 0);
+  constexpr int N2 = ((pw2_of(S).neg && pw2_of(S).kind <= 2 ? 1 : 0) + ... + 0);
+  uint64_t a[N1 > 0 ? N1 : 1], b[N1 > 0 ? N1 : 1], t[N1 > 0 ? N1 : 1];
+  uint32_t hl[K];
+  int j = 0;
+#pragma unroll
+  for (int i = 0; i < K; i++) {
+    const uint64_t x = v[i];
+    const int e = I[i].e;
+    hl[i] = 0;
+    if (I[i].kind == 1) {
+      v[i] = reduce96(x << e, x >> (64 - e));
+    } else if (I[i].kind == 2) {
+      a[j] = x << e;
+      b[j] = x >> (96 - e);  // the 2^96 part of the 128-bit x << e
+      hl[i] = (uint32_t)(x >> (64 - e));
+      j++;
+    } else if (I[i].kind == 3) {
+      const uint64_t v1 = (x << (e - 64)) & EPS, v2 = x >> (96 - e), v1e = (v1 << 32) - v1;
+      a[j] = I[i].neg ? v2 : v1e;
+      b[j] = I[i].neg ? v1e : v2;
+      j++;
+    }
+  }
+  if constexpr (N1 > 0) subn<N1>(t, a, b);
+  j = 0;
+#pragma unroll
+  for (int i = 0; i < K; i++) {
+    if (I[i].kind == 2) v[i] = add_eps_prod(t[j++], hl[i]);
+    else if (I[i].kind == 3) v[i] = t[j++];
+  }
+  if constexpr (N2 > 0) {
+    uint64_t z[N2], nb[N2], r[N2];
+    j = 0;
+#pragma unroll
+    for (int i = 0; i < K; i++)
+      if (I[i].neg && I[i].kind <= 2) z[j] = 0, nb[j++] = v[i];
+    subn<N2>(r, z, nb);
+    j = 0;
+#pragma unroll
+    for (int i = 0; i < K; i++)
+      if (I[i].neg && I[i].kind <= 2) v[i] = r[j++];
   }
 }
 

@@ -345,15 +345,14 @@ __global__ __launch_bounds__(512, TP ? 2 : 1) void k_blind_rotate_lat(const uint
 #pragma unroll
         for (int u = 0; u < 4; u++) {
           cc[u] = 0;
+          gl::mul_full(lds[(q + u) * T + tt], kw[u][0], lo[u], hi[u]);
+        }
 #pragma unroll
-          for (int gl_ = 0; gl_ < NPF; gl_++) {
-            uint64_t plo, phi;
-            gl::mul_full(lds[(gl_ * 8 + q + u) * T + tt], kw[u][gl_], plo, phi);
-            if (gl_ == 0)
-              lo[u] = plo, hi[u] = phi;
-            else
-              gl::acc128(lo[u], hi[u], cc[u], plo, phi);
-          }
+        for (int gl_ = 1; gl_ < NPF; gl_++) {
+          uint64_t plo[4], phi[4];
+#pragma unroll
+          for (int u = 0; u < 4; u++) gl::mul_full(lds[(gl_ * 8 + q + u) * T + tt], kw[u][gl_], plo[u], phi[u]);
+          gl::acc128x4(lo, hi, cc, plo, phi);
         }
         gl::reduce160x4(o, lo, hi, cc);
 #pragma unroll
@@ -584,15 +583,14 @@ __global__ __launch_bounds__(512, 1) void k_blind_rotate_split(const uint64_t* _
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       cc[j] = 0;
+      gl::mul_full(v[0][j], kw[j][0][0], lo[j], hi[j]);
+    }
 #pragma unroll
-      for (int l = 0; l < L; l++) {
-        uint64_t plo, phi;
-        gl::mul_full(v[l][j], kw[j][0][l], plo, phi);
-        if (l == 0)
-          lo[j] = plo, hi[j] = phi;
-        else
-          gl::acc128(lo[j], hi[j], cc[j], plo, phi);
-      }
+    for (int l = 1; l < L; l++) {
+      uint64_t plo[4], phi[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) gl::mul_full(v[l][j], kw[j][0][l], plo[j], phi[j]);
+      gl::acc128x4(lo, hi, cc, plo, phi);
     }
     const uint64_t* rcv = xc + (size_t)((i & 1) * 2 + (1 - h)) * L * 8 * T4 + t;
     uint64_t got[L][4][2];
@@ -619,14 +617,15 @@ __global__ __launch_bounds__(512, 1) void k_blind_rotate_split(const uint64_t* _
       }
     }
 #pragma unroll
-    for (int j = 0; j < 4; j++)
+    for (int l = 0; l < L; l++) {
+      uint64_t plo[4], phi[4];
 #pragma unroll
-      for (int l = 0; l < L; l++) {
+      for (int j = 0; j < 4; j++) {
         const uint64_t d = (got[l][j][1] << 32) | (uint32_t)got[l][j][0];
-        uint64_t plo, phi;
-        gl::mul_full(d, kw[j][1][l], plo, phi);
-        gl::acc128(lo[j], hi[j], cc[j], plo, phi);
+        gl::mul_full(d, kw[j][1][l], plo[j], phi[j]);
       }
+      gl::acc128x4(lo, hi, cc, plo, phi);
+    }
     uint64_t z[4];
     gl::reduce160x4(z, lo, hi, cc);
     inv_passes_x4(z, bx, by, tw, t);

@@ -70,17 +70,25 @@ GL_HD void dft8(uint64_t* x) {
   gl::sub4(d0, d1, d2, d3, x[0], x[1], x[2], x[3], x[4], x[5], x[6], x[7]);
   x[0] = s0; x[1] = s1; x[2] = s2; x[3] = s3;
   x[4] = d0;
-  x[5] = gl::mul_pow2c<S1>(d1);
-  x[6] = gl::mul_pow2c<2 * S1>(d2);
-  x[7] = gl::mul_pow2c<3 * S1>(d3);
+  {
+    uint64_t w[3] = {d1, d2, d3};
+    gl::mul_pow2c_n<S1, 2 * S1, 3 * S1>(w);
+    x[5] = w[0];
+    x[6] = w[1];
+    x[7] = w[2];
+  }
   // stage 2: pairs (h+j, h+j+2), twiddle root^(2j)
   s0 = gl::add(x[0], x[2]); s1 = gl::add(x[1], x[3]); s2 = gl::add(x[4], x[6]); s3 = gl::add(x[5], x[7]);
   gl::sub4(d0, d1, d2, d3, x[0], x[1], x[4], x[5], x[2], x[3], x[6], x[7]);
   x[0] = s0; x[1] = s1; x[4] = s2; x[5] = s3;
   x[2] = d0;
-  x[3] = gl::mul_pow2c<2 * S1>(d1);
   x[6] = d2;
-  x[7] = gl::mul_pow2c<2 * S1>(d3);
+  {
+    uint64_t w[2] = {d1, d3};
+    gl::mul_pow2c_n<2 * S1, 2 * S1>(w);
+    x[3] = w[0];
+    x[7] = w[1];
+  }
   // stage 3: pairs (2j, 2j+1); the outputs land bit-reversed (1 <-> 4, 3 <-> 6), which
   // the assignment below undoes
   s0 = gl::add(x[0], x[1]); s1 = gl::add(x[2], x[3]); s2 = gl::add(x[4], x[5]); s3 = gl::add(x[6], x[7]);
@@ -96,8 +104,12 @@ GL_HD void dft4x2(uint64_t* x) {
   uint64_t c0 = gl::add(x[4], x[6]), e0 = gl::add(x[5], x[7]);
   uint64_t a1, b1, c1, e1;
   gl::sub4(a1, b1, c1, e1, x[0], x[1], x[4], x[5], x[2], x[3], x[6], x[7]);
-  b1 = gl::mul_pow2c<S2>(b1);
-  e1 = gl::mul_pow2c<S2>(e1);
+  {
+    uint64_t w[2] = {b1, e1};
+    gl::mul_pow2c_n<S2, S2>(w);
+    b1 = w[0];
+    e1 = w[1];
+  }
   x[0] = gl::add(a0, b0);
   x[1] = gl::add(a1, b1);
   x[4] = gl::add(c0, e0);
@@ -125,28 +137,20 @@ GL_HD void idft4x2(uint64_t* x) {
   swap_(x[5], x[7]);
 }
 
-// Twists by compile-time powers of two over registers K..7 / K..3 (template recursion keeps
-// every shift amount a constant, so each wave-uniform case compiles to straight-line code).
-template <int M0, int K = 0>
+// Twists by compile-time powers of two (every shift amount a constant, so each wave-uniform
+// case compiles to straight-line code; the lanes' subtractions are interleaved).
+template <int M0>
 GL_HD void a1_post_twist(uint64_t* v) {  // v[k1] *= 2^(3 M0 (1 + 2 k1))
-  if constexpr (K < 8) {
-    v[K] = gl::mul_pow2c<3 * M0 * (1 + 2 * K)>(v[K]);
-    a1_post_twist<M0, K + 1>(v);
-  }
+  gl::mul_pow2c_n<3 * M0, 9 * M0, 15 * M0, 21 * M0, 27 * M0, 33 * M0, 39 * M0, 45 * M0>(v);
 }
-template <int K1, int M0 = 0>
-GL_HD void ia2_twist(uint64_t* v) {  // v[m0] *= -2^(-3 m0 (2 K1 + 1)) = 2^(96 - 3 m0 (2 K1 + 1))
-  if constexpr (M0 < 4) {
-    v[M0] = gl::mul_pow2c<96 - 3 * M0 * (2 * K1 + 1)>(v[M0]);
-    ia2_twist<K1, M0 + 1>(v);
-  }
+// v[h * 4 + m0] *= -2^(-3 m0 (2 k1 + 1)) = 2^(96 - 3 m0 (2 k1 + 1)), k1 = 2 Q + h
+template <int Q>
+GL_HD void ia2_twist(uint64_t* v) {
+  constexpr int F = 3 * (4 * Q + 1), G = 3 * (4 * Q + 3);
+  gl::mul_pow2c_n<96, 96 - F, 96 - 2 * F, 96 - 3 * F, 96, 96 - G, 96 - 2 * G, 96 - 3 * G>(v);
 }
-template <int M1 = 0>
 GL_HD void ia1_twist(uint64_t* v) {  // v[m1] *= -2^(-12 m1) = 2^(96 - 12 m1)
-  if constexpr (M1 < 8) {
-    v[M1] = gl::mul_pow2c<96 - 12 * M1>(v[M1]);
-    ia1_twist<M1 + 1>(v);
-  }
+  gl::mul_pow2c_n<96, 84, 72, 60, 48, 36, 24, 12>(v);
 }
 
 // Wave-uniform copy of a value that is uniform within each 64-lane wave (the A-pass
@@ -165,15 +169,7 @@ GL_HD int wave_uniform(int x) {
 // psi^m0 Omega^(k1 m0) = 2^(3 m0 (1 + 2 k1)) that is uniform per wave.
 // pretwisted: v already holds p[n] psi^(4 m1) (the blind rotation's digit lift applies it).
 GL_HD void fwd_a1_compute(uint64_t* v, int m0u, bool pretwisted = false) {
-  if (!pretwisted) {
-    v[1] = gl::mul_pow2c<12>(v[1]);
-    v[2] = gl::mul_pow2c<24>(v[2]);
-    v[3] = gl::mul_pow2c<36>(v[3]);
-    v[4] = gl::mul_pow2c<48>(v[4]);
-    v[5] = gl::mul_pow2c<60>(v[5]);
-    v[6] = gl::mul_pow2c<72>(v[6]);
-    v[7] = gl::mul_pow2c<84>(v[7]);
-  }
+  if (!pretwisted) gl::mul_pow2c_n<12, 24, 36, 48, 60, 72, 84>(v + 1);
   dft8<24>(v);
   switch (m0u) {
     case 1: a1_post_twist<1>(v); break;
@@ -219,13 +215,11 @@ GL_HD int c1_poly(int j, int h) { return NP == 1 ? 0 : h + 2 * (j >> 1); }
 template <int NP>
 GL_HD int c1_r0(int j, int w, int h) { return NP == 1 ? r0_of(w, h) : r0_of(w, j & 1); }
 
-// v[w1] *= 8^(S w1 R0) = 2^(3 S w1 R0), w1 = W1..7 (S = 1 forward, -1 inverse).
-template <int S, int R0, int W1 = 1>
+// v[w1] *= 8^(S w1 R0) = 2^(3 S w1 R0), w1 = 1..7 (S = 1 forward, -1 inverse).
+template <int S, int R0>
 GL_HD void c_twist(uint64_t* v) {
-  if constexpr (W1 < 8) {
-    v[W1] = gl::mul_pow2c<3 * S * W1 * R0>(v[W1]);
-    c_twist<S, R0, W1 + 1>(v);
-  }
+  constexpr int F = 3 * S * R0;
+  gl::mul_pow2c_n<F, 2 * F, 3 * F, 4 * F, 5 * F, 6 * F, 7 * F>(v + 1);
 }
 template <int S, int NP, int W>
 GL_HD void c_twist_wave(uint64_t (*v)[8]) {
@@ -356,10 +350,10 @@ GL_HD void inv_a2_load(uint64_t* v, const uint64_t* lds, int r, int q) {
 GL_HD void inv_a2_compute(uint64_t* v, int qu) {
   idft4x2(v);
   switch (qu) {
-    case 0: ia2_twist<0>(v); ia2_twist<1>(v + 4); break;
-    case 1: ia2_twist<2>(v); ia2_twist<3>(v + 4); break;
-    case 2: ia2_twist<4>(v); ia2_twist<5>(v + 4); break;
-    default: ia2_twist<6>(v); ia2_twist<7>(v + 4); break;
+    case 0: ia2_twist<0>(v); break;
+    case 1: ia2_twist<1>(v); break;
+    case 2: ia2_twist<2>(v); break;
+    default: ia2_twist<3>(v); break;
   }
 }
 GL_HD void inv_a2_store(const uint64_t* v, uint64_t* lds, int r, int q) {

@@ -61,14 +61,19 @@ GL_HD int idx_la(int m0, int s, int j0, int r) { return (8 * m0 + 4 * s + j0) * 
 GL_HD int idx_u2(int k, int wq, int rp) { return 64 * k + ((16 * wq + rp) ^ k); }
 GL_HD int idx_u3(int k, int wb, int wq, int ra) { return 64 * k + ((16 * wb + 4 * wq + ra) ^ k); }
 
-// DFT-4 in natural order with root 2^S (S = 48 forward).
+// DFT-4 in natural order with root 2^S (S = 48 forward).  The subtractions go in pairs
+// (gl::sub2): x0 - x2 with x1 - x3, then the reduction of (x1 - x3) 2^S (S in (32, 64): a
+// 128-bit value lo + 2^64 hl + 2^96 hh, reduced as in gl::reduce128) with a0 - b0.
 template <int S>
 GL_HD void dft4(uint64_t* x) {
+  static_assert(S > 32 && S < 64, "the paired reduction assumes a shift in (32, 64)");
   const uint64_t a0 = gl::add(x[0], x[2]), b0 = gl::add(x[1], x[3]);
-  const uint64_t a1 = gl::sub(x[0], x[2]), b1 = gl::mul_pow2c<S>(gl::sub(x[1], x[3]));
+  uint64_t a1, d, t;
+  gl::sub2(a1, d, x[0], x[1], x[2], x[3]);
+  gl::sub2(t, x[2], d << S, a0, d >> (96 - S), b0);
+  const uint64_t b1 = gl::add_eps_prod(t, (uint32_t)(d >> (64 - S)));
   x[0] = gl::add(a0, b0);
   x[1] = gl::add(a1, b1);
-  x[2] = gl::sub(a0, b0);
   x[3] = gl::sub(a1, b1);
 }
 GL_HD void idft4(uint64_t* x) {
@@ -99,18 +104,15 @@ struct Map {
 // Out: v[2 s + e] = u_s[j0 = 2G + e].
 template <int G, bool PRE>
 GL_HD void a1a(uint64_t* v) {
-  if constexpr (!PRE) {
-    v[0] = gl::mul_pow2c<12 * (2 * G)>(v[0]);
-    v[1] = gl::mul_pow2c<12 * (2 * G + 1)>(v[1]);
-    v[2] = gl::mul_pow2c<12 * (2 * G + 4)>(v[2]);
-    v[3] = gl::mul_pow2c<12 * (2 * G + 5)>(v[3]);
-  }
+  if constexpr (!PRE) gl::mul_pow2c_n<12 * (2 * G), 12 * (2 * G + 1), 12 * (2 * G + 4), 12 * (2 * G + 5)>(v);
   const uint64_t u00 = gl::add(v[0], v[2]), u01 = gl::add(v[1], v[3]);
-  const uint64_t d0 = gl::sub(v[0], v[2]), d1 = gl::sub(v[1], v[3]);
+  uint64_t d[2];
+  gl::sub2(d[0], d[1], v[0], v[1], v[2], v[3]);
+  gl::mul_pow2c_n<24 * (2 * G), 24 * (2 * G + 1)>(d);
   v[0] = u00;
   v[1] = u01;
-  v[2] = gl::mul_pow2c<24 * (2 * G)>(d0);
-  v[3] = gl::mul_pow2c<24 * (2 * G + 1)>(d1);
+  v[2] = d[0];
+  v[3] = d[1];
 }
 template <bool PRE>
 GL_HD void fwd_a1a_compute(uint64_t* v, int gu) {
@@ -135,10 +137,8 @@ GL_HD void fwd_a1b_load(uint64_t* v, const uint64_t* lds, const Map& m) {
 template <int M0, int S>
 GL_HD void a1b(uint64_t* v) {
   dft4<48>(v);
-  v[0] = gl::mul_pow2c<3 * M0 * (1 + 2 * S)>(v[0]);
-  v[1] = gl::mul_pow2c<3 * M0 * (1 + 2 * (2 + S))>(v[1]);
-  v[2] = gl::mul_pow2c<3 * M0 * (1 + 2 * (4 + S))>(v[2]);
-  v[3] = gl::mul_pow2c<3 * M0 * (1 + 2 * (6 + S))>(v[3]);
+  gl::mul_pow2c_n<3 * M0 * (1 + 2 * S), 3 * M0 * (1 + 2 * (2 + S)), 3 * M0 * (1 + 2 * (4 + S)),
+                  3 * M0 * (1 + 2 * (6 + S))>(v);
 }
 GL_HD void fwd_a1b_compute(uint64_t* v, int Wu) {
   switch (Wu) {
@@ -194,9 +194,7 @@ GL_HD void fwd_c2_load(uint64_t* v, const uint64_t* lds, const Map& m) {
 template <int RA>
 GL_HD void c2(uint64_t* v) {
   dft4<48>(v);
-  v[1] = gl::mul_pow2c<12 * RA>(v[1]);
-  v[2] = gl::mul_pow2c<24 * RA>(v[2]);
-  v[3] = gl::mul_pow2c<36 * RA>(v[3]);
+  gl::mul_pow2c_n<12 * RA, 24 * RA, 36 * RA>(v + 1);
 }
 GL_HD void fwd_c2_compute(uint64_t* v, int rau) {
   switch (rau) {
@@ -233,9 +231,7 @@ GL_HD void inv_c2_load(uint64_t* v, const uint64_t* lds, const Map& m) {
 }
 template <int RA>
 GL_HD void ic2(uint64_t* v) {  // 2^(-12 RA wb), then wb -> rb
-  v[1] = gl::mul_pow2c<-12 * RA>(v[1]);
-  v[2] = gl::mul_pow2c<-24 * RA>(v[2]);
-  v[3] = gl::mul_pow2c<-36 * RA>(v[3]);
+  gl::mul_pow2c_n<-12 * RA, -24 * RA, -36 * RA>(v + 1);
   idft4(v);
 }
 GL_HD void inv_c2_compute(uint64_t* v, int rau) {
@@ -274,10 +270,7 @@ GL_HD void inv_a2_load(uint64_t* v, const uint64_t* lds, const Map& m) {
 template <int K1>
 GL_HD void ia2(uint64_t* v) {  // k0 -> m0, then 2^(96 - 3 m0 (2 K1 + 1))
   idft4(v);
-  v[0] = gl::mul_pow2c<96>(v[0]);
-  v[1] = gl::mul_pow2c<96 - 3 * (2 * K1 + 1)>(v[1]);
-  v[2] = gl::mul_pow2c<96 - 6 * (2 * K1 + 1)>(v[2]);
-  v[3] = gl::mul_pow2c<96 - 9 * (2 * K1 + 1)>(v[3]);
+  gl::mul_pow2c_n<96, 96 - 3 * (2 * K1 + 1), 96 - 6 * (2 * K1 + 1), 96 - 9 * (2 * K1 + 1)>(v);
 }
 GL_HD void inv_a2_compute(uint64_t* v, int Wu) {
   switch (Wu) {
@@ -316,13 +309,13 @@ GL_HD void inv_a1a_load(uint64_t* v, const uint64_t* lds, const Map& m) {
 }
 template <int G>
 GL_HD void ia1a(uint64_t* v) {
-  const uint64_t b0 = gl::mul_pow2c<-24 * (2 * G)>(v[2]), b1 = gl::mul_pow2c<-24 * (2 * G + 1)>(v[3]);
-  const uint64_t x0 = gl::add(v[0], b0), x1 = gl::add(v[1], b1);
-  const uint64_t x4 = gl::sub(v[0], b0), x5 = gl::sub(v[1], b1);
-  v[0] = gl::mul_pow2c<96 - 12 * (2 * G)>(x0);
-  v[1] = gl::mul_pow2c<96 - 12 * (2 * G + 1)>(x1);
-  v[2] = gl::mul_pow2c<96 - 12 * (2 * G + 4)>(x4);
-  v[3] = gl::mul_pow2c<96 - 12 * (2 * G + 5)>(x5);
+  uint64_t b[2] = {v[2], v[3]};
+  gl::mul_pow2c_n<-24 * (2 * G), -24 * (2 * G + 1)>(b);
+  const uint64_t x0 = gl::add(v[0], b[0]), x1 = gl::add(v[1], b[1]);
+  gl::sub2(v[2], v[3], v[0], v[1], b[0], b[1]);
+  v[0] = x0;
+  v[1] = x1;
+  gl::mul_pow2c_n<96 - 12 * (2 * G), 96 - 12 * (2 * G + 1), 96 - 12 * (2 * G + 4), 96 - 12 * (2 * G + 5)>(v);
 }
 GL_HD void inv_a1a_compute(uint64_t* v, int gu) {
   if (gu == 0)

@@ -182,21 +182,19 @@ GL_HD void mac_with(uint64_t (*z)[8], KeyWord key) {
     for (int u = 0; u < 4; u++)
 #pragma unroll
       for (int gl_ = 0; gl_ < K1 * L; gl_++) kw[u][gl_] = key((((q + u) / K1) * (K1 * L) + gl_) * K1 + (q + u) % K1);
+    // per output the sum of K1*L full 128-bit products, carries past 2^128 counted in c, one
+    // reduction; the four outputs' accumulations interleaved (acc128x4)
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-      const int w0 = (q + u) / K1;
-      // sum of K1*L full 128-bit products, carries past 2^128 counted in c, one reduction
       c[u] = 0;
+      gl::mul_full(z[0][(q + u) / K1], kw[u][0], lo[u], hi[u]);
+    }
 #pragma unroll
-      for (int gl_ = 0; gl_ < K1 * L; gl_++) {
-        const uint64_t b = kw[u][gl_];
-        uint64_t plo, phi;
-        gl::mul_full(z[gl_][w0], b, plo, phi);
-        if (gl_ == 0)
-          lo[u] = plo, hi[u] = phi;
-        else
-          gl::acc128(lo[u], hi[u], c[u], plo, phi);
-      }
+    for (int gl_ = 1; gl_ < K1 * L; gl_++) {
+      uint64_t plo[4], phi[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) gl::mul_full(z[gl_][(q + u) / K1], kw[u][gl_], plo[u], phi[u]);
+      gl::acc128x4(lo, hi, c, plo, phi);
     }
     gl::reduce160x4(o, lo, hi, c);
 #pragma unroll

@@ -8,7 +8,7 @@
 #include "../../luthe_amd/csrc/pbs_core.h"
 
 namespace {
-constexpr int NOUT = 33;  // results per operand triple, in the order listed in prims_run
+constexpr int NOUT = 58;  // results per operand triple, in the order listed in prims_run
 
 __global__ void k_prims(const uint64_t* a, const uint64_t* b, const uint32_t* c, int n, uint64_t* out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -53,6 +53,19 @@ __global__ void k_prims(const uint64_t* a, const uint64_t* b, const uint32_t* c,
   // signed lift into Z_p, full range and a shifted (small-magnitude) value
   o[31] = gl::lift64((int64_t)x);
   o[32] = gl::lift64((int64_t)x >> 17);
+  // interleaved sub_folds (exact 64-bit results, operands any a and canonical b)
+  gl::sub2(o[33], o[34], x, y, yc, xc);
+  gl::sub3(o[35], o[36], o[37], x ^ y, ~x, y, yc, xc, gl::canonical(x ^ y));
+  // four interleaved 128-bit accumulations
+  uint64_t al[4] = {x, y, x ^ y, ~x}, ah[4] = {y, x, ~x, ~y};
+  uint32_t ac[4] = {z & 3, (z >> 2) & 3, 1, 0};
+  const uint64_t pl[4] = {y, x, ~y, ~x}, ph[4] = {x, y, x, ~y};
+  gl::acc128x4(al, ah, ac, pl, ph);
+  for (int i = 0; i < 4; i++) o[38 + 3 * i] = al[i], o[39 + 3 * i] = ah[i], o[40 + 3 * i] = ac[i];
+  // batched compile-time shifts (every kind: e = 0, <= 32, < 64, < 96, each negated or not)
+  uint64_t tv[8] = {xc, yc, xc, yc, gl::canonical(x ^ y), xc, yc, xc};
+  gl::mul_pow2c_n<24, 48, 72, 120, 168, 0, 96, 36>(tv);
+  for (int i = 0; i < 8; i++) o[50 + i] = tv[i];
 }
 }  // namespace
 

@@ -40,8 +40,8 @@ def test_device_primitives_exact(prims, oracle):
     a = np.array([p[0] for p in pairs], np.uint64)
     b = np.array([p[1] for p in pairs], np.uint64)
     c = np.array([zs[i % len(zs)] for i in range(n)], np.uint32)
-    out = np.zeros((n, 33), np.uint64)
-    print(f"case: {n} operand pairs through 21 device primitives", flush=True)
+    out = np.zeros((n, 58), np.uint64)
+    print(f"case: {n} operand pairs through the device primitives", flush=True)
     assert prims.prims_run(a.ctypes.data, b.ctypes.data, c.ctypes.data, n, out.ctypes.data) == 0
     for i, (x, y) in enumerate(pairs):
         z = int(c[i])
@@ -62,5 +62,15 @@ def test_device_primitives_exact(prims, oracle):
                 (x + (x << 64)) % P, (y + (ny << 64) + (0xFFFFFFFE << 128)) % P]
         xs = x - 2**64 if x >= 2**63 else x  # x as a signed 64-bit value
         exp += [xs % P, (xs >> 17) % P]
+        def sf(a, b):  # sub_fold
+            return a - b if a >= b else a - b + 2**64 - EPS
+        xy = x ^ y
+        exp += [sf(x, yc), sf(y, xc), sf(xy, yc), sf(nx, xc), sf(y, xy % P)]
+        for lo, hi, cin, pl, ph in ((x, y, z & 3, y, x), (y, x, (z >> 2) & 3, x, y), (xy, nx, 1, ny, x),
+                                    (nx, ny, 0, nx, ny)):
+            acc = (lo + (hi << 64) + (cin << 128)) + (pl + (ph << 64))
+            exp += [acc & M64, (acc >> 64) & M64, acc >> 128]
+        exp += [v * pow(2, s, P) % P for v, s in zip((xc, yc, xc, yc, xy % P, xc, yc, xc),
+                                                    (24, 48, 72, 120, 168, 0, 96, 36))]
         got = [int(v) for v in out[i]]
-        assert got == exp, (hex(x), hex(y), z, [k for k in range(33) if got[k] != exp[k]])
+        assert got == exp, (hex(x), hex(y), z, [k for k in range(58) if got[k] != exp[k]])
