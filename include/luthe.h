@@ -319,6 +319,13 @@ int luthe_get_br_grid(luthe_ctx* ctx);
  * reference counterpart (the HPU's PBS latency is fixed by its pipeline). */
 int luthe_set_br_latency_max(luthe_ctx* ctx, int max_cohort);
 int luthe_get_br_latency_max(luthe_ctx* ctx);
+/* Split variant (luthe_tuning.br_split_max): 1 = the two workgroups of a ciphertext exchange
+ * their digits through the L2 of the XCC they share (workgroup-scope stores, the default when
+ * luthe_create found every pair on one XCC; LUTHE_ERR_STATE otherwise), 0 = agent-scope
+ * stores, correct wherever the pair runs.  Results are identical.  No reference counterpart
+ * (the HPU has no multi-CU split). */
+int luthe_set_br_split_local(luthe_ctx* ctx, int local);
+int luthe_get_br_split_local(luthe_ctx* ctx);
 /* Tuning knobs and the autotuner.  luthe_autotune times, with the context's keys and
  * synthetic inputs, (1) the persistent blind-rotation width (ciphertexts resident per CU,
  * batches of `batch`), (2) the largest cohort for which the latency variant is faster,

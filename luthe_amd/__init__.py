@@ -199,6 +199,8 @@ def lib():
         "luthe_get_tuning": (i32, [vp, C.POINTER(Tuning)]),
         "luthe_set_tuning": (i32, [vp, C.POINTER(Tuning)]),
         "luthe_get_br_latency_max": (i32, [vp]),
+        "luthe_set_br_split_local": (i32, [vp, i32]),
+        "luthe_get_br_split_local": (i32, [vp]),
         "luthe_microbench_modmul": (i32, [vp, C.POINTER(C.c_double)]),
         "luthe_dop_schedule": (i32, [vp, sz, u32, u32, u32, vp]),
         "luthe_dop_plan": (i32, [vp, sz, u32, u32, u32, vp, vp, sz, C.POINTER(sz), C.POINTER(u32)]),

@@ -83,6 +83,8 @@ struct luthe_ctx {
   int ks_mfma_min = -1;         // smallest batch for the matrix-core key switch (-1: default)
   uint32_t* h_status = nullptr;  // pinned: set by a split pair whose exchange timed out
   uint32_t* d_status = nullptr;  // its device address
+  bool split_local_ok = false;   // split pairs share an XCC (probed at creation)
+  bool split_local = false;      // their exchange goes through that XCC's L2 (needs _ok)
   mutable std::mutex cnt_mu;  // counters and the error message are also written by a server thread
   std::mutex err_mu;
   uint64_t next_ticket = 1;

@@ -78,11 +78,16 @@ hipError_t launch_keyswitch_mfma(const PbsJob* jobs, int nct, const int8_t* plan
 hipError_t launch_blind_rotate(const uint64_t* bsk, const uint64_t* luts, const uint16_t* lwe2n, const PbsJob* jobs,
                                int nct, const uint64_t* tab, const BrArgs& a,
                                int pbs_level, int grid_override, hipStream_t s, int latency_max, bool pair,
-                               int split_max = 0, uint64_t* xch = nullptr, uint32_t* status = nullptr);
+                               int split_max = 0, uint64_t* xch = nullptr, uint32_t* status = nullptr,
+                               bool split_local = false);
 // Split blind rotation (two workgroups per ciphertext) for cohorts of at most split_max
 // ciphertexts when xch (blind_rotate_split_words(pbs_level) u64 per ciphertext) and status
-// (a word the host can read; set to 1 if a pair's exchange timed out) are given.
+// (a word the host can read; set to 1 if a pair's exchange timed out) are given.  split_local:
+// the pair's words go through the shared L2 of its XCC (split_pairs_share_xcc() must hold).
 size_t blind_rotate_split_words(int pbs_level);
+// Whether the device places workgroups b and b ^ 8 of a split grid on one XCC (a probe kernel
+// on stream s, synchronous; false on any error).
+bool split_pairs_share_xcc(hipStream_t s);
 // Ciphertexts the split variant runs with one workgroup per CU (CUs / 2).
 int blind_rotate_split_slots();
 // Resident workgroups of the throughput kernel (CUs x workgroups per CU).

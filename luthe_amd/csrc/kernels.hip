@@ -391,9 +391,16 @@ __global__ __launch_bounds__(512, TP ? 2 : 1) void k_blind_rotate_lat(const uint
 // NTTs of its L digit polynomials (ntt2048x4.h, four values per thread), output column h of
 // the external product and its inverse NTT.  The product also needs the partner's digits:
 // thread t of one workgroup sends its NTT-domain digits to thread t of the other through
-// global memory, as 64-bit words of 32 data bits under a 32-bit tag (CMUX index + 1), stored
-// and polled with relaxed agent-scope atomics.  A word whose tag matches carries its own
-// data, so neither fences nor flags are needed.  The two parities of the exchange area
+// global memory, as 64-bit words of 32 data bits under a 32-bit tag (CMUX index + 1), polled
+// with relaxed agent-scope atomic loads (which miss the CU's L1 and read the L2).  A word whose
+// tag matches carries its own data, so neither fences nor flags are needed.  LOCAL: the stores
+// are relaxed atomics of workgroup scope, written through the L1 into the L2 of the XCC that
+// both workgroups run on, where the partner's loads find them (on MI355X a partner's word
+// arrives about 0.4 us per CMUX sooner than with agent-scope stores, which go past the L2);
+// otherwise they are agent-scope stores.  LOCAL needs both workgroups of every pair on one
+// XCC, which the host checks once per context (split_pairs_share_xcc); the pair also checks
+// it at the start of each launch: each publishes its XCC id (hardware register XCC_ID) in a
+// handshake word with agent-scope atomics and reads its partner's.  The two parities of the exchange area
 // alternate per CMUX; a workgroup overwrites parity i & 1 at CMUX i + 2 only after receiving
 // its partner's CMUX i + 1 words, which the partner sends after it has read CMUX i.  The area
 // must hold no tag of this launch when the launch starts (the launcher zeroes it first).
@@ -408,12 +415,31 @@ __global__ __launch_bounds__(512, TP ? 2 : 1) void k_blind_rotate_lat(const uint
 constexpr uint64_t SPLIT_WAIT_TICKS = 200000000ull;  // 2 s
 
 template <int L>
-constexpr size_t split_words() {  // [parity 2][side 2][L][j 4][half 2][512 threads]
+constexpr size_t split_xch_words() {  // [parity 2][side 2][L][j 4][half 2][512 threads]
   return (size_t)2 * 2 * L * 8 * ntt4::T4;
 }
+template <int L>
+constexpr size_t split_words() {  // the exchange words, then 8 words: the handshake [side]
+  return split_xch_words<L>() + 8;
+}
 
-__device__ __forceinline__ void split_put(uint64_t* p, uint64_t tag, uint32_t x) {
-  __hip_atomic_store(p, tag | x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// This thread's 8 L tagged words of its digits v to snd (word k at snd[k * T4]); LOCAL: the
+// partner runs on this workgroup's XCC (the words stay in the shared L2).
+template <int L, bool LOCAL>
+__device__ __forceinline__ void split_send(uint64_t* snd, uint64_t tag, const uint64_t (*v)[4]) {
+#pragma unroll
+  for (int l = 0; l < L; l++)
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+      for (int q = 0; q < 2; q++) {
+        const uint64_t w = tag | (uint32_t)(v[l][j] >> (32 * q));
+        uint64_t* p = snd + ((l * 4 + j) * 2 + q) * ntt4::T4;
+        if (LOCAL)
+          __hip_atomic_store(p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        else
+          __hip_atomic_store(p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
 }
 
 __global__ __launch_bounds__(256) void k_zero_words(uint64_t* __restrict__ p, size_t n) {
@@ -502,7 +528,7 @@ __device__ __forceinline__ void inv_passes_x4(uint64_t* v, uint64_t* p, uint64_t
 // NTT-domain indices 64k + w1 + 8 (wd + 2j) (ntt2048x4.h's C3 map), so it reads the key words
 // of thread t & 255 of the blind-rotation key layout, rows w0 = wd + 2j, and exchanges 8 L
 // words with thread t of its partner.
-template <int L, bool HI>
+template <int L, bool HI, bool LOCAL>
 __global__ __launch_bounds__(512, 1) void k_blind_rotate_split(const uint64_t* __restrict__ bsk,
                                                                const uint64_t* __restrict__ luts,
                                                                const uint16_t* __restrict__ lwe2n,
@@ -528,8 +554,35 @@ __global__ __launch_bounds__(512, 1) void k_blind_rotate_split(const uint64_t* _
   uint64_t* xc = xch + (size_t)c * split_words<L>();
   ntt4::Tw4 tw;
   tw.load(tab, t);
-  if (t == 0) abort_ = 0;
   const uint64_t wait_ticks = a.split_wait_us < 0 ? SPLIT_WAIT_TICKS : (uint64_t)a.split_wait_us * 100;
+  if (t == 0) abort_ = 0;
+  if (LOCAL && t == 0) {
+    // handshake: bit 63 marks a published word, bits 0..3 the XCC id; a partner on another
+    // XCC would not see this workgroup's words, so the pair gives up as for a silent partner
+    uint32_t xcc;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(xcc));
+    xcc &= 15u;
+    uint64_t* hs = xc + split_xch_words<L>();
+    __hip_atomic_store(hs + h, (1ull << 63) | xcc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint64_t pw, deadline = 0;
+    int ab = 0;
+    for (;;) {
+      pw = __hip_atomic_load(hs + (1 - h), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (pw >> 63) break;
+      const uint64_t now = __builtin_amdgcn_s_memrealtime();
+      if (deadline == 0) deadline = now + wait_ticks;
+      if (now >= deadline) {
+        __hip_atomic_store(status, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        ab = 1;
+        break;
+      }
+    }
+    if (!ab && (uint32_t)(pw & 15u) != xcc) {
+      __hip_atomic_store(status, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      ab = 1;
+    }
+    abort_ = ab;
+  }
   uint64_t acc[4];
 #pragma unroll
   for (int i = 0; i < 4; i++) acc[i] = lut[h * N + 64 * (m0 + 4 * (2 * g + (i & 1) + 4 * (i >> 1))) + r];
@@ -571,13 +624,7 @@ __global__ __launch_bounds__(512, 1) void k_blind_rotate_split(const uint64_t* _
     // send this polynomial's digit rows h * L + l
     const uint64_t tag = (uint64_t)(i + 1) << 32;
     uint64_t* snd = xc + (size_t)((i & 1) * 2 + h) * L * 8 * T4 + t;
-#pragma unroll
-    for (int l = 0; l < L; l++)
-#pragma unroll
-      for (int j = 0; j < 4; j++) {
-        split_put(snd + ((l * 4 + j) * 2) * T4, tag, (uint32_t)v[l][j]);
-        split_put(snd + ((l * 4 + j) * 2 + 1) * T4, tag, (uint32_t)(v[l][j] >> 32));
-      }
+    split_send<L, LOCAL>(snd, tag, v);
     uint64_t lo[4], hi[4];
     uint32_t cc[4];
 #pragma unroll
@@ -870,12 +917,15 @@ static int br_lat_max_grid() {
 template <int L, bool HI>
 static void br_launch(const uint64_t* bsk, const uint64_t* luts, const uint16_t* lwe2n, const PbsJob* jobs, int nct,
                       const uint64_t* tab, const BrArgs& a, int grid_override, hipStream_t s,
-                      bool latency, bool pair, uint64_t* xch, uint32_t* status) {
+                      bool latency, bool pair, uint64_t* xch, uint32_t* status, bool split_local) {
   if (xch) {
     const size_t words = (size_t)nct * split_words<L>();
     k_zero_words<<<dim3((unsigned)std::min<size_t>((words + 255) / 256, 1024)), 256, 0, s>>>(xch, words);
-    k_blind_rotate_split<L, HI><<<dim3((unsigned)(16 * ((nct + 7) / 8))), 512, 0, s>>>(bsk, luts, lwe2n, jobs, nct,
-                                                                                         tab, a, xch, status);
+    const dim3 grid((unsigned)(16 * ((nct + 7) / 8)));
+    if (split_local)
+      k_blind_rotate_split<L, HI, true><<<grid, 512, 0, s>>>(bsk, luts, lwe2n, jobs, nct, tab, a, xch, status);
+    else
+      k_blind_rotate_split<L, HI, false><<<grid, 512, 0, s>>>(bsk, luts, lwe2n, jobs, nct, tab, a, xch, status);
   } else if (latency) {
     int g = br_lat_max_grid<2, L, false>();
     g = g < nct ? g : nct;
@@ -894,7 +944,7 @@ static void br_launch(const uint64_t* bsk, const uint64_t* luts, const uint16_t*
 hipError_t launch_blind_rotate(const uint64_t* bsk, const uint64_t* luts, const uint16_t* lwe2n, const PbsJob* jobs,
                                int nct, const uint64_t* tab, const BrArgs& a,
                                int pbs_level, int grid_override, hipStream_t s, int latency_max, bool pair,
-                               int split_max, uint64_t* xch, uint32_t* status) {
+                               int split_max, uint64_t* xch, uint32_t* status, bool split_local) {
   if (nct == 0) return hipSuccess;
   if (pbs_level != 1 && pbs_level != 2) return hipErrorInvalidValue;
   // the batched kernel addresses the key through a buffer resource of at most 2^31 - 1 bytes
@@ -904,16 +954,39 @@ hipError_t launch_blind_rotate(const uint64_t* bsk, const uint64_t* luts, const 
   const bool hi = (uint32_t)pbs_level * a.base_log <= 31 && a.base_log <= 23;
   if (pbs_level == 1) {
     if (hi)
-      br_launch<1, true>(bsk, luts, lwe2n, jobs, nct, tab, a, grid_override, s, latency, pair, xch, status);
+      br_launch<1, true>(bsk, luts, lwe2n, jobs, nct, tab, a, grid_override, s, latency, pair, xch, status, split_local);
     else
-      br_launch<1, false>(bsk, luts, lwe2n, jobs, nct, tab, a, grid_override, s, latency, pair, xch, status);
+      br_launch<1, false>(bsk, luts, lwe2n, jobs, nct, tab, a, grid_override, s, latency, pair, xch, status, split_local);
   } else {
     if (hi)
-      br_launch<2, true>(bsk, luts, lwe2n, jobs, nct, tab, a, grid_override, s, latency, pair, xch, status);
+      br_launch<2, true>(bsk, luts, lwe2n, jobs, nct, tab, a, grid_override, s, latency, pair, xch, status, split_local);
     else
-      br_launch<2, false>(bsk, luts, lwe2n, jobs, nct, tab, a, grid_override, s, latency, pair, xch, status);
+      br_launch<2, false>(bsk, luts, lwe2n, jobs, nct, tab, a, grid_override, s, latency, pair, xch, status, split_local);
   }
   return hipGetLastError();
+}
+
+// Whether workgroups b and b ^ 8 of a split-kernel grid (512 threads, up to 256 workgroups)
+// run on one XCC: a probe kernel records each workgroup's XCC id.  false on any error.
+__global__ __launch_bounds__(512) void k_xcc_ids(uint32_t* __restrict__ out) {
+  uint32_t x;
+  asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(x));
+  if (threadIdx.x == 0) out[blockIdx.x] = x & 15u;
+}
+bool split_pairs_share_xcc(hipStream_t s) {
+  constexpr int G = 256;
+  uint32_t* d = nullptr;
+  uint32_t h[G];
+  if (hipMalloc(&d, G * sizeof(uint32_t)) != hipSuccess) return false;
+  bool ok = false;
+  k_xcc_ids<<<dim3(G), 512, 0, s>>>(d);
+  if (hipGetLastError() == hipSuccess && hipMemcpyAsync(h, d, sizeof h, hipMemcpyDeviceToHost, s) == hipSuccess &&
+      hipStreamSynchronize(s) == hipSuccess) {
+    ok = true;
+    for (int b = 0; b < G; b++) ok = ok && h[b] == h[b ^ 8];
+  }
+  (void)hipFree(d);
+  return ok;
 }
 
 size_t blind_rotate_split_words(int pbs_level) { return pbs_level == 1 ? split_words<1>() : split_words<2>(); }

@@ -305,7 +305,7 @@ int run_pbs_on(luthe_ctx* c, const PbsJob* d_jobs, int n, hipStream_t s,
   lk::BrArgs ba{(int)p.lwe_dim, c->lwe_stride, p.pbs_base_log, c->br_split_wait_us};
   HIPCHK(c, lk::launch_blind_rotate(c->d_bsk, c->d_luts, lwe2n, d_jobs, n, c->d_tab, ba,
                                     (int)p.pbs_level, c->br_grid_override, s, latency_max(c), use_pair(c),
-                                    split_max(c), xch, c->d_status));
+                                    split_max(c), xch, c->d_status, c->split_local));
   {
     std::lock_guard<std::mutex> g(c->cnt_mu);  // an attached server's thread counts here too
     c->cnt.pbs += (uint64_t)n;
@@ -413,6 +413,7 @@ int luthe_create(const luthe_params* p, int device, luthe_ctx** out) {
   ntt::make_twist_tables(tab.data());
   if ((rc = dev_alloc(c, (void**)&c->d_tab, tab.size() * 8))) return bail(rc);
   if (hipMemcpy(c->d_tab, tab.data(), tab.size() * 8, hipMemcpyHostToDevice) != hipSuccess) return bail(LUTHE_ERR_HIP);
+  c->split_local = c->split_local_ok = lk::split_pairs_share_xcc(c->stream);
   *out = c;
   return LUTHE_OK;
 }
@@ -469,6 +470,20 @@ int luthe_set_br_latency_max(luthe_ctx* c, int max_cohort) {
   if (c->server) return fail(c, LUTHE_ERR_BUSY, "keys and launch settings are fixed while a server is attached");
   c->br_latency_max = max_cohort;
   return LUTHE_OK;
+}
+
+int luthe_set_br_split_local(luthe_ctx* c, int local) {
+  if (!c || (local != 0 && local != 1)) return LUTHE_ERR_ARG;
+  if (c->server) return fail(c, LUTHE_ERR_BUSY, "keys and launch settings are fixed while a server is attached");
+  if (local && !c->split_local_ok)
+    return fail(c, LUTHE_ERR_STATE, "the split blind rotation's workgroup pairs do not share an XCC on this device");
+  c->split_local = local != 0;
+  return LUTHE_OK;
+}
+
+int luthe_get_br_split_local(luthe_ctx* c) {
+  if (!c) return LUTHE_ERR_ARG;
+  return c->split_local ? 1 : 0;
 }
 
 int luthe_get_br_latency_max(luthe_ctx* c) {
@@ -643,7 +658,8 @@ int luthe_blind_rotate_batch(luthe_ctx* c, const uint16_t* d_lwe2n, uint64_t* d_
   lk::BrArgs ba{(int)p.lwe_dim, (int)p.lwe_dim + 1, p.pbs_base_log, c->br_split_wait_us};
   HIPCHK(c, lk::launch_blind_rotate(c->d_bsk, c->d_luts, d_lwe2n, c->d_jobs, (int)n, c->d_tab, ba,
                                     (int)p.pbs_level, c->br_grid_override, s, latency_max(c), use_pair(c),
-                                    split_max(c), n <= c->bs.xch_cap ? c->bs.xch : nullptr, c->d_status));
+                                    split_max(c), n <= c->bs.xch_cap ? c->bs.xch : nullptr, c->d_status,
+                                    c->split_local));
   return LUTHE_OK;
 }
 

@@ -59,6 +59,28 @@ def test_split_equals_latency_kernel(env):
         assert np.array_equal(a, b), lg
 
 
+def test_split_exchange_paths(env):
+    """Both exchange paths (the shared XCC's L2, the default on MI355X, and agent-scope stores)
+    give the latency kernel's bits; an invalid setting is refused."""
+    level, _, prm, keys, ctx, _, rng = env
+    lib, h = la.lib(), ctx.handle
+    assert lib.luthe_get_br_split_local(h) == 1  # MI355X places workgroups b and b ^ 8 on one XCC
+    smax = ctx.tuning()["br_split_max"]
+    for n in (1, 9):
+        cts = keys.encrypt_msg(rng.integers(0, 16, n), seed=1500 + n)
+        gids = rng.integers(0, 8, n).astype(np.uint32)
+        ref = None
+        for local in (0, 1):
+            print(f"case: split exchange path local={local}, L={level}, batch {n}", flush=True)
+            assert lib.luthe_set_br_split_local(h, local) == 0
+            a, b = _both(ctx, smax, cts, gids)
+            assert np.array_equal(a, b), (n, local)
+            ref = a if ref is None else ref
+            assert np.array_equal(a, ref), (n, local)
+    assert lib.luthe_set_br_split_local(h, 2) == -1  # LUTHE_ERR_ARG
+    assert lib.luthe_get_br_split_local(h) == 1
+
+
 def test_split_vs_oracle(env, oracle):
     level, base_log, prm, keys, ctx, luts, _ = env
     print(f"case: split blind rotation against the oracle, L={level}", flush=True)
