@@ -14,9 +14,11 @@ constexpr int KS_MAX_GRID_Y = 65535;  // grid.y limit: launches split their ciph
 }  // namespace
 
 // ------------------------------------------------------------------------------------
-// Key switch.  Workgroup = 32 ciphertexts x 256 output coefficients; K dimension =
-// (k*N mask coefficients) x KSL levels, streamed in chunks of 64 coefficients whose
-// digits are decomposed once into LDS and broadcast to all threads.
+// Key switch.  Workgroup = CT ciphertexts x 256 output coefficients; K dimension =
+// (k*N mask coefficients) x KSL levels, streamed in chunks of CHUNK coefficients whose
+// digits are decomposed once into LDS and broadcast to all threads.  CT = 32, CHUNK = 64; for
+// batches of at most 16 ciphertexts CT = 16, CHUNK = 32, and of at most 4 CT = 4, CHUNK = 16
+// (no MACs for empty ciphertext slots, and the split below can go two or four times finer).
 // PART (small batches): workgroup z of gridDim.z takes only the mask coefficients
 // [z * kN / gridDim.z, (z + 1) * kN / gridDim.z) and writes its sums to part[z][c][j];
 // k_ks_finish adds the gridDim.z partial sums (mod 2^32, in a fixed order) and finishes.
@@ -24,12 +26,22 @@ constexpr int KS_MAX_GRID_Y = 65535;  // grid.y limit: launches split their ciph
 // ------------------------------------------------------------------------------------
 constexpr int KS_CT = 32;
 constexpr int KS_CHUNK = 64;
+// (CT, CHUNK, largest split) of a batch of nct ciphertexts
+struct KsShape {
+  int ct, chunk, smax;
+};
+inline KsShape ks_shape(int nct) {
+  if (nct <= 4) return KsShape{4, 16, 128};
+  if (nct <= 16) return KsShape{16, 32, 64};
+  return KsShape{KS_CT, KS_CHUNK, 32};
+}
 
-template <int KSL, bool PART>
+template <int KSL, bool PART, int CT = KS_CT, int CHUNK = KS_CHUNK>
 __global__ __launch_bounds__(256) void k_keyswitch(const PbsJob* __restrict__ jobs,
                                                    int nct, const uint32_t* __restrict__ ksk,
                                                    uint16_t* __restrict__ lwe2n, uint32_t* __restrict__ part,
                                                    KsArgs a) {
+  constexpr int KS_CT = CT, KS_CHUNK = CHUNK;
   __shared__ int32_t dig[KS_CHUNK][KSL][KS_CT / 4];  // 4 signed 8-bit digits per word
   const int t = threadIdx.x;
   const int c0 = blockIdx.y * KS_CT;
@@ -43,10 +55,11 @@ __global__ __launch_bounds__(256) void k_keyswitch(const PbsJob* __restrict__ jo
   const int i_begin = PART ? (int)blockIdx.z * span : 0, i_end = i_begin + span;
   for (int i0 = i_begin; i0 < i_end; i0 += KS_CHUNK) {
     __syncthreads();
-    // 32 x 64 coefficients, 8 per thread; a row's coefficients are contiguous.
+    // CT x CHUNK coefficients (32 x 64: 8 per thread); a row's coefficients are contiguous.
 #pragma unroll
-    for (int e = 0; e < (KS_CT * KS_CHUNK) / 256; e++) {
+    for (int e = 0; e < (KS_CT * KS_CHUNK + 255) / 256; e++) {
       int idx = e * 256 + t;
+      if (KS_CT * KS_CHUNK % 256 != 0 && idx >= KS_CT * KS_CHUNK) break;
       int c = idx / KS_CHUNK, ii = idx % KS_CHUNK;
       uint64_t x = 0;
       if (c0 + c < nct && i0 + ii < a.kN) {
@@ -111,6 +124,7 @@ __global__ __launch_bounds__(256) void k_ks_finish(const PbsJob* __restrict__ jo
   const int j = blockIdx.x * 256 + threadIdx.x, c = blockIdx.y;
   if (j >= n1 || c >= nct) return;
   uint32_t acc = 0;
+#pragma unroll 8
   for (int z = 0; z < nsplit; z++) acc += part[((size_t)z * nct + c) * n1 + j];
   uint32_t body = 0;
   if (j == a.n_lwe) {
@@ -123,10 +137,11 @@ __global__ __launch_bounds__(256) void k_ks_finish(const PbsJob* __restrict__ jo
 int keyswitch_split(int nct, int kN, int n_lwe) {
   // A workgroup's time is its pass over all k*N inputs, whatever the batch: small batches
   // split that pass until the launch has up to 512 workgroups (2 per CU; measured: splitting an
-  // 8192-ciphertext batch in two is slower, 3.4 -> 4.3 ms); each keeps >= one 64-coefficient chunk.
-  const long wgs = (long)((n_lwe + 1 + 255) / 256) * ((nct + KS_CT - 1) / KS_CT);
+  // 8192-ciphertext batch in two is slower, 3.4 -> 4.3 ms); each keeps >= one chunk.
+  const KsShape sh = ks_shape(nct);
+  const long wgs = (long)((n_lwe + 1 + 255) / 256) * ((nct + sh.ct - 1) / sh.ct);
   int s = 1;
-  while (s < 32 && wgs * s * 2 <= 512 && (kN % (s * 2 * KS_CHUNK)) == 0) s *= 2;
+  while (s < sh.smax && wgs * s * 2 <= 512 && (kN % (s * 2 * sh.chunk)) == 0) s *= 2;
   return s;
 }
 
@@ -134,13 +149,30 @@ hipError_t launch_keyswitch(const PbsJob* jobs, int nct, const uint32_t* ksk, ui
                             const KsArgs& a, int ks_level, hipStream_t s, uint32_t* part, int nsplit) {
   if (ks_level < 1 || ks_level > 8) return hipErrorInvalidValue;
   if (nsplit > 1) {
-    if (!part || nct > KS_MAX_GRID_Y || a.kN % (nsplit * KS_CHUNK)) return hipErrorInvalidValue;
-    dim3 grid((unsigned)((a.n_lwe + 1 + 255) / 256), (unsigned)((nct + KS_CT - 1) / KS_CT), (unsigned)nsplit);
-    switch (ks_level) {
+    const KsShape sh = ks_shape(nct);
+    if (!part || nct > KS_MAX_GRID_Y || a.kN % (nsplit * sh.chunk)) return hipErrorInvalidValue;
+    dim3 grid((unsigned)((a.n_lwe + 1 + 255) / 256), (unsigned)((nct + sh.ct - 1) / sh.ct), (unsigned)nsplit);
+    if (sh.ct == 4) {
+      switch (ks_level) {
+#define KS_SCASE(L) \
+  case L: k_keyswitch<L, true, 4, 16><<<grid, 256, 0, s>>>(jobs, nct, ksk, nullptr, part, a); break;
+        KS_SCASE(1) KS_SCASE(2) KS_SCASE(3) KS_SCASE(4) KS_SCASE(5) KS_SCASE(6) KS_SCASE(7) KS_SCASE(8)
+#undef KS_SCASE
+      }
+    } else if (sh.ct == 16) {
+      switch (ks_level) {
+#define KS_MCASE(L) \
+  case L: k_keyswitch<L, true, 16, 32><<<grid, 256, 0, s>>>(jobs, nct, ksk, nullptr, part, a); break;
+        KS_MCASE(1) KS_MCASE(2) KS_MCASE(3) KS_MCASE(4) KS_MCASE(5) KS_MCASE(6) KS_MCASE(7) KS_MCASE(8)
+#undef KS_MCASE
+      }
+    } else {
+      switch (ks_level) {
 #define KS_PCASE(L) \
   case L: k_keyswitch<L, true><<<grid, 256, 0, s>>>(jobs, nct, ksk, nullptr, part, a); break;
-      KS_PCASE(1) KS_PCASE(2) KS_PCASE(3) KS_PCASE(4) KS_PCASE(5) KS_PCASE(6) KS_PCASE(7) KS_PCASE(8)
+        KS_PCASE(1) KS_PCASE(2) KS_PCASE(3) KS_PCASE(4) KS_PCASE(5) KS_PCASE(6) KS_PCASE(7) KS_PCASE(8)
 #undef KS_PCASE
+      }
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;

@@ -150,7 +150,8 @@ int ensure_ks_part(luthe_ctx* c, PbsScratch& sc, size_t n) {
   if (n <= sc.ks_part_for) return LUTHE_OK;
   // the largest split scratch of any batch size up to n: the split depends on the number of
   // 32-ciphertext tiles only, so the maximum sits at the last size of some tile count
-  size_t need = 0;
+  // (and the finer splits of batches of at most 4 and at most 16 ciphertexts)
+  size_t need = std::max(ks_part_words(c, std::min<size_t>(n, 4)), ks_part_words(c, std::min<size_t>(n, 16)));
   for (size_t tiles = 1; tiles <= (n + 31) / 32; tiles++) {
     const size_t w = ks_part_words(c, std::min(n, tiles * 32));
     if (w == 0) break;  // more tiles: no split

@@ -2,7 +2,8 @@
 the VALU key switch bit for bit, and sampled rows against the oracle, over parameter sets
 that exercise every digit width the int8 path admits (balanced digits up to +-64), the
 widest KSK modulus (2^24), an output width that is not a multiple of the 128-column tile,
-and batch sizes around the 64-row tile (a single row, partial tiles, several tiles)."""
+and batch sizes around the 64-row tile (a single row, partial tiles, several tiles) and on
+each tier of the VALU kernel's small-batch shapes (<= 4, <= 16, more ciphertexts)."""
 import os
 
 import numpy as np
@@ -39,7 +40,7 @@ def test_mfma_keyswitch_matches(oracle, over):
     ctx.upload_ksk(keys.ksk)
     kw = {k: v for k, v in over.items() if k in ("lwe_dim", "ks_base_log", "ks_level", "ks_mod_log")}
     rp = oracle.params(**kw)
-    for n in (1, 3, 64, 65, 200):
+    for n in (1, 3, 5, 16, 17, 64, 65, 200):
         blwe = _inputs(prm, n, 62 + n)
         ctx.set_tuning(ks_mfma_min=1)
         got = ctx.keyswitch_batch(blwe)
