@@ -88,6 +88,12 @@ size_t blind_rotate_split_words(int pbs_level);
 // Whether the device places workgroups b and b ^ 8 of a split grid on one XCC (a probe kernel
 // on stream s, synchronous; false on any error).
 bool split_pairs_share_xcc(hipStream_t s);
+// The split variant's launch (split.hip): clears the pairs' exchange area (xch, nct x
+// blind_rotate_split_words(pbs_level) words), then the pairs; hi and local select the
+// template instance as in launch_blind_rotate.
+void launch_blind_rotate_split(int pbs_level, bool hi, bool local, const uint64_t* bsk, const uint64_t* luts,
+                               const uint16_t* lwe2n, const PbsJob* jobs, int nct, const uint64_t* tab,
+                               const BrArgs& a, hipStream_t s, uint64_t* xch, uint32_t* status);
 // Ciphertexts the split variant runs with one workgroup per CU (CUs / 2).
 int blind_rotate_split_slots();
 // Resident workgroups of the throughput kernel (CUs x workgroups per CU).

@@ -58,18 +58,19 @@ __global__ __launch_bounds__(256) void k_keyswitch(const PbsJob* __restrict__ jo
     // CT x CHUNK coefficients (32 x 64: 8 per thread); a row's coefficients are contiguous.
 #pragma unroll
     for (int e = 0; e < (KS_CT * KS_CHUNK + 255) / 256; e++) {
-      int idx = e * 256 + t;
-      if (KS_CT * KS_CHUNK % 256 != 0 && idx >= KS_CT * KS_CHUNK) break;
-      int c = idx / KS_CHUNK, ii = idx % KS_CHUNK;
-      uint64_t x = 0;
-      if (c0 + c < nct && i0 + ii < a.kN) {
-        const uint64_t in = jobs[c0 + c].in_addr;  // 0: an empty slot of a batch list
-        if (in) x = reinterpret_cast<const uint64_t*>(in)[i0 + ii];
+      const int idx = e * 256 + t;
+      if (KS_CT * KS_CHUNK % 256 == 0 || idx < KS_CT * KS_CHUNK) {
+        const int c = idx / KS_CHUNK, ii = idx % KS_CHUNK;
+        uint64_t x = 0;
+        if (c0 + c < nct && i0 + ii < a.kN) {
+          const uint64_t in = jobs[c0 + c].in_addr;  // 0: an empty slot of a batch list
+          if (in) x = reinterpret_cast<const uint64_t*>(in)[i0 + ii];
+        }
+        int64_t d[KSL];
+        pbs::decompose<KSL>(x, a.base_log, d);
+  #pragma unroll
+        for (int l = 0; l < KSL; l++) dig8[(ii * KSL + l) * KS_CT + c] = (int8_t)d[l];
       }
-      int64_t d[KSL];
-      pbs::decompose<KSL>(x, a.base_log, d);
-#pragma unroll
-      for (int l = 0; l < KSL; l++) dig8[(ii * KSL + l) * KS_CT + c] = (int8_t)d[l];
     }
     __syncthreads();
     if (j < n1) {
