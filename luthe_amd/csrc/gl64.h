@@ -36,6 +36,16 @@ GL_HD uint64_t add(uint64_t a, uint64_t b) {
   return s + (uint64_t)x;
 }
 
+// a + b (mod p) for a + b < 2p (canonical a and b), folded into 64 bits but not necessarily
+// canonical: the wrap of the 64-bit sum is folded (it is then a + b - 2^64 < p - EPS, so adding
+// EPS stays below 2^64), the result can lie in [p, 2^64).  For sums that only feed a product
+// (mul, mul_full: any 64-bit operand): one compare and one select fewer than add().
+GL_HD uint64_t add_wrap(uint64_t a, uint64_t b) {
+  uint64_t s;
+  const bool wrapped = __builtin_add_overflow(a, b, &s);
+  return s + (uint64_t)(wrapped ? 0xFFFFFFFFu : 0u);
+}
+
 // t0 + hl * EPS (mod p), canonical, for any 64-bit t0 and hl < 2^32 (the tail of every
 // reduction: hl * EPS <= p - 2^32, so the sum is < 2p and add()'s rule applies).  On the
 // device the product and the sum are one v_mad_u64_u32, whose carry-out is the wrap flag;

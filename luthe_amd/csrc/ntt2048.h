@@ -62,7 +62,9 @@ GL_HD int idx_u(int k, int w1, int r0) {
 
 // DFT-8 in natural order with root 2^S1 (S1 = 24).  Each radix-2 stage forms its four sums
 // first, then its four differences in one gl::sub4 (the operands are dead after it).
-template <int S1>
+// CANON = false: the last stage's sums are gl::add_wrap (outputs below 2^64 but possibly >= p),
+// for outputs that only feed products.
+template <int S1, bool CANON = true>
 GL_HD void dft8(uint64_t* x) {
   uint64_t s0, s1, s2, s3, d0, d1, d2, d3;
   // stage 1: pairs (j, j+4), twiddle root^j
@@ -91,14 +93,19 @@ GL_HD void dft8(uint64_t* x) {
   }
   // stage 3: pairs (2j, 2j+1); the outputs land bit-reversed (1 <-> 4, 3 <-> 6), which
   // the assignment below undoes
-  s0 = gl::add(x[0], x[1]); s1 = gl::add(x[2], x[3]); s2 = gl::add(x[4], x[5]); s3 = gl::add(x[6], x[7]);
+  if (CANON) {
+    s0 = gl::add(x[0], x[1]); s1 = gl::add(x[2], x[3]); s2 = gl::add(x[4], x[5]); s3 = gl::add(x[6], x[7]);
+  } else {
+    s0 = gl::add_wrap(x[0], x[1]); s1 = gl::add_wrap(x[2], x[3]);
+    s2 = gl::add_wrap(x[4], x[5]); s3 = gl::add_wrap(x[6], x[7]);
+  }
   gl::sub4(d0, d1, d2, d3, x[0], x[2], x[4], x[6], x[1], x[3], x[5], x[7]);
   x[0] = s0; x[4] = d0; x[2] = s1; x[6] = d1; x[1] = s2; x[5] = d2; x[3] = s3; x[7] = d3;
 }
 
 // Two DFT-4s in natural order with root 2^S2 (S2 = 48), over x[0..3] and x[4..7]; each of
-// the two radix-2 stages takes its four differences in one gl::sub4.
-template <int S2>
+// the two radix-2 stages takes its four differences in one gl::sub4.  CANON as in dft8.
+template <int S2, bool CANON = true>
 GL_HD void dft4x2(uint64_t* x) {
   uint64_t a0 = gl::add(x[0], x[2]), b0 = gl::add(x[1], x[3]);
   uint64_t c0 = gl::add(x[4], x[6]), e0 = gl::add(x[5], x[7]);
@@ -110,10 +117,17 @@ GL_HD void dft4x2(uint64_t* x) {
     b1 = w[0];
     e1 = w[1];
   }
-  x[0] = gl::add(a0, b0);
-  x[1] = gl::add(a1, b1);
-  x[4] = gl::add(c0, e0);
-  x[5] = gl::add(c1, e1);
+  if (CANON) {
+    x[0] = gl::add(a0, b0);
+    x[1] = gl::add(a1, b1);
+    x[4] = gl::add(c0, e0);
+    x[5] = gl::add(c1, e1);
+  } else {
+    x[0] = gl::add_wrap(a0, b0);
+    x[1] = gl::add_wrap(a1, b1);
+    x[4] = gl::add_wrap(c0, e0);
+    x[5] = gl::add_wrap(c1, e1);
+  }
   gl::sub4(x[2], x[3], x[6], x[7], a0, a1, c0, c1, b0, b1, e0, e1);
 }
 
@@ -125,8 +139,9 @@ GL_HD void swap_(uint64_t& a, uint64_t& b) {
   a = b;
   b = t;
 }
+template <bool CANON = true>
 GL_HD void idft8(uint64_t* x) {
-  dft8<24>(x);
+  dft8<24, CANON>(x);
   swap_(x[1], x[7]);
   swap_(x[2], x[6]);
   swap_(x[3], x[5]);
@@ -190,7 +205,7 @@ GL_HD void fwd_a2_load(uint64_t* v, const uint64_t* lds, int r, int q) {
     for (int m0 = 0; m0 < 4; m0++) v[h * 4 + m0] = lds[idx_y1(2 * q + h, m0, r)];
 }
 GL_HD void fwd_a2_compute(uint64_t* v, int r, int q, const uint64_t* tw) {
-  dft4x2<48>(v);
+  dft4x2<48, false>(v);  // the outputs feed the B twist products
 #pragma unroll
   for (int h = 0; h < 2; h++) {
     uint64_t w[4];
@@ -286,7 +301,10 @@ GL_HD void fwd_c2_load(uint64_t* v, const uint64_t* lds, int k, int w1) {
 #pragma unroll
   for (int r0 = 0; r0 < 8; r0++) v[r0] = lds[idx_u(k, w1, r0)];
 }
-GL_HD void fwd_c2_compute(uint64_t* v) { dft8<24>(v); }  // v[w0] = Z[64k + w1 + 8 w0]
+// v[w0] = Z[64k + w1 + 8 w0], below 2^64 but possibly >= p: the values only feed products (the
+// external product's MAC, the NTT product's pointwise multiply); stored results are made
+// canonical by their kernels
+GL_HD void fwd_c2_compute(uint64_t* v) { dft8<24, false>(v); }
 
 // ---------------------------------------------------------------- inverse passes
 // IC2: inverse DFT-8 over w0 (C2 map); the twiddle omega^(-w1 r0) is applied by IC1.
@@ -314,7 +332,7 @@ GL_HD void inv_c1_compute(uint64_t (*v)[8], int t, const uint64_t* itw, const ui
   const int k = t & 31, h = (t >> 5) & 1, w = t >> 6;
 #pragma unroll
   for (int j = 0; j < NP; j++) {
-    idft8(v[j]);
+    idft8<false>(v[j]);  // the outputs feed the B untwist products
     const int r0 = c1_r0<NP>(j, w, h);
     uint64_t w[8];
 #pragma unroll

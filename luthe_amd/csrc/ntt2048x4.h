@@ -64,7 +64,9 @@ GL_HD int idx_u3(int k, int wb, int wq, int ra) { return 64 * k + ((16 * wb + 4 
 // DFT-4 in natural order with root 2^S (S = 48 forward).  The subtractions go in pairs
 // (gl::sub2): x0 - x2 with x1 - x3, then the reduction of (x1 - x3) 2^S (S in (32, 64): a
 // 128-bit value lo + 2^64 hl + 2^96 hh, reduced as in gl::reduce128) with a0 - b0.
-template <int S>
+// CANON = false: the final sums x[0], x[1] as gl::add_wrap (below 2^64, possibly >= p), for
+// outputs that only feed products.
+template <int S, bool CANON = true>
 GL_HD void dft4(uint64_t* x) {
   static_assert(S > 32 && S < 64, "the paired reduction assumes a shift in (32, 64)");
   const uint64_t a0 = gl::add(x[0], x[2]), b0 = gl::add(x[1], x[3]);
@@ -72,12 +74,13 @@ GL_HD void dft4(uint64_t* x) {
   gl::sub2(a1, d, x[0], x[1], x[2], x[3]);
   gl::sub2(t, x[2], d << S, a0, d >> (96 - S), b0);
   const uint64_t b1 = gl::add_eps_prod(t, (uint32_t)(d >> (64 - S)));
-  x[0] = gl::add(a0, b0);
-  x[1] = gl::add(a1, b1);
+  x[0] = CANON ? gl::add(a0, b0) : gl::add_wrap(a0, b0);
+  x[1] = CANON ? gl::add(a1, b1) : gl::add_wrap(a1, b1);
   x[3] = gl::sub(a1, b1);
 }
+template <bool CANON = true>
 GL_HD void idft4(uint64_t* x) {
-  dft4<48>(x);
+  dft4<48, CANON>(x);
   ntt::swap_(x[1], x[3]);
 }
 
@@ -163,7 +166,7 @@ GL_HD void fwd_a2_load(uint64_t* v, const uint64_t* lds, const Map& m) {
   for (int m0 = 0; m0 < 4; m0++) v[m0] = lds[ntt::idx_y1(m.W, m0, m.r())];
 }
 GL_HD void fwd_a2_compute(uint64_t* v, const uint64_t* tw4) {
-  dft4<48>(v);
+  dft4<48, false>(v);  // the outputs feed the B twist products
   gl::mul4(v, v, tw4);
 }
 GL_HD void fwd_a2_store(const uint64_t* v, uint64_t* lds, const Map& m) {
@@ -177,7 +180,7 @@ GL_HD void fwd_c1_load(uint64_t* v, const uint64_t* lds, const Map& m) {
   for (int rc = 0; rc < 4; rc++) v[rc] = lds[ntt::idx_t(k, rp + 16 * rc)];
 }
 GL_HD void fwd_c1_compute(uint64_t* v, const uint64_t* c1w) {
-  dft4<48>(v);
+  dft4<48, false>(v);  // the outputs feed the twist products
   gl::mul4(v, v, c1w);
 }
 GL_HD void fwd_c1_store(const uint64_t* v, uint64_t* lds, const Map& m) {
@@ -215,7 +218,8 @@ GL_HD void fwd_c3_load(uint64_t* v, const uint64_t* lds, const Map& m) {
 #pragma unroll
   for (int ra = 0; ra < 4; ra++) v[ra] = lds[idx_u3(k, wb, wq, ra)];
 }
-GL_HD void fwd_c3_compute(uint64_t* v) { dft4<48>(v); }
+// outputs below 2^64 but possibly >= p: they only feed the MAC's products
+GL_HD void fwd_c3_compute(uint64_t* v) { dft4<48, false>(v); }
 
 // ---------------------------------------------------------------- inverse passes
 GL_HD void inv_c3_compute(uint64_t* v) { idft4(v); }  // wa -> ra
@@ -255,7 +259,7 @@ GL_HD void inv_c1_load(uint64_t* v, const uint64_t* lds, const Map& m) {
 // 8^(-r' wq) (ic1w), wq -> rc, then the B untwist itw4[rc] = itw((r' + 16 rc), k).
 GL_HD void inv_c1_compute(uint64_t* v, const uint64_t* ic1w, const uint64_t* itw4) {
   gl::mul4(v, v, ic1w);
-  idft4(v);
+  idft4<false>(v);  // the outputs feed the B untwist products
   gl::mul4(v, v, itw4);
 }
 GL_HD void inv_c1_store(const uint64_t* v, uint64_t* lds, const Map& m) {

@@ -86,6 +86,7 @@ void emu_ntt_inv(const uint64_t* in, uint64_t* out) {
 
 uint64_t emu_gl_mul(uint64_t a, uint64_t b) { return gl::mul(a, b); }
 uint64_t emu_gl_add(uint64_t a, uint64_t b) { return gl::add(a, b); }
+uint64_t emu_gl_add_wrap(uint64_t a, uint64_t b) { return gl::add_wrap(a, b); }
 uint64_t emu_gl_sub(uint64_t a, uint64_t b) { return gl::sub(a, b); }
 uint64_t emu_gl_mul_pow2(uint64_t a, uint32_t s) { return gl::mul_pow2(a, s); }
 // every compile-time shift: out[i] = x * 2^(i - 192) for i < 384 (negative amounts included)

@@ -17,10 +17,10 @@ rng = np.random.default_rng(777)
 
 @pytest.fixture(scope="module")
 def emu():
+    import subprocess
+    # make rebuilds the library when the kernel headers or the emulation changed
+    subprocess.check_call(["make", "-s", "-C", os.path.join(HERE, "emu")])
     path = os.path.join(HERE, "emu", "libkernel_emu.so")
-    if not os.path.exists(path):
-        import subprocess
-        subprocess.check_call(["make", "-s", "-C", os.path.join(HERE, "emu")])
     E = C.CDLL(path)
     u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
     E.emu_ntt_fwd.argtypes = [u64p, u64p]
@@ -31,8 +31,9 @@ def emu():
     E.emu_modswitch.argtypes = [C.c_uint64]
     E.emu_gl_reduce160.restype = C.c_uint64
     E.emu_gl_reduce160.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32]
-    for f in ("emu_gl_add", "emu_gl_sub", "emu_gl_mul", "emu_gl_reduce128"):
+    for f in ("emu_gl_add", "emu_gl_add_wrap", "emu_gl_sub", "emu_gl_mul", "emu_gl_reduce128"):
         getattr(E, f).argtypes = [C.c_uint64, C.c_uint64]
+    E.emu_gl_add_wrap.restype = C.c_uint64
     return E
 
 
@@ -43,6 +44,15 @@ def test_device_arithmetic(emu):
     for a, b in zip(canon[:1500], canon[1500:3000]):
         assert emu.emu_gl_add(a, b) == (a + b) % P
         assert emu.emu_gl_sub(a, b) == (a - b) % P
+    # the wrap-only add: congruent, below 2^64, equal to the plain sum when it does not wrap,
+    # and non-canonical exactly when the sum lies in [p, 2^64)
+    edge = [(P - 1, P - 1), (P - 1, 1), (P - 1, 2**32), (2**63, 2**63), (2**63, 2**63 - 2**32),
+            (P - 2**32, 2**32 - 1), (0, 0), (P - 1, 0)]
+    for a, b in edge + list(zip(canon[:1500], canon[1500:3000])):
+        r = emu.emu_gl_add_wrap(a, b)
+        assert r % P == (a + b) % P and r < 2**64
+        assert r == (a + b if a + b < 2**64 else a + b - P)
+    assert emu.emu_gl_add_wrap(P - 1, 5) == P + 4  # in [p, 2^64): left as is
     for a, b in zip(anyv[:1500], anyv[1500:3000]):
         assert emu.emu_gl_mul(a, b) == (a * b) % P
         assert emu.emu_gl_reduce128(a, b) == (a + (b << 64)) % P
