@@ -24,6 +24,9 @@ constexpr uint64_t EPS = 0xFFFFFFFFull;  // 2^64 mod p
 
 GL_HD uint64_t canonical(uint64_t x) { return x >= P ? x - P : x; }
 
+// canonical(x) as x + (EPS or 0): x - p == x + EPS (mod 2^64); the select picks a 32-bit addend
+GL_HD uint64_t canonical_fold(uint64_t x) { return x + (uint64_t)(x > P - 1 ? 0xFFFFFFFFu : 0u); }
+
 // a + b (mod p), canonical, for any 64-bit a, b with a + b < 2p: canonical a and b, or the
 // reductions' any-64-bit a plus b <= p - 2^32.  When the 64-bit sum wrapped (it is then
 // a + b - 2^64 < p - EPS) or is >= p, a + b - p is the 64-bit sum plus EPS (mod 2^64),
@@ -44,6 +47,13 @@ GL_HD uint64_t add_wrap(uint64_t a, uint64_t b) {
   uint64_t s;
   const bool wrapped = __builtin_add_overflow(a, b, &s);
   return s + (uint64_t)(wrapped ? 0xFFFFFFFFu : 0u);
+}
+
+// add_wrap when W is nonzero, add otherwise (per-output choice in the DFT networks)
+template <int W>
+GL_HD uint64_t add_w(uint64_t a, uint64_t b) {
+  if constexpr (W != 0) return add_wrap(a, b);
+  else return add(a, b);
 }
 
 // t0 + hl * EPS (mod p), canonical, for any 64-bit t0 and hl < 2^32 (the tail of every

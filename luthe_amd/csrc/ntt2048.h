@@ -62,9 +62,10 @@ GL_HD int idx_u(int k, int w1, int r0) {
 
 // DFT-8 in natural order with root 2^S1 (S1 = 24).  Each radix-2 stage forms its four sums
 // first, then its four differences in one gl::sub4 (the operands are dead after it).
-// CANON = false: the last stage's sums are gl::add_wrap (outputs below 2^64 but possibly >= p),
-// for outputs that only feed products.
-template <int S1, bool CANON = true>
+// WRAP: bit i set makes the last stage's sum s_i (output x[0], x[2], x[1], x[3] for i = 0..3) a
+// gl::add_wrap (below 2^64 but possibly >= p), for outputs that only feed products or shift
+// twists by a nonzero amount (both take any 64-bit operand).
+template <int S1, int WRAP = 0>
 GL_HD void dft8(uint64_t* x) {
   uint64_t s0, s1, s2, s3, d0, d1, d2, d3;
   // stage 1: pairs (j, j+4), twiddle root^j
@@ -93,19 +94,18 @@ GL_HD void dft8(uint64_t* x) {
   }
   // stage 3: pairs (2j, 2j+1); the outputs land bit-reversed (1 <-> 4, 3 <-> 6), which
   // the assignment below undoes
-  if (CANON) {
-    s0 = gl::add(x[0], x[1]); s1 = gl::add(x[2], x[3]); s2 = gl::add(x[4], x[5]); s3 = gl::add(x[6], x[7]);
-  } else {
-    s0 = gl::add_wrap(x[0], x[1]); s1 = gl::add_wrap(x[2], x[3]);
-    s2 = gl::add_wrap(x[4], x[5]); s3 = gl::add_wrap(x[6], x[7]);
-  }
+  s0 = gl::add_w<WRAP & 1>(x[0], x[1]);
+  s1 = gl::add_w<WRAP & 2>(x[2], x[3]);
+  s2 = gl::add_w<WRAP & 4>(x[4], x[5]);
+  s3 = gl::add_w<WRAP & 8>(x[6], x[7]);
   gl::sub4(d0, d1, d2, d3, x[0], x[2], x[4], x[6], x[1], x[3], x[5], x[7]);
   x[0] = s0; x[4] = d0; x[2] = s1; x[6] = d1; x[1] = s2; x[5] = d2; x[3] = s3; x[7] = d3;
 }
 
 // Two DFT-4s in natural order with root 2^S2 (S2 = 48), over x[0..3] and x[4..7]; each of
-// the two radix-2 stages takes its four differences in one gl::sub4.  CANON as in dft8.
-template <int S2, bool CANON = true>
+// the two radix-2 stages takes its four differences in one gl::sub4.  WRAP as in dft8, over
+// the sums x[0], x[1], x[4], x[5] (bits 0..3).
+template <int S2, int WRAP = 0>
 GL_HD void dft4x2(uint64_t* x) {
   uint64_t a0 = gl::add(x[0], x[2]), b0 = gl::add(x[1], x[3]);
   uint64_t c0 = gl::add(x[4], x[6]), e0 = gl::add(x[5], x[7]);
@@ -117,17 +117,10 @@ GL_HD void dft4x2(uint64_t* x) {
     b1 = w[0];
     e1 = w[1];
   }
-  if (CANON) {
-    x[0] = gl::add(a0, b0);
-    x[1] = gl::add(a1, b1);
-    x[4] = gl::add(c0, e0);
-    x[5] = gl::add(c1, e1);
-  } else {
-    x[0] = gl::add_wrap(a0, b0);
-    x[1] = gl::add_wrap(a1, b1);
-    x[4] = gl::add_wrap(c0, e0);
-    x[5] = gl::add_wrap(c1, e1);
-  }
+  x[0] = gl::add_w<WRAP & 1>(a0, b0);
+  x[1] = gl::add_w<WRAP & 2>(a1, b1);
+  x[4] = gl::add_w<WRAP & 4>(c0, e0);
+  x[5] = gl::add_w<WRAP & 8>(c1, e1);
   gl::sub4(x[2], x[3], x[6], x[7], a0, a1, c0, c1, b0, b1, e0, e1);
 }
 
@@ -139,15 +132,18 @@ GL_HD void swap_(uint64_t& a, uint64_t& b) {
   a = b;
   b = t;
 }
-template <bool CANON = true>
+// WRAP as in dft8 (bit i: the sum that ends in output x[0], x[6], x[7], x[5] for i = 0..3)
+template <int WRAP = 0>
 GL_HD void idft8(uint64_t* x) {
-  dft8<24, CANON>(x);
+  dft8<24, WRAP>(x);
   swap_(x[1], x[7]);
   swap_(x[2], x[6]);
   swap_(x[3], x[5]);
 }
+// WRAP as in dft4x2 (bit i: the sum that ends in output x[0], x[3], x[4], x[7] for i = 0..3)
+template <int WRAP = 0>
 GL_HD void idft4x2(uint64_t* x) {
-  dft4x2<48>(x);
+  dft4x2<48, WRAP>(x);
   swap_(x[1], x[3]);
   swap_(x[5], x[7]);
 }
@@ -183,14 +179,20 @@ GL_HD int wave_uniform(int x) {
 // a compile-time pre-twist psi^(4 m1) = 2^(12 m1) and a post-twist
 // psi^m0 Omega^(k1 m0) = 2^(3 m0 (1 + 2 k1)) that is uniform per wave.
 // pretwisted: v already holds p[n] psi^(4 m1) (the blind rotation's digit lift applies it).
+// The DFT's last sums (outputs 0..3) skip canonicalisation: for m0 > 0 every output is twisted
+// by a nonzero shift, which takes any 64-bit operand; for m0 = 0 (no twist) the four sums are
+// made canonical afterwards, in the wave with the least twist work (m0 is uniform per wave).
 GL_HD void fwd_a1_compute(uint64_t* v, int m0u, bool pretwisted = false) {
   if (!pretwisted) gl::mul_pow2c_n<12, 24, 36, 48, 60, 72, 84>(v + 1);
-  dft8<24>(v);
+  dft8<24, 0xF>(v);
   switch (m0u) {
     case 1: a1_post_twist<1>(v); break;
     case 2: a1_post_twist<2>(v); break;
     case 3: a1_post_twist<3>(v); break;
-    default: break;
+    default:
+#pragma unroll
+      for (int i = 0; i < 4; i++) v[i] = gl::canonical_fold(v[i]);
+      break;
   }
 }
 GL_HD void fwd_a1_store(const uint64_t* v, uint64_t* lds, int r, int m0) {
@@ -205,7 +207,7 @@ GL_HD void fwd_a2_load(uint64_t* v, const uint64_t* lds, int r, int q) {
     for (int m0 = 0; m0 < 4; m0++) v[h * 4 + m0] = lds[idx_y1(2 * q + h, m0, r)];
 }
 GL_HD void fwd_a2_compute(uint64_t* v, int r, int q, const uint64_t* tw) {
-  dft4x2<48, false>(v);  // the outputs feed the B twist products
+  dft4x2<48, 0xF>(v);  // the outputs feed the B twist products
 #pragma unroll
   for (int h = 0; h < 2; h++) {
     uint64_t w[4];
@@ -279,10 +281,18 @@ GL_HD void fwd_c1_load(uint64_t (*v)[8], const uint64_t* lds, int t) {
     }
   }
 }
+// The sums ending in outputs 1..7 (s1..s3) feed the twist: with NP = 1 a product (any 64-bit
+// operand); with NP >= 2 shifts that are nonzero for the odd tasks (r0 = R0_PAIR[w][1] > 0), so
+// those skip canonicalisation.  Output 0 is not twisted and stays canonical.
 template <int NP>
 GL_HD void fwd_c1_compute(uint64_t (*v)[8], int t, const uint64_t* c64) {
 #pragma unroll
-  for (int j = 0; j < NP; j++) dft8<24>(v[j]);
+  for (int j = 0; j < NP; j++) {
+    if (NP == 1 || (j & 1))
+      dft8<24, 0xE>(v[j]);
+    else
+      dft8<24>(v[j]);
+  }
   c_twist_all<1, NP>(v, t, c64);
 }
 template <int NP>
@@ -304,7 +314,7 @@ GL_HD void fwd_c2_load(uint64_t* v, const uint64_t* lds, int k, int w1) {
 // v[w0] = Z[64k + w1 + 8 w0], below 2^64 but possibly >= p: the values only feed products (the
 // external product's MAC, the NTT product's pointwise multiply); stored results are made
 // canonical by their kernels
-GL_HD void fwd_c2_compute(uint64_t* v) { dft8<24, false>(v); }
+GL_HD void fwd_c2_compute(uint64_t* v) { dft8<24, 0xF>(v); }
 
 // ---------------------------------------------------------------- inverse passes
 // IC2: inverse DFT-8 over w0 (C2 map); the twiddle omega^(-w1 r0) is applied by IC1.
@@ -332,7 +342,7 @@ GL_HD void inv_c1_compute(uint64_t (*v)[8], int t, const uint64_t* itw, const ui
   const int k = t & 31, h = (t >> 5) & 1, w = t >> 6;
 #pragma unroll
   for (int j = 0; j < NP; j++) {
-    idft8<false>(v[j]);  // the outputs feed the B untwist products
+    idft8<0xF>(v[j]);  // the outputs feed the B untwist products
     const int r0 = c1_r0<NP>(j, w, h);
     uint64_t w[8];
 #pragma unroll
@@ -365,8 +375,10 @@ GL_HD void inv_a2_load(uint64_t* v, const uint64_t* lds, int r, int q) {
 // IA2: Omega^(-k1 m0) psi^(-m0) = 2^(-3 m0 (2 k1 + 1)), uniform per wave (k1 = 2q + h).
 // The psi^(-m0) part of the final untwist psi^-(m0 + 4 m1) is constant over IA1's
 // DFT-8 and moves here; both stages carry a factor -1 (the cheaper shift forms).
+// Outputs 0 and 4 take the twist -1 (a negation, which needs canonical input); the sums that
+// end in outputs 3 and 7 meet nonzero shifts and skip canonicalisation.
 GL_HD void inv_a2_compute(uint64_t* v, int qu) {
-  idft4x2(v);
+  idft4x2<0xA>(v);
   switch (qu) {
     case 0: ia2_twist<0>(v); break;
     case 1: ia2_twist<1>(v); break;
@@ -386,8 +398,10 @@ GL_HD void inv_a1_load(uint64_t* v, const uint64_t* lds, int r, int m0) {
 }
 // Output v[m1] = p[64 (m0 + 4 m1) + r] (canonical): the remaining untwist psi^(-4 m1)
 // times the -1 that cancels IA2's.
+// Output 0's twist is -1 (needs canonical input); the sums ending in outputs 5, 6, 7 meet
+// nonzero shifts and skip canonicalisation.
 GL_HD void inv_a1_compute(uint64_t* v) {
-  idft8(v);
+  idft8<0xE>(v);
   ia1_twist(v);
 }
 

@@ -64,9 +64,9 @@ GL_HD int idx_u3(int k, int wb, int wq, int ra) { return 64 * k + ((16 * wb + 4 
 // DFT-4 in natural order with root 2^S (S = 48 forward).  The subtractions go in pairs
 // (gl::sub2): x0 - x2 with x1 - x3, then the reduction of (x1 - x3) 2^S (S in (32, 64): a
 // 128-bit value lo + 2^64 hl + 2^96 hh, reduced as in gl::reduce128) with a0 - b0.
-// CANON = false: the final sums x[0], x[1] as gl::add_wrap (below 2^64, possibly >= p), for
-// outputs that only feed products.
-template <int S, bool CANON = true>
+// WRAP: bit i set makes the final sum x[i] (i = 0, 1) a gl::add_wrap (below 2^64, possibly
+// >= p), for outputs that only feed products or shift twists by a nonzero amount.
+template <int S, int WRAP = 0>
 GL_HD void dft4(uint64_t* x) {
   static_assert(S > 32 && S < 64, "the paired reduction assumes a shift in (32, 64)");
   const uint64_t a0 = gl::add(x[0], x[2]), b0 = gl::add(x[1], x[3]);
@@ -74,13 +74,14 @@ GL_HD void dft4(uint64_t* x) {
   gl::sub2(a1, d, x[0], x[1], x[2], x[3]);
   gl::sub2(t, x[2], d << S, a0, d >> (96 - S), b0);
   const uint64_t b1 = gl::add_eps_prod(t, (uint32_t)(d >> (64 - S)));
-  x[0] = CANON ? gl::add(a0, b0) : gl::add_wrap(a0, b0);
-  x[1] = CANON ? gl::add(a1, b1) : gl::add_wrap(a1, b1);
+  x[0] = gl::add_w<WRAP & 1>(a0, b0);
+  x[1] = gl::add_w<WRAP & 2>(a1, b1);
   x[3] = gl::sub(a1, b1);
 }
-template <bool CANON = true>
+// WRAP as in dft4 (bit 0: output x[0], bit 1: output x[3])
+template <int WRAP = 0>
 GL_HD void idft4(uint64_t* x) {
-  dft4<48, CANON>(x);
+  dft4<48, WRAP>(x);
   ntt::swap_(x[1], x[3]);
 }
 
@@ -139,7 +140,7 @@ GL_HD void fwd_a1b_load(uint64_t* v, const uint64_t* lds, const Map& m) {
 }
 template <int M0, int S>
 GL_HD void a1b(uint64_t* v) {
-  dft4<48>(v);
+  dft4<48, M0 == 0 ? 0 : 3>(v);  // M0 > 0: every output meets a nonzero shift
   gl::mul_pow2c_n<3 * M0 * (1 + 2 * S), 3 * M0 * (1 + 2 * (2 + S)), 3 * M0 * (1 + 2 * (4 + S)),
                   3 * M0 * (1 + 2 * (6 + S))>(v);
 }
@@ -166,7 +167,7 @@ GL_HD void fwd_a2_load(uint64_t* v, const uint64_t* lds, const Map& m) {
   for (int m0 = 0; m0 < 4; m0++) v[m0] = lds[ntt::idx_y1(m.W, m0, m.r())];
 }
 GL_HD void fwd_a2_compute(uint64_t* v, const uint64_t* tw4) {
-  dft4<48, false>(v);  // the outputs feed the B twist products
+  dft4<48, 3>(v);  // the outputs feed the B twist products
   gl::mul4(v, v, tw4);
 }
 GL_HD void fwd_a2_store(const uint64_t* v, uint64_t* lds, const Map& m) {
@@ -180,7 +181,7 @@ GL_HD void fwd_c1_load(uint64_t* v, const uint64_t* lds, const Map& m) {
   for (int rc = 0; rc < 4; rc++) v[rc] = lds[ntt::idx_t(k, rp + 16 * rc)];
 }
 GL_HD void fwd_c1_compute(uint64_t* v, const uint64_t* c1w) {
-  dft4<48, false>(v);  // the outputs feed the twist products
+  dft4<48, 3>(v);  // the outputs feed the twist products
   gl::mul4(v, v, c1w);
 }
 GL_HD void fwd_c1_store(const uint64_t* v, uint64_t* lds, const Map& m) {
@@ -196,7 +197,7 @@ GL_HD void fwd_c2_load(uint64_t* v, const uint64_t* lds, const Map& m) {
 }
 template <int RA>
 GL_HD void c2(uint64_t* v) {
-  dft4<48>(v);
+  dft4<48, RA == 0 ? 0 : 2>(v);  // output 1 meets a nonzero shift when RA > 0
   gl::mul_pow2c_n<12 * RA, 24 * RA, 36 * RA>(v + 1);
 }
 GL_HD void fwd_c2_compute(uint64_t* v, int rau) {
@@ -219,7 +220,7 @@ GL_HD void fwd_c3_load(uint64_t* v, const uint64_t* lds, const Map& m) {
   for (int ra = 0; ra < 4; ra++) v[ra] = lds[idx_u3(k, wb, wq, ra)];
 }
 // outputs below 2^64 but possibly >= p: they only feed the MAC's products
-GL_HD void fwd_c3_compute(uint64_t* v) { dft4<48, false>(v); }
+GL_HD void fwd_c3_compute(uint64_t* v) { dft4<48, 3>(v); }
 
 // ---------------------------------------------------------------- inverse passes
 GL_HD void inv_c3_compute(uint64_t* v) { idft4(v); }  // wa -> ra
@@ -236,7 +237,7 @@ GL_HD void inv_c2_load(uint64_t* v, const uint64_t* lds, const Map& m) {
 template <int RA>
 GL_HD void ic2(uint64_t* v) {  // 2^(-12 RA wb), then wb -> rb
   gl::mul_pow2c_n<-12 * RA, -24 * RA, -36 * RA>(v + 1);
-  idft4(v);
+  idft4<3>(v);  // the outputs feed inverse C1's products
 }
 GL_HD void inv_c2_compute(uint64_t* v, int rau) {
   switch (rau) {
@@ -259,7 +260,7 @@ GL_HD void inv_c1_load(uint64_t* v, const uint64_t* lds, const Map& m) {
 // 8^(-r' wq) (ic1w), wq -> rc, then the B untwist itw4[rc] = itw((r' + 16 rc), k).
 GL_HD void inv_c1_compute(uint64_t* v, const uint64_t* ic1w, const uint64_t* itw4) {
   gl::mul4(v, v, ic1w);
-  idft4<false>(v);  // the outputs feed the B untwist products
+  idft4<3>(v);  // the outputs feed the B untwist products
   gl::mul4(v, v, itw4);
 }
 GL_HD void inv_c1_store(const uint64_t* v, uint64_t* lds, const Map& m) {
@@ -273,7 +274,7 @@ GL_HD void inv_a2_load(uint64_t* v, const uint64_t* lds, const Map& m) {
 }
 template <int K1>
 GL_HD void ia2(uint64_t* v) {  // k0 -> m0, then 2^(96 - 3 m0 (2 K1 + 1))
-  idft4(v);
+  idft4<2>(v);  // output 3 meets a nonzero shift; output 0 a negation (canonical input)
   gl::mul_pow2c_n<96, 96 - 3 * (2 * K1 + 1), 96 - 6 * (2 * K1 + 1), 96 - 9 * (2 * K1 + 1)>(v);
 }
 GL_HD void inv_a2_compute(uint64_t* v, int Wu) {
@@ -315,7 +316,8 @@ template <int G>
 GL_HD void ia1a(uint64_t* v) {
   uint64_t b[2] = {v[2], v[3]};
   gl::mul_pow2c_n<-24 * (2 * G), -24 * (2 * G + 1)>(b);
-  const uint64_t x0 = gl::add(v[0], b[0]), x1 = gl::add(v[1], b[1]);
+  // both sums meet nonzero shifts below, except x0 for G = 0 (a negation: canonical input)
+  const uint64_t x0 = gl::add_w<G>(v[0], b[0]), x1 = gl::add_wrap(v[1], b[1]);
   gl::sub2(v[2], v[3], v[0], v[1], b[0], b[1]);
   v[0] = x0;
   v[1] = x1;

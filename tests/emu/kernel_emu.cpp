@@ -87,6 +87,7 @@ void emu_ntt_inv(const uint64_t* in, uint64_t* out) {
 uint64_t emu_gl_mul(uint64_t a, uint64_t b) { return gl::mul(a, b); }
 uint64_t emu_gl_add(uint64_t a, uint64_t b) { return gl::add(a, b); }
 uint64_t emu_gl_add_wrap(uint64_t a, uint64_t b) { return gl::add_wrap(a, b); }
+uint64_t emu_gl_canonical_fold(uint64_t x) { return gl::canonical_fold(x); }
 uint64_t emu_gl_sub(uint64_t a, uint64_t b) { return gl::sub(a, b); }
 uint64_t emu_gl_mul_pow2(uint64_t a, uint32_t s) { return gl::mul_pow2(a, s); }
 // every compile-time shift: out[i] = x * 2^(i - 192) for i < 384 (negative amounts included)
@@ -267,9 +268,19 @@ void pow2c_all(uint64_t x, uint64_t* out) {
     pow2c_all<S + 1>(x, out);
   }
 }
+// the batched form the DFT networks use, one lane per amount
+template <int S>
+void pow2c_n_all(uint64_t x, uint64_t* out) {
+  if constexpr (S < 192) {
+    out[S + 192] = x;
+    gl::mul_pow2c_n<S>(out + S + 192);
+    pow2c_n_all<S + 1>(x, out);
+  }
+}
 }  // namespace
 
 extern "C" void emu_gl_mul_pow2c_all(uint64_t x, uint64_t* out) { pow2c_all<-192>(x, out); }
+extern "C" void emu_gl_mul_pow2c_n_all(uint64_t x, uint64_t* out) { pow2c_n_all<-192>(x, out); }
 
 namespace {
 template <int M1>

@@ -34,6 +34,8 @@ def emu():
     for f in ("emu_gl_add", "emu_gl_add_wrap", "emu_gl_sub", "emu_gl_mul", "emu_gl_reduce128"):
         getattr(E, f).argtypes = [C.c_uint64, C.c_uint64]
     E.emu_gl_add_wrap.restype = C.c_uint64
+    E.emu_gl_canonical_fold.restype = C.c_uint64
+    E.emu_gl_canonical_fold.argtypes = [C.c_uint64]
     return E
 
 
@@ -53,6 +55,8 @@ def test_device_arithmetic(emu):
         assert r % P == (a + b) % P and r < 2**64
         assert r == (a + b if a + b < 2**64 else a + b - P)
     assert emu.emu_gl_add_wrap(P - 1, 5) == P + 4  # in [p, 2^64): left as is
+    for x in anyv + [P + 2**32 - 2, 2**64 - 2**32]:
+        assert emu.emu_gl_canonical_fold(x) == x % P
     for a, b in zip(anyv[:1500], anyv[1500:3000]):
         assert emu.emu_gl_mul(a, b) == (a * b) % P
         assert emu.emu_gl_reduce128(a, b) == (a + (b << 64)) % P
@@ -70,10 +74,20 @@ def test_device_arithmetic(emu):
     # reduction (amounts <= 32) and the swapped subtraction (amounts in [64, 96) + 96)
     out = np.zeros(384, np.uint64)
     emu.emu_gl_mul_pow2c_all.argtypes = [C.c_uint64, C.c_void_p]
+    emu.emu_gl_mul_pow2c_n_all.argtypes = [C.c_uint64, C.c_void_p]
     for x in canon[:300]:
-        emu.emu_gl_mul_pow2c_all(x, out.ctypes.data_as(C.c_void_p))
         exp = [(x * pow(2, s % 192, P)) % P for s in range(-192, 192)]
+        emu.emu_gl_mul_pow2c_all(x, out.ctypes.data_as(C.c_void_p))
         assert [int(v) for v in out] == exp, x
+        emu.emu_gl_mul_pow2c_n_all(x, out.ctypes.data_as(C.c_void_p))
+        assert [int(v) for v in out] == exp, x
+    # operands in [p, 2^64) (the DFTs' wrap-only sums): every amount with a nonzero shift part
+    # (S mod 96 != 0) takes them; amounts that are 0 or a pure negation need canonical input
+    for x in [P, P + 1, P + 5, 2**64 - 1, 2**64 - 2**32] + [int(v) for v in rng.integers(P, 2**64, 20, dtype=np.uint64)]:
+        emu.emu_gl_mul_pow2c_n_all(x, out.ctypes.data_as(C.c_void_p))
+        for i, s in enumerate(range(-192, 192)):
+            if s % 96:
+                assert int(out[i]) == (x * pow(2, s % 192, P)) % P, (x, s)
 
 
 def test_kernel_ntt_matches_oracle(emu):
