@@ -64,12 +64,17 @@ GL_HD int idx_u(int k, int w1, int r0) {
 // first, then its four differences in one gl::sub4 (the operands are dead after it).
 // WRAP: bit i set makes the last stage's sum s_i (output x[0], x[2], x[1], x[3] for i = 0..3) a
 // gl::add_wrap (below 2^64 but possibly >= p), for outputs that only feed products or shift
-// twists by a nonzero amount (both take any 64-bit operand).
+// twists by a nonzero amount (both take any 64-bit operand).  Bit 4 (with bits 0..3: every
+// output may be >= p) also wraps the earlier sums that reach the last stage only as its first
+// operands (stage 1's x[0], stage 2's x[0] and x[4]): add_wrap and sub_fold take any 64-bit
+// first operand with a canonical second one.
 template <int S1, int WRAP = 0>
 GL_HD void dft8(uint64_t* x) {
+  static_assert(!(WRAP & 16) || (WRAP & 15) == 15, "bit 4 needs every output non-canonical");
+  constexpr int DEEP = WRAP & 16;
   uint64_t s0, s1, s2, s3, d0, d1, d2, d3;
   // stage 1: pairs (j, j+4), twiddle root^j
-  s0 = gl::add(x[0], x[4]); s1 = gl::add(x[1], x[5]); s2 = gl::add(x[2], x[6]); s3 = gl::add(x[3], x[7]);
+  s0 = gl::add_w<DEEP>(x[0], x[4]); s1 = gl::add(x[1], x[5]); s2 = gl::add(x[2], x[6]); s3 = gl::add(x[3], x[7]);
   gl::sub4(d0, d1, d2, d3, x[0], x[1], x[2], x[3], x[4], x[5], x[6], x[7]);
   x[0] = s0; x[1] = s1; x[2] = s2; x[3] = s3;
   x[4] = d0;
@@ -81,7 +86,8 @@ GL_HD void dft8(uint64_t* x) {
     x[7] = w[2];
   }
   // stage 2: pairs (h+j, h+j+2), twiddle root^(2j)
-  s0 = gl::add(x[0], x[2]); s1 = gl::add(x[1], x[3]); s2 = gl::add(x[4], x[6]); s3 = gl::add(x[5], x[7]);
+  s0 = gl::add_w<DEEP>(x[0], x[2]); s1 = gl::add(x[1], x[3]);
+  s2 = gl::add_w<DEEP>(x[4], x[6]); s3 = gl::add(x[5], x[7]);
   gl::sub4(d0, d1, d2, d3, x[0], x[1], x[4], x[5], x[2], x[3], x[6], x[7]);
   x[0] = s0; x[1] = s1; x[4] = s2; x[5] = s3;
   x[2] = d0;
@@ -104,11 +110,13 @@ GL_HD void dft8(uint64_t* x) {
 
 // Two DFT-4s in natural order with root 2^S2 (S2 = 48), over x[0..3] and x[4..7]; each of
 // the two radix-2 stages takes its four differences in one gl::sub4.  WRAP as in dft8, over
-// the sums x[0], x[1], x[4], x[5] (bits 0..3).
+// the sums x[0], x[1], x[4], x[5] (bits 0..3); bit 4 also wraps the first-stage sums a0 and c0
+// (first operands of the second stage only).
 template <int S2, int WRAP = 0>
 GL_HD void dft4x2(uint64_t* x) {
-  uint64_t a0 = gl::add(x[0], x[2]), b0 = gl::add(x[1], x[3]);
-  uint64_t c0 = gl::add(x[4], x[6]), e0 = gl::add(x[5], x[7]);
+  static_assert(!(WRAP & 16) || (WRAP & 15) == 15, "bit 4 needs every output non-canonical");
+  uint64_t a0 = gl::add_w<WRAP & 16>(x[0], x[2]), b0 = gl::add(x[1], x[3]);
+  uint64_t c0 = gl::add_w<WRAP & 16>(x[4], x[6]), e0 = gl::add(x[5], x[7]);
   uint64_t a1, b1, c1, e1;
   gl::sub4(a1, b1, c1, e1, x[0], x[1], x[4], x[5], x[2], x[3], x[6], x[7]);
   {
@@ -207,7 +215,7 @@ GL_HD void fwd_a2_load(uint64_t* v, const uint64_t* lds, int r, int q) {
     for (int m0 = 0; m0 < 4; m0++) v[h * 4 + m0] = lds[idx_y1(2 * q + h, m0, r)];
 }
 GL_HD void fwd_a2_compute(uint64_t* v, int r, int q, const uint64_t* tw) {
-  dft4x2<48, 0xF>(v);  // the outputs feed the B twist products
+  dft4x2<48, 0x1F>(v);  // the outputs feed the B twist products
 #pragma unroll
   for (int h = 0; h < 2; h++) {
     uint64_t w[4];
@@ -314,7 +322,7 @@ GL_HD void fwd_c2_load(uint64_t* v, const uint64_t* lds, int k, int w1) {
 // v[w0] = Z[64k + w1 + 8 w0], below 2^64 but possibly >= p: the values only feed products (the
 // external product's MAC, the NTT product's pointwise multiply); stored results are made
 // canonical by their kernels
-GL_HD void fwd_c2_compute(uint64_t* v) { dft8<24, 0xF>(v); }
+GL_HD void fwd_c2_compute(uint64_t* v) { dft8<24, 0x1F>(v); }
 
 // ---------------------------------------------------------------- inverse passes
 // IC2: inverse DFT-8 over w0 (C2 map); the twiddle omega^(-w1 r0) is applied by IC1.
@@ -342,7 +350,7 @@ GL_HD void inv_c1_compute(uint64_t (*v)[8], int t, const uint64_t* itw, const ui
   const int k = t & 31, h = (t >> 5) & 1, w = t >> 6;
 #pragma unroll
   for (int j = 0; j < NP; j++) {
-    idft8<0xF>(v[j]);  // the outputs feed the B untwist products
+    idft8<0x1F>(v[j]);  // the outputs feed the B untwist products
     const int r0 = c1_r0<NP>(j, w, h);
     uint64_t w[8];
 #pragma unroll

@@ -65,11 +65,14 @@ GL_HD int idx_u3(int k, int wb, int wq, int ra) { return 64 * k + ((16 * wb + 4 
 // (gl::sub2): x0 - x2 with x1 - x3, then the reduction of (x1 - x3) 2^S (S in (32, 64): a
 // 128-bit value lo + 2^64 hl + 2^96 hh, reduced as in gl::reduce128) with a0 - b0.
 // WRAP: bit i set makes the final sum x[i] (i = 0, 1) a gl::add_wrap (below 2^64, possibly
-// >= p), for outputs that only feed products or shift twists by a nonzero amount.
+// >= p), for outputs that only feed products or shift twists by a nonzero amount; bit 2 (with
+// bits 0, 1: every output may be >= p) also wraps a0, which reaches the second stage only as a
+// first operand (add_wrap and sub_fold take any 64-bit first operand with a canonical second).
 template <int S, int WRAP = 0>
 GL_HD void dft4(uint64_t* x) {
   static_assert(S > 32 && S < 64, "the paired reduction assumes a shift in (32, 64)");
-  const uint64_t a0 = gl::add(x[0], x[2]), b0 = gl::add(x[1], x[3]);
+  static_assert(!(WRAP & 4) || (WRAP & 3) == 3, "bit 2 needs every output non-canonical");
+  const uint64_t a0 = gl::add_w<WRAP & 4>(x[0], x[2]), b0 = gl::add(x[1], x[3]);
   uint64_t a1, d, t;
   gl::sub2(a1, d, x[0], x[1], x[2], x[3]);
   gl::sub2(t, x[2], d << S, a0, d >> (96 - S), b0);
@@ -167,7 +170,7 @@ GL_HD void fwd_a2_load(uint64_t* v, const uint64_t* lds, const Map& m) {
   for (int m0 = 0; m0 < 4; m0++) v[m0] = lds[ntt::idx_y1(m.W, m0, m.r())];
 }
 GL_HD void fwd_a2_compute(uint64_t* v, const uint64_t* tw4) {
-  dft4<48, 3>(v);  // the outputs feed the B twist products
+  dft4<48, 7>(v);  // the outputs feed the B twist products
   gl::mul4(v, v, tw4);
 }
 GL_HD void fwd_a2_store(const uint64_t* v, uint64_t* lds, const Map& m) {
@@ -181,7 +184,7 @@ GL_HD void fwd_c1_load(uint64_t* v, const uint64_t* lds, const Map& m) {
   for (int rc = 0; rc < 4; rc++) v[rc] = lds[ntt::idx_t(k, rp + 16 * rc)];
 }
 GL_HD void fwd_c1_compute(uint64_t* v, const uint64_t* c1w) {
-  dft4<48, 3>(v);  // the outputs feed the twist products
+  dft4<48, 7>(v);  // the outputs feed the twist products
   gl::mul4(v, v, c1w);
 }
 GL_HD void fwd_c1_store(const uint64_t* v, uint64_t* lds, const Map& m) {
@@ -220,7 +223,7 @@ GL_HD void fwd_c3_load(uint64_t* v, const uint64_t* lds, const Map& m) {
   for (int ra = 0; ra < 4; ra++) v[ra] = lds[idx_u3(k, wb, wq, ra)];
 }
 // outputs below 2^64 but possibly >= p: they only feed the MAC's products
-GL_HD void fwd_c3_compute(uint64_t* v) { dft4<48, 3>(v); }
+GL_HD void fwd_c3_compute(uint64_t* v) { dft4<48, 7>(v); }
 
 // ---------------------------------------------------------------- inverse passes
 GL_HD void inv_c3_compute(uint64_t* v) { idft4(v); }  // wa -> ra
@@ -237,7 +240,7 @@ GL_HD void inv_c2_load(uint64_t* v, const uint64_t* lds, const Map& m) {
 template <int RA>
 GL_HD void ic2(uint64_t* v) {  // 2^(-12 RA wb), then wb -> rb
   gl::mul_pow2c_n<-12 * RA, -24 * RA, -36 * RA>(v + 1);
-  idft4<3>(v);  // the outputs feed inverse C1's products
+  idft4<7>(v);  // the outputs feed inverse C1's products
 }
 GL_HD void inv_c2_compute(uint64_t* v, int rau) {
   switch (rau) {
@@ -260,7 +263,7 @@ GL_HD void inv_c1_load(uint64_t* v, const uint64_t* lds, const Map& m) {
 // 8^(-r' wq) (ic1w), wq -> rc, then the B untwist itw4[rc] = itw((r' + 16 rc), k).
 GL_HD void inv_c1_compute(uint64_t* v, const uint64_t* ic1w, const uint64_t* itw4) {
   gl::mul4(v, v, ic1w);
-  idft4<3>(v);  // the outputs feed the B untwist products
+  idft4<7>(v);  // the outputs feed the B untwist products
   gl::mul4(v, v, itw4);
 }
 GL_HD void inv_c1_store(const uint64_t* v, uint64_t* lds, const Map& m) {
