@@ -17,7 +17,21 @@
 #define GL_HD static inline
 #endif
 
+// Reduction tails (add_eps_prod) of the four-lane reductions and of the batched shift twists:
+// up to four lanes per asm block when a translation unit defines GL_GROUPED_TAILS to 1 before
+// including this header, one block per lane otherwise.  The choice only changes the device
+// instruction schedule; results are identical.
+#ifndef GL_GROUPED_TAILS
+#define GL_GROUPED_TAILS 0
+#endif
+
 namespace gl {
+
+#if defined(__HIP_DEVICE_COMPILE__) && GL_GROUPED_TAILS
+constexpr bool GROUPED_TAILS = true;
+#else
+constexpr bool GROUPED_TAILS = false;
+#endif
 
 constexpr uint64_t P = 0xFFFFFFFF00000001ull;
 constexpr uint64_t EPS = 0xFFFFFFFFull;  // 2^64 mod p
@@ -79,6 +93,115 @@ GL_HD uint64_t add_eps_prod(uint64_t t0, uint32_t hl) {
 #else
   return add(t0, ((uint64_t)hl << 32) - hl);
 #endif
+}
+
+// r_i = add_eps_prod(t0_i, hl_i) for N <= 4 independent lanes in one asm block, the lanes'
+// steps interleaved.  One block instead of N: the compiler puts a wait state after an asm block
+// whose results the next instruction touches, so N separate blocks in a row cost up to N of
+// them.  Each select mask is the SALU OR into its own register pair, which no VALU writes.  The
+// sums are formed in place over t0_i and the selects over hl_i (tied operands): no registers
+// beyond the inputs.
+template <int N>
+GL_HD void add_eps_prodn(uint64_t* r, const uint64_t* t0, const uint32_t* hl) {
+  static_assert(N >= 1 && N <= 4, "one to four lanes");
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (N == 1) {
+    r[0] = add_eps_prod(t0[0], hl[0]);
+  } else if constexpr (N == 2) {
+    uint64_t r0 = t0[0], r1 = t0[1];
+    uint32_t x0 = hl[0], x1 = hl[1];
+    uint64_t c0, m0, n0, c1, m1, n1;
+    asm("v_mad_u64_u32 %0, %4, %2, -1, %0\n\t"
+        "v_mad_u64_u32 %1, %5, %3, -1, %1\n\t"
+        "v_cmp_lt_u64_e64 %6, %10, %0\n\t"
+        "v_cmp_lt_u64_e64 %7, %10, %1\n\t"
+        "s_or_b64 %8, %6, %4\n\t"
+        "s_or_b64 %9, %7, %5\n\t"
+        "v_cndmask_b32_e64 %2, 0, -1, %8\n\t"
+        "v_cndmask_b32_e64 %3, 0, -1, %9\n\t"
+        "v_mad_u64_u32 %0, %4, %2, 1, %0\n\t"
+        "v_mad_u64_u32 %1, %5, %3, 1, %1"
+        : "+v"(r0), "+v"(r1), "+v"(x0), "+v"(x1),
+          "=&s"(c0), "=&s"(c1), "=&s"(m0), "=&s"(m1), "=&s"(n0), "=&s"(n1)
+        : "s"(P - 1)
+        : "scc");
+    r[0] = r0;
+    r[1] = r1;
+  } else if constexpr (N == 3) {
+    uint64_t r0 = t0[0], r1 = t0[1], r2 = t0[2];
+    uint32_t x0 = hl[0], x1 = hl[1], x2 = hl[2];
+    uint64_t c0, m0, n0, c1, m1, n1, c2, m2, n2;
+    asm("v_mad_u64_u32 %0, %6, %3, -1, %0\n\t"
+        "v_mad_u64_u32 %1, %7, %4, -1, %1\n\t"
+        "v_mad_u64_u32 %2, %8, %5, -1, %2\n\t"
+        "v_cmp_lt_u64_e64 %9, %15, %0\n\t"
+        "v_cmp_lt_u64_e64 %10, %15, %1\n\t"
+        "v_cmp_lt_u64_e64 %11, %15, %2\n\t"
+        "s_or_b64 %12, %9, %6\n\t"
+        "s_or_b64 %13, %10, %7\n\t"
+        "s_or_b64 %14, %11, %8\n\t"
+        "v_cndmask_b32_e64 %3, 0, -1, %12\n\t"
+        "v_cndmask_b32_e64 %4, 0, -1, %13\n\t"
+        "v_cndmask_b32_e64 %5, 0, -1, %14\n\t"
+        "v_mad_u64_u32 %0, %6, %3, 1, %0\n\t"
+        "v_mad_u64_u32 %1, %7, %4, 1, %1\n\t"
+        "v_mad_u64_u32 %2, %8, %5, 1, %2"
+        : "+v"(r0), "+v"(r1), "+v"(r2), "+v"(x0), "+v"(x1), "+v"(x2),
+          "=&s"(c0), "=&s"(c1), "=&s"(c2), "=&s"(m0), "=&s"(m1), "=&s"(m2),
+          "=&s"(n0), "=&s"(n1), "=&s"(n2)
+        : "s"(P - 1)
+        : "scc");
+    r[0] = r0;
+    r[1] = r1;
+    r[2] = r2;
+  } else {
+    uint64_t r0 = t0[0], r1 = t0[1], r2 = t0[2], r3 = t0[3];
+    uint32_t x0 = hl[0], x1 = hl[1], x2 = hl[2], x3 = hl[3];
+    uint64_t c0, m0, n0, c1, m1, n1, c2, m2, n2, c3, m3, n3;
+    asm("v_mad_u64_u32 %0, %8, %4, -1, %0\n\t"
+        "v_mad_u64_u32 %1, %9, %5, -1, %1\n\t"
+        "v_mad_u64_u32 %2, %10, %6, -1, %2\n\t"
+        "v_mad_u64_u32 %3, %11, %7, -1, %3\n\t"
+        "v_cmp_lt_u64_e64 %12, %20, %0\n\t"
+        "v_cmp_lt_u64_e64 %13, %20, %1\n\t"
+        "v_cmp_lt_u64_e64 %14, %20, %2\n\t"
+        "v_cmp_lt_u64_e64 %15, %20, %3\n\t"
+        "s_or_b64 %16, %12, %8\n\t"
+        "s_or_b64 %17, %13, %9\n\t"
+        "s_or_b64 %18, %14, %10\n\t"
+        "s_or_b64 %19, %15, %11\n\t"
+        "v_cndmask_b32_e64 %4, 0, -1, %16\n\t"
+        "v_cndmask_b32_e64 %5, 0, -1, %17\n\t"
+        "v_cndmask_b32_e64 %6, 0, -1, %18\n\t"
+        "v_cndmask_b32_e64 %7, 0, -1, %19\n\t"
+        "v_mad_u64_u32 %0, %8, %4, 1, %0\n\t"
+        "v_mad_u64_u32 %1, %9, %5, 1, %1\n\t"
+        "v_mad_u64_u32 %2, %10, %6, 1, %2\n\t"
+        "v_mad_u64_u32 %3, %11, %7, 1, %3"
+        : "+v"(r0), "+v"(r1), "+v"(r2), "+v"(r3), "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3),
+          "=&s"(c0), "=&s"(c1), "=&s"(c2), "=&s"(c3), "=&s"(m0), "=&s"(m1), "=&s"(m2), "=&s"(m3),
+          "=&s"(n0), "=&s"(n1), "=&s"(n2), "=&s"(n3)
+        : "s"(P - 1)
+        : "scc");
+    r[0] = r0;
+    r[1] = r1;
+    r[2] = r2;
+    r[3] = r3;
+  }
+#else
+  for (int i = 0; i < N; i++) r[i] = add_eps_prod(t0[i], hl[i]);
+#endif
+}
+
+// add_eps_prodn over any number of lanes, four per block (r may alias t0)
+template <int N>
+GL_HD void add_eps_prod_all(uint64_t* r, const uint64_t* t0, const uint32_t* hl) {
+  if constexpr (N > 4) {
+    add_eps_prodn<4>(r, t0, hl);
+    add_eps_prod_all<N - 4>(r + 4, t0 + 4, hl + 4);
+  } else {
+    add_eps_prodn<N>(r, t0, hl);
+  }
 }
 
 // a - b with the borrow folded back as 2^64 == EPS: (a - b) mod 2^64, minus EPS when the
@@ -354,8 +477,13 @@ GL_HD void reduce128x4(uint64_t* r, const uint64_t* lo, const uint64_t* hi) {
   uint64_t t[4];
   sub4(t[0], t[1], t[2], t[3], lo[0], lo[1], lo[2], lo[3], hi[0] >> 32, hi[1] >> 32, hi[2] >> 32,
        hi[3] >> 32);
+  const uint32_t hl[4] = {(uint32_t)hi[0], (uint32_t)hi[1], (uint32_t)hi[2], (uint32_t)hi[3]};
+  if constexpr (GROUPED_TAILS) {
+    add_eps_prodn<4>(r, t, hl);
+  } else {
 #pragma unroll
-  for (int i = 0; i < 4; i++) r[i] = add_eps_prod(t[i], (uint32_t)hi[i]);
+    for (int i = 0; i < 4; i++) r[i] = add_eps_prod(t[i], hl[i]);
+  }
 }
 
 // reduce160 over four lanes (c_i < 2^32 - 1), the four borrow folds in one sub4.
@@ -364,8 +492,13 @@ GL_HD void reduce160x4(uint64_t* r, const uint64_t* lo, const uint64_t* hi, cons
 #pragma unroll
   for (int i = 0; i < 4; i++) hh[i] = (hi[i] >> 32) | ((uint64_t)c[i] << 32);
   sub4(t[0], t[1], t[2], t[3], lo[0], lo[1], lo[2], lo[3], hh[0], hh[1], hh[2], hh[3]);
+  const uint32_t hl[4] = {(uint32_t)hi[0], (uint32_t)hi[1], (uint32_t)hi[2], (uint32_t)hi[3]};
+  if constexpr (GROUPED_TAILS) {
+    add_eps_prodn<4>(r, t, hl);
+  } else {
 #pragma unroll
-  for (int i = 0; i < 4; i++) r[i] = add_eps_prod(t[i], (uint32_t)hi[i]);
+    for (int i = 0; i < 4; i++) r[i] = add_eps_prod(t[i], hl[i]);
+  }
 }
 
 GL_HD void mul_full(uint64_t a, uint64_t b, uint64_t& lo, uint64_t& hi) {
@@ -469,7 +602,7 @@ GL_HD void mul_pow2c_n(uint64_t* v) {
     const int e = I[i].e;
     hl[i] = 0;
     if (I[i].kind == 1) {
-      v[i] = reduce96(x << e, x >> (64 - e));
+      if constexpr (!GROUPED_TAILS) v[i] = reduce96(x << e, x >> (64 - e));
     } else if (I[i].kind == 2) {
       a[j] = x << e;
       b[j] = x >> (96 - e);  // the 2^96 part of the 128-bit x << e
@@ -484,10 +617,29 @@ GL_HD void mul_pow2c_n(uint64_t* v) {
   }
   if constexpr (N1 > 0) subn<N1>(t, a, b);
   j = 0;
+  if constexpr (GROUPED_TAILS) {
+    // the reduction tails of kind-1 and kind-2 lanes together, up to four per asm block
+    constexpr int NE = ((pw2_of(S).kind == 1 || pw2_of(S).kind == 2 ? 1 : 0) + ... + 0);
+    uint64_t et[NE > 0 ? NE : 1];
+    uint32_t eh[NE > 0 ? NE : 1];
+    int q = 0;
 #pragma unroll
-  for (int i = 0; i < K; i++) {
-    if (I[i].kind == 2) v[i] = add_eps_prod(t[j++], hl[i]);
-    else if (I[i].kind == 3) v[i] = t[j++];
+    for (int i = 0; i < K; i++) {
+      if (I[i].kind == 1) et[q] = v[i] << I[i].e, eh[q++] = (uint32_t)(v[i] >> (64 - I[i].e));
+      else if (I[i].kind == 2) et[q] = t[j++], eh[q++] = hl[i];
+      else if (I[i].kind == 3) v[i] = t[j++];
+    }
+    if constexpr (NE > 0) add_eps_prod_all<NE>(et, et, eh);
+    q = 0;
+#pragma unroll
+    for (int i = 0; i < K; i++)
+      if (I[i].kind == 1 || I[i].kind == 2) v[i] = et[q++];
+  } else {
+#pragma unroll
+    for (int i = 0; i < K; i++) {
+      if (I[i].kind == 2) v[i] = add_eps_prod(t[j++], hl[i]);
+      else if (I[i].kind == 3) v[i] = t[j++];
+    }
   }
   if constexpr (N2 > 0) {
     uint64_t z[N2], nb[N2], r[N2];

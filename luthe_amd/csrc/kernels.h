@@ -96,6 +96,14 @@ void launch_blind_rotate_split(int pbs_level, bool hi, bool local, const uint64_
                                const BrArgs& a, hipStream_t s, uint64_t* xch, uint32_t* status);
 // Ciphertexts the split variant runs with one workgroup per CU (CUs / 2).
 int blind_rotate_split_slots();
+// The 512-thread blind rotation (latency.hip): one ciphertext per workgroup, the halves owning
+// one GLWE polynomial each; pair: two workgroups per CU (the level-2 throughput variant), else
+// one (the latency variant for small cohorts).  grid_override > 0 fixes the grid.
+void launch_blind_rotate_lat(int pbs_level, bool hi, bool pair, int grid_override, const uint64_t* bsk,
+                             const uint64_t* luts, const uint16_t* lwe2n, const PbsJob* jobs, int nct,
+                             const uint64_t* tab, const BrArgs& a, hipStream_t s);
+// Resident workgroups of the 512-thread kernel (CUs x workgroups per CU).
+int blind_rotate_lat_max_grid(int pbs_level, bool pair);
 // Resident workgroups of the throughput kernel (CUs x workgroups per CU).
 int blind_rotate_occupancy(int pbs_level, bool pair);
 // Ciphertexts the latency variant holds resident at once (its workgroups per CU x CUs).

@@ -3,17 +3,24 @@
 //  k_ntt_*        batched negacyclic NTT / INTT / NTT-multiply-INTT, N = 2048
 //  k_bsk_prepare  bootstrapping key -> NTT domain, blind-rotation layout
 //  (k_keyswitch, the batched LWE key switch, is in keyswitch.hip, its matrix-core form in
-//  ks_mfma.hip; the split blind rotation for the smallest cohorts in split.hip)
+//  ks_mfma.hip; the 512-thread latency and pair blind rotations in latency.hip; the split
+//  blind rotation for the smallest cohorts in split.hip)
 //  k_blind_rotate persistent blind rotation: one ciphertext per 256-thread workgroup,
 //                 accumulator in registers, NTT work buffer in LDS, BSK streamed from
 //                 HBM/L2 in a coalesced per-thread layout; sample extract fused
 //  k_alu, k_copy  DOp ALU and LD/ST on BLWE rows
+//
+// The reduction tails of the four-lane reductions and shift twists go up to four lanes per
+// asm block here (GL_GROUPED_TAILS): at four waves per SIMD the fewer wait states between
+// blocks outweigh the less flexible schedule.  The latency kernels, with fewer waves to hide
+// a longer dependency chain, keep one block per lane.
+#define GL_GROUPED_TAILS 1
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 
 #include "kernels.h"
-#include "ntt2048.h"
+#include "passes.h"
 #include "pbs_core.h"
 
 using namespace ntt;
@@ -25,78 +32,6 @@ constexpr int MAX_GRID_Y = 65535;  // launches split their per-op grid dimension
 // ------------------------------------------------------------------------------------
 // NTT kernels: NP polynomials per 256-thread workgroup.
 // ------------------------------------------------------------------------------------
-// tab: twiddle tables (ntt2048.h TABLE_WORDS layout).
-// PRE: the inputs are already pretwisted by psi^(4 m1) (blind-rotation digits, HI path).
-template <int NP, bool PRE = false>
-__device__ __forceinline__ void fwd_passes(uint64_t (*v)[8], uint64_t* lds, const uint64_t* tab, int t) {
-  const int r = t & 63, m0 = t >> 6, k = t >> 3, j = t & 7;
-  const int m0u = wave_uniform(m0);
-  const uint64_t* tw = tab + TW_OFF;
-  const uint64_t* c64 = tab + C64_OFF;
-#pragma unroll
-  for (int p = 0; p < NP; p++) fwd_a1_compute(v[p], m0u, PRE);
-#pragma unroll
-  for (int p = 0; p < NP; p++) fwd_a1_store(v[p], lds + p * N, r, m0);
-  __syncthreads();
-#pragma unroll
-  for (int p = 0; p < NP; p++) {
-    fwd_a2_load(v[p], lds + p * N, r, m0);
-    fwd_a2_compute(v[p], r, m0u, tw);
-  }
-  __syncthreads();
-#pragma unroll
-  for (int p = 0; p < NP; p++) fwd_a2_store(v[p], lds + p * N, r, m0);
-  __syncthreads();
-  fwd_c1_load<NP>(v, lds, t);
-  fwd_c1_compute<NP>(v, t, c64);
-  __syncthreads();
-  fwd_c1_store<NP>(v, lds, t);
-  __syncthreads();
-#pragma unroll
-  for (int p = 0; p < NP; p++) {
-    fwd_c2_load(v[p], lds + p * N, k, j);
-    fwd_c2_compute(v[p]);
-  }
-}
-
-// Entry: v in NTT domain (thread (k, w1) layout), LDS free for writing only after a
-// barrier that orders the previous readers.  Exit: v[p][m1] canonical coefficients.
-template <int NP>
-__device__ __forceinline__ void inv_passes(uint64_t (*v)[8], uint64_t* lds, const uint64_t* tab, int t) {
-  // Opaque copy of the thread id: the index values below are derived here rather than
-  // kept live across the caller's code (which would spill at 128 VGPRs).
-  asm volatile("" : "+v"(t));
-  const int r = t & 63, m0 = t >> 6, k = t >> 3, j = t & 7;
-  const int m0u = wave_uniform(m0);
-  const uint64_t* itw = tab + ITW_OFF;
-  const uint64_t* ic64 = tab + IC64_OFF;
-#pragma unroll
-  for (int p = 0; p < NP; p++) inv_c2_compute(v[p]);
-  __syncthreads();
-#pragma unroll
-  for (int p = 0; p < NP; p++) inv_c2_store(v[p], lds + p * N, k, j);
-  __syncthreads();
-  inv_c1_load<NP>(v, lds, t);
-  inv_c1_compute<NP>(v, t, itw, ic64);
-  __syncthreads();
-  inv_c1_store<NP>(v, lds, t);
-  __syncthreads();
-#pragma unroll
-  for (int p = 0; p < NP; p++) {
-    inv_a2_load(v[p], lds + p * N, r, m0);
-    inv_a2_compute(v[p], m0u);
-  }
-  __syncthreads();
-#pragma unroll
-  for (int p = 0; p < NP; p++) inv_a2_store(v[p], lds + p * N, r, m0);
-  __syncthreads();
-#pragma unroll
-  for (int p = 0; p < NP; p++) {
-    inv_a1_load(v[p], lds + p * N, r, m0);
-    inv_a1_compute(v[p]);
-  }
-}
-
 // mode 0: forward (out NTT domain), 1: inverse (in NTT domain), 2: polymul
 template <int NP, int MODE>
 __global__ __launch_bounds__(256) void k_ntt(const uint64_t* __restrict__ a, const uint64_t* __restrict__ bn,
@@ -262,124 +197,6 @@ __global__ __launch_bounds__(256, (L == 1 ? 4 : 2)) void k_blind_rotate(const ui
           o[g * N + m] = pbs::extract_coef(lds, g, m, b, offs);
         }
       if (t == 0) o[(K1 - 1) * N] = pbs::extract_coef(lds, K1 - 1, 0, b, offs);
-    }
-  }
-}
-
-// ------------------------------------------------------------------------------------
-// Latency variant of the blind rotation for small cohorts: one ciphertext per 512-thread
-// workgroup.  Half h (256 threads, same per-thread mapping as above) owns GLWE
-// polynomial h: its rotation and decomposition, the forward NTTs of its L digit
-// polynomials, output column j = h of the external product and its inverse NTT.  The
-// halves exchange NTT-domain digits through LDS before the product.  Arithmetic is the
-// same canonical Goldilocks arithmetic, so results equal k_blind_rotate's bit for bit.
-// The key words are read in groups of four outputs through a buffer resource and the
-// products summed as 128-bit values with one reduction per output, as in k_blind_rotate.
-// TP (the throughput pair variant): two workgroups per CU (at most 128 VGPRs, so four
-// waves per SIMD); without TP one workgroup per CU (the latency variant for small cohorts).
-// ------------------------------------------------------------------------------------
-template <int K1, int L, bool HI, bool TP>
-__global__ __launch_bounds__(512, TP ? 2 : 1) void k_blind_rotate_lat(const uint64_t* __restrict__ bsk,
-                                                             const uint64_t* __restrict__ luts,
-                                                             const uint16_t* __restrict__ lwe2n,
-                                                             const PbsJob* __restrict__ jobs, int nct,
-                                                             const uint64_t* __restrict__ tab, BrArgs a) {
-  static_assert(K1 == 2, "one half per GLWE polynomial");
-  constexpr int NPF = K1 * L;
-  __shared__ uint64_t lds[NPF * N];  // acc (K1 polys) / digit NTTs (K1*L) / exchange (K1*L*8*T)
-  const int t2 = threadIdx.x;
-  const int h = t2 >> 8, t = t2 & 255;
-  const int r = t & 63, m0 = t >> 6;
-  const size_t blen = (size_t)(K1 - 1) * N + 1;
-  const size_t bsk_stride = (size_t)8 * NPF * K1 * T;
-  const __amdgpu_buffer_rsrc_t bsk_rsrc = __builtin_amdgcn_make_buffer_rsrc(
-      (void*)bsk, (short)0, (int)((size_t)a.n_lwe * bsk_stride * 8), 0x00020000);
-  for (int c = blockIdx.x; c < nct; c += gridDim.x) {
-    const PbsJob job = jobs[c];
-    if (job.out_addr == 0) continue;  // an empty slot of a batch list (uniform per workgroup)
-    const uint16_t* a2n = lwe2n + (size_t)c * a.lwe_stride;
-    const uint64_t* lut = luts + (size_t)job.gid * K1 * N;
-    uint64_t acc[8];
-#pragma unroll
-    for (int m1 = 0; m1 < 8; m1++) acc[m1] = lut[h * N + 64 * (m0 + 4 * m1) + r];
-    uint32_t rot_next = a2n[0];
-    for (int i = 0; i < a.n_lwe; i++) {
-      const uint32_t rot = rot_next;
-      rot_next = a2n[i + 1];  // i + 1 <= n_lwe: the body slot, always present
-      int tt = t;
-      asm volatile("" : "+v"(tt));
-      const int rr = tt & 63, mm0 = tt >> 6;
-      __syncthreads();
-#pragma unroll
-      for (int m1 = 0; m1 < 8; m1++) lds[h * N + 64 * (mm0 + 4 * m1) + rr] = acc[m1];
-      __syncthreads();
-      uint64_t v[L][8];
-      pbs::rotate_decompose<L, HI>(lds + h * N, acc, tt, rot, a.base_log, v);
-      __syncthreads();
-      fwd_passes<L, HI>(v, lds + h * L * N, tab, tt);
-      // exchange the digit NTTs: row gl = g * L + l at [(gl * 8 + w0) * T + t]
-      __syncthreads();
-#pragma unroll
-      for (int l = 0; l < L; l++)
-#pragma unroll
-        for (int w0 = 0; w0 < 8; w0++) lds[((h * L + l) * 8 + w0) * T + tt] = v[l][w0];
-      __syncthreads();
-      uint64_t z[1][8];
-      // outputs w0 in groups of four: the group's key words first, then the 128-bit sums
-      const int hu = wave_uniform(h);
-      const int slice_off = (int)((size_t)i * bsk_stride * 8);
-      const int lane_off = tt * 8;
-#pragma unroll
-      for (int q = 0; q < 8; q += 4) {
-        uint64_t kw[4][NPF];
-#pragma unroll
-        for (int u = 0; u < 4; u++)
-#pragma unroll
-          for (int gl_ = 0; gl_ < NPF; gl_++) {
-            const int e = ((q + u) * NPF + gl_) * K1 + hu;
-            const auto w = __builtin_amdgcn_raw_buffer_load_b64(bsk_rsrc, lane_off, slice_off + e * T * 8, 0);
-            kw[u][gl_] = ((uint64_t)w[1] << 32) | w[0];
-          }
-        uint64_t lo[4], hi[4], o[4];
-        uint32_t cc[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-          cc[u] = 0;
-          gl::mul_full(lds[(q + u) * T + tt], kw[u][0], lo[u], hi[u]);
-        }
-#pragma unroll
-        for (int gl_ = 1; gl_ < NPF; gl_++) {
-          uint64_t plo[4], phi[4];
-#pragma unroll
-          for (int u = 0; u < 4; u++) gl::mul_full(lds[(gl_ * 8 + q + u) * T + tt], kw[u][gl_], plo[u], phi[u]);
-          gl::acc128x4(lo, hi, cc, plo, phi);
-        }
-        gl::reduce160x4(o, lo, hi, cc);
-#pragma unroll
-        for (int u = 0; u < 4; u++) z[0][q + u] = o[u];
-      }
-      inv_passes<1>(z, lds + h * N, tab, tt);
-#pragma unroll
-      for (int m1 = 0; m1 < 8; m1++) acc[m1] += gl::modswitch_p_to_q(z[0][m1]);
-    }
-    // sample extract
-    __syncthreads();
-#pragma unroll
-    for (int m1 = 0; m1 < 8; m1++) lds[h * N + 64 * (m0 + 4 * m1) + r] = acc[m1];
-    __syncthreads();
-    const uint32_t b = a2n[a.n_lwe];
-    const int lut_nb = 1 << job.log_lut_nb;
-    for (int lu = 0; lu < lut_nb; lu++) {
-      const uint32_t offs = (uint32_t)(lu * (N >> job.log_lut_nb));
-      uint64_t* o = reinterpret_cast<uint64_t*>(job.out_addr) + (size_t)lu * blen;
-#pragma unroll
-      for (int g = 0; g < K1 - 1; g++)
-#pragma unroll
-        for (int e = 0; e < N / (2 * T); e++) {
-          int m = t2 + 2 * T * e;
-          o[g * N + m] = pbs::extract_coef(lds, g, m, b, offs);
-        }
-      if (t2 == 0) o[(K1 - 1) * N] = pbs::extract_coef(lds, K1 - 1, 0, b, offs);
     }
   }
 }
@@ -579,32 +396,14 @@ static int br_max_grid() {
   return cached;
 }
 
-template <int K1, int L, bool TP>
-static int br_lat_max_grid() {
-  static const int cached = [] {
-    int dev = 0, cus = 0, per = 0;
-    (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_blind_rotate_lat<K1, L, true, TP>, 512, 0);
-    return (cus > 0 ? cus : 256) * (per > 0 ? per : 1);
-  }();
-  return cached;
-}
-
 template <int L, bool HI>
 static void br_launch(const uint64_t* bsk, const uint64_t* luts, const uint16_t* lwe2n, const PbsJob* jobs, int nct,
                       const uint64_t* tab, const BrArgs& a, int grid_override, hipStream_t s,
                       bool latency, bool pair, uint64_t* xch, uint32_t* status, bool split_local) {
   if (xch) {
     launch_blind_rotate_split(L, HI, split_local, bsk, luts, lwe2n, jobs, nct, tab, a, s, xch, status);
-  } else if (latency) {
-    int g = br_lat_max_grid<2, L, false>();
-    g = g < nct ? g : nct;
-    k_blind_rotate_lat<2, L, HI, false><<<dim3(g), 512, 0, s>>>(bsk, luts, lwe2n, jobs, nct, tab, a);
-  } else if (pair) {
-    int g = grid_override > 0 ? grid_override : br_lat_max_grid<2, L, true>();
-    g = g < nct ? g : nct;
-    k_blind_rotate_lat<2, L, HI, true><<<dim3(g), 512, 0, s>>>(bsk, luts, lwe2n, jobs, nct, tab, a);
+  } else if (latency || pair) {
+    launch_blind_rotate_lat(L, HI, !latency, latency ? 0 : grid_override, bsk, luts, lwe2n, jobs, nct, tab, a, s);
   } else {
     int g = grid_override > 0 ? grid_override : br_max_grid<2, L>();
     g = g < nct ? g : nct;
@@ -638,13 +437,11 @@ hipError_t launch_blind_rotate(const uint64_t* bsk, const uint64_t* luts, const 
 }
 
 int blind_rotate_occupancy(int pbs_level, bool pair) {
-  if (pair) return pbs_level == 1 ? br_lat_max_grid<2, 1, true>() : br_lat_max_grid<2, 2, true>();
+  if (pair) return blind_rotate_lat_max_grid(pbs_level, true);
   return pbs_level == 1 ? br_max_grid<2, 1>() : br_max_grid<2, 2>();
 }
 
-int blind_rotate_latency_slots(int pbs_level) {
-  return pbs_level == 1 ? br_lat_max_grid<2, 1, false>() : br_lat_max_grid<2, 2, false>();
-}
+int blind_rotate_latency_slots(int pbs_level) { return blind_rotate_lat_max_grid(pbs_level, false); }
 
 hipError_t launch_alu(uint64_t* base, const AluOp* ops, int nops, int blen, uint32_t useful_bits, hipStream_t s) {
   if (useful_bits < 1 || useful_bits > 11) return hipErrorInvalidValue;

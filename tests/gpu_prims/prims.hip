@@ -1,14 +1,17 @@
 // Test library: runs the device builds of the Goldilocks primitives (gl64.h) and the
 // high-word decomposition on caller-given operands, so that the device code paths
 // (inline-asm carry chains included) are checked against exact integer arithmetic at edge
-// values.  Test infrastructure only.
+// values.  Test infrastructure only.  The four-lane reductions and batched shifts below run
+// with grouped reduction tails (several lanes per asm block), as in the throughput kernels;
+// the single-lane forms (o[4] - o[6]) cover the per-lane blocks.
+#define GL_GROUPED_TAILS 1
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "../../luthe_amd/csrc/pbs_core.h"
 
 namespace {
-constexpr int NOUT = 58;  // results per operand triple, in the order listed in prims_run
+constexpr int NOUT = 67;  // results per operand triple, in the order listed in prims_run
 
 __global__ void k_prims(const uint64_t* a, const uint64_t* b, const uint32_t* c, int n, uint64_t* out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -66,6 +69,12 @@ __global__ void k_prims(const uint64_t* a, const uint64_t* b, const uint32_t* c,
   uint64_t tv[8] = {xc, yc, xc, yc, gl::canonical(x ^ y), xc, yc, xc};
   gl::mul_pow2c_n<24, 48, 72, 120, 168, 0, 96, 36>(tv);
   for (int i = 0; i < 8; i++) o[50 + i] = tv[i];
+  // interleaved reduction tails t0 + hl * EPS over two, three and four lanes
+  const uint64_t et[4] = {x, y, x ^ y, ~x};
+  const uint32_t eh[4] = {z, z >> 1, (uint32_t)y, 0xFFFFFFFFu};
+  gl::add_eps_prodn<2>(o + 58, et, eh);
+  gl::add_eps_prodn<3>(o + 60, et + 1, eh + 1);
+  gl::add_eps_prodn<4>(o + 63, et, eh);
 }
 }  // namespace
 

@@ -40,7 +40,7 @@ def test_device_primitives_exact(prims, oracle):
     a = np.array([p[0] for p in pairs], np.uint64)
     b = np.array([p[1] for p in pairs], np.uint64)
     c = np.array([zs[i % len(zs)] for i in range(n)], np.uint32)
-    out = np.zeros((n, 58), np.uint64)
+    out = np.zeros((n, 67), np.uint64)
     print(f"case: {n} operand pairs through the device primitives", flush=True)
     assert prims.prims_run(a.ctypes.data, b.ctypes.data, c.ctypes.data, n, out.ctypes.data) == 0
     for i, (x, y) in enumerate(pairs):
@@ -72,5 +72,8 @@ def test_device_primitives_exact(prims, oracle):
             exp += [acc & M64, (acc >> 64) & M64, acc >> 128]
         exp += [v * pow(2, s, P) % P for v, s in zip((xc, yc, xc, yc, xy % P, xc, yc, xc),
                                                     (24, 48, 72, 120, 168, 0, 96, 36))]
+        et, eh = (x, y, xy, nx), (z, z >> 1, y & EPS, EPS)
+        tails = [(t + h * EPS) % P for t, h in zip(et, eh)]
+        exp += tails[:2] + tails[1:4] + tails
         got = [int(v) for v in out[i]]
-        assert got == exp, (hex(x), hex(y), z, [k for k in range(58) if got[k] != exp[k]])
+        assert got == exp, (hex(x), hex(y), z, [k for k in range(67) if got[k] != exp[k]])
