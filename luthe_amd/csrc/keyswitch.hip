@@ -95,10 +95,8 @@ __global__ __launch_bounds__(256) void k_keyswitch(const PbsJob* __restrict__ jo
   if (PART) {
     if (j < n1) {
 #pragma unroll
-      for (int c = 0; c < KS_CT; c++) {
-        if (c0 + c >= nct) break;
-        part[((size_t)blockIdx.z * nct + c0 + c) * n1 + j] = acc[c];
-      }
+      for (int c = 0; c < KS_CT; c++)
+        if (c0 + c < nct) part[((size_t)blockIdx.z * nct + c0 + c) * n1 + j] = acc[c];
     }
     return;
   }
